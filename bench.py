@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark: 7-point FP32 diffusion on a 512^3 grid on one MI355X.
+
+Workload (the diffusion benchmark's Physis program, reference
+examples/diffusion-benchmark/diffusion3d_physis.c:29-76, at 512^3): seeded
+uniform [0,1) field, cc=0.4 and the six neighbour coefficients 0.1,
+clamped (zero-flux) boundaries; one step = one PSStencilMap application
+over the whole grid (one full-grid 7-point update, g1->g2 or g2->g1).
+Grids are device resident before the timed region; the timed region runs
+exactly --steps steps.  The periodic form is timed too and reported beside.
+
+value     = GLUPS = 512^3 * steps / seconds / 1e9 (clamped form)
+roofline  = algorithmic HBM bytes (8 B per lattice update: one FP32 read,
+            one FP32 write; diffusion3d.h:97-100) / average kernel duration
+            from HIP events on the runtime stream, against 8 TB/s
+cpu_baseline = the reference's own single-threaded CPU baseline
+            (examples/diffusion-benchmark/{main,baseline,diffusion3d}.cc built
+            -O3 into oracle/_ref) on 20 steps of the same 512^3 update.
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GLUPS + % HBM roofline, 512³ 7-pt FP32; served-mix p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_LUP = 8.0
+COEF = [0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.4]
+
+
+def time_steps(rt, g1, g2, boundary, steps, variant):
+    timer = rt.Timer()
+    rt.synchronize()
+    t0 = time.perf_counter()
+    timer.begin()
+    rt.diffusion7_run(g1, g2, COEF, boundary, steps, variant=variant)
+    timer.end()
+    rt.synchronize()
+    t1 = time.perf_counter()
+    return t1 - t0, timer.elapsed_ms() / 1e3
+
+
+def cpu_baseline(n):
+    """The reference's CPU baseline on a bounded sample (20 steps at n^3)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "diffusion_baseline.exe")
+    count = 20
+    if os.path.exists(exe):
+        try:
+            out = subprocess.run([exe, "--size", str(n), "--count", str(count)], capture_output=True, text=True,
+                                 timeout=240, check=True).stdout
+            m = re.search(r"Elapsed time\s*:\s*([0-9.]+)", out)
+            if m:
+                sec = float(m.group(1))
+                return {"value": n ** 3 * count / sec / 1e9, "unit": "GLUPS", "cores": 1, "kind": "reference",
+                        "sample": "%d steps of %d^3 FP32 7-pt clamped diffusion, reference "
+                                  "examples/diffusion-benchmark baseline built -O3, 1 thread" % (count, n)}
+        except (subprocess.SubprocessError, OSError) as e:
+            print("cpu baseline (reference) failed: %s" % e, file=sys.stderr)
+    # Fallback: the oracle's restatement of the ref target.
+    import numpy as np
+    from oracle import oracle as O
+    steps = 4
+    field = O.uniform_field((n, n, n), seed=1)
+    t0 = time.perf_counter()
+    O.diffusion7(field, COEF, periodic=False, steps=steps)
+    sec = time.perf_counter() - t0
+    del field, np
+    return {"value": n ** 3 * steps / sec / 1e9, "unit": "GLUPS", "cores": 1, "kind": "port",
+            "sample": "%d steps of %d^3 FP32 7-pt clamped diffusion, ref-target restatement, 1 thread" % (steps, n)}
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "diffusion7_512_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--variant", type=int, default=0, help="kernel variant id (0 = default)")
+    ap.add_argument("--sweep", action="store_true", help="also time every kernel variant (stderr)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs the stencil path on a single MI355X "
+              "(no multi-GPU decomposition); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from physis_amd import runtime as rt
+    rt.init(["bench"])
+    n = args.size
+    print("device: %s" % rt.device_name(), file=sys.stderr, flush=True)
+    g1 = rt.Grid((n, n, n))
+    g2 = rt.Grid((n, n, n))
+    g1.fill_uniform(1)
+
+    if args.sweep:
+        for v, name in rt.diffusion7_variants().items():
+            for boundary in (0, 1):
+                rt.diffusion7_run(g1, g2, COEF, boundary, 10, variant=v)
+                wall, dev = time_steps(rt, g1, g2, boundary, 200, v)
+                print("variant %2d %-28s %s: %.1f GLUPS  %.1f GB/s  (%.1f us/step)" % (
+                    v, name, "periodic" if boundary else "clamped ", n ** 3 * 200 / dev / 1e9,
+                    n ** 3 * 200 * BYTES_PER_LUP / dev / 1e9, dev / 200 * 1e6), file=sys.stderr, flush=True)
+        g1.fill_uniform(1)
+
+    results = {}
+    for boundary in (0, 1):
+        if args.warmup:
+            rt.diffusion7_run(g1, g2, COEF, boundary, args.warmup, variant=args.variant)
+        wall, dev = time_steps(rt, g1, g2, boundary, args.steps, args.variant)
+        results[boundary] = (wall, dev)
+        print("%s: %.1f GLUPS wall, kernel avg %.2f us" % (
+            "periodic" if boundary else "clamped", n ** 3 * args.steps / wall / 1e9, dev / args.steps * 1e6),
+            file=sys.stderr, flush=True)
+    g1.free()
+    g2.free()
+
+    wall, dev = results[0]
+    glups = n ** 3 * args.steps / wall / 1e9
+    achieved = BYTES_PER_LUP * n ** 3 / (dev / args.steps) / 1e9
+    traffic = load_traffic()
+    cpu = None if args.no_cpu else cpu_baseline(n)
+    line = {
+        "metric": METRIC,
+        "value": round(glups, 2),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded uniform [0,1) field, SplitMix64)",
+        "config": {"workload": "7-point diffusion, FP32 %d^3, clamped boundaries, cc=0.4, c_nb=0.1" % n,
+                   "grid": [n, n, n], "boundary": "clamped", "steps_timed": args.steps,
+                   "variant": args.variant,
+                   "periodic_glups": round(n ** 3 * args.steps / results[1][0] / 1e9, 2),
+                   "hbm_roofline_pct": round(100 * achieved / HBM_PEAK_GBS, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,74 @@
+// storage.h -- how grid elements are held in HBM on the MI355X target.
+//
+// A grid of a primitive type T is stored as T.  FP32 grids may instead be
+// stored as bfloat16 or IEEE half: loads widen to FP32 and stores round to
+// nearest-even, so the stencil arithmetic is FP32 either way and only the
+// bytes moved per update change.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace psk {
+
+// Native storage: the stored type is the value type.
+template <class T>
+struct Native {
+  using value_t = T;
+  using stored_t = T;
+  static constexpr int kBytes = sizeof(T);
+  __device__ __forceinline__ static T load(const stored_t *p) { return *p; }
+  __device__ __forceinline__ static void store(stored_t *p, T v) {
+    __builtin_nontemporal_store(v, p);
+  }
+  __device__ __forceinline__ static void store_cached(stored_t *p, T v) { *p = v; }
+  __device__ __forceinline__ static T widen(stored_t s) { return s; }
+  __device__ __forceinline__ static stored_t narrow(T v) { return v; }
+};
+
+// bfloat16 storage of an FP32 value: the upper 16 bits, rounded to nearest
+// even; NaN stays NaN.
+struct BF16 {
+  using value_t = float;
+  using stored_t = uint16_t;
+  static constexpr int kBytes = 2;
+  __device__ __forceinline__ static float widen(uint16_t s) {
+    return __uint_as_float(((uint32_t)s) << 16);
+  }
+  __device__ __forceinline__ static uint16_t narrow(float v) {
+    uint32_t u = __float_as_uint(v);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+  }
+  __device__ __forceinline__ static float load(const stored_t *p) { return widen(*p); }
+  __device__ __forceinline__ static void store(stored_t *p, float v) {
+    __builtin_nontemporal_store(narrow(v), p);
+  }
+  __device__ __forceinline__ static void store_cached(stored_t *p, float v) { *p = narrow(v); }
+};
+
+// IEEE binary16 storage of an FP32 value (hardware conversion, nearest even).
+struct FP16 {
+  using value_t = float;
+  using stored_t = uint16_t;
+  static constexpr int kBytes = 2;
+  __device__ __forceinline__ static float widen(uint16_t s) {
+    _Float16 h;
+    __builtin_memcpy(&h, &s, 2);
+    return (float)h;
+  }
+  __device__ __forceinline__ static uint16_t narrow(float v) {
+    _Float16 h = (_Float16)v;
+    uint16_t s;
+    __builtin_memcpy(&s, &h, 2);
+    return s;
+  }
+  __device__ __forceinline__ static float load(const stored_t *p) { return widen(*p); }
+  __device__ __forceinline__ static void store(stored_t *p, float v) {
+    __builtin_nontemporal_store(narrow(v), p);
+  }
+  __device__ __forceinline__ static void store_cached(stored_t *p, float v) { *p = narrow(v); }
+};
+
+}  // namespace psk

@@ -1,0 +1,109 @@
+/*
+ * physis_hip_ext.h -- additive C ABI of libphysis_rt_hip.
+ *
+ * Nothing here replaces a reference entry point; these calls extend the
+ * runtime for callers that need more than the void/abort ABI of
+ * physis_hip.h: status codes instead of process exit, 16-bit grid storage,
+ * FP64 reduction results, device-side timing, and the built-in stencil
+ * programs (the code physisc-hip emits for the diffusion and Himeno
+ * benchmarks, compiled into the library).
+ *
+ * All calls are ordered on the runtime stream (__PSGetStream()).
+ */
+#ifndef PHYSIS_PHYSIS_HIP_EXT_H_
+#define PHYSIS_PHYSIS_HIP_EXT_H_
+
+#include "physis/physis_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes of the non-aborting calls. */
+enum PSHipStatus {
+  PS_HIP_OK = 0,
+  PS_HIP_ERR_INVALID = 1,   /* bad argument (shape, op, storage, domain) */
+  PS_HIP_ERR_NOMEM = 2,     /* device allocation failed or budget exceeded */
+  PS_HIP_ERR_DEVICE = 3,    /* a HIP call failed */
+  PS_HIP_ERR_UNSUPPORTED = 4
+};
+
+/* How a grid's elements are held in HBM.  16-bit storage applies to FP32
+ * grids only: host copies and all arithmetic stay FP32. */
+enum PSHipStorage {
+  PS_HIP_STORAGE_NATIVE = 0,
+  PS_HIP_STORAGE_BF16 = 1,
+  PS_HIP_STORAGE_FP16 = 2
+};
+
+/* __PSGridNew with the grid attribute of PSGrid3DFloatNew(nx, ny, nz, attr):
+ * PS_GRID_STORAGE_BF16 / PS_GRID_STORAGE_FP16 select 16-bit storage. */
+extern __PSGrid *__PSGridNewAttr(int elm_size, int num_dims, PSVectorInt dim,
+                                 int attr);
+/* Non-aborting grid creation; *out is NULL on failure. */
+extern int PSHipGridNew(__PSGrid **out, int elm_size, int num_dims,
+                        const int *dim, int storage);
+extern int PSHipGridFree(__PSGrid *g);
+extern int PSHipGridCopyin(__PSGrid *g, const void *src);
+extern int PSHipGridCopyout(__PSGrid *g, void *dst);
+/* Asynchronous device-to-device copy of a whole grid (same shape/storage). */
+extern int PSHipGridCopyDevice(__PSGrid *dst, const __PSGrid *src);
+extern int PSHipGridStorage(const __PSGrid *g);
+/* Bytes per element as stored in HBM. */
+extern int PSHipGridStoredElmSize(const __PSGrid *g);
+/* Raw device pointer (same as g->p0). */
+extern void *PSHipGridDevicePtr(const __PSGrid *g);
+
+/* Fills a grid on the device with the seeded uniform [lo, hi) field
+ * lo + (hi - lo) * u_i, u_i = (splitmix64(seed + (i+1)*0x9E3779B97F4A7C15) >> 40) * 2^-24,
+ * i the linear (x fastest) index.  Asynchronous. */
+extern int PSHipGridFillUniform(__PSGrid *g, uint64_t seed, float lo, float hi);
+
+/* Reduction with an FP64 result.  SUM and PROD accumulate in FP64 in a
+ * fixed order (bitwise reproducible); MAX and MIN are exact. */
+extern int PSHipReduceGrid(double *out, enum PSReduceOp op, __PSGrid *g);
+
+/* Blocks until the runtime stream is idle; returns a PSHipStatus. */
+extern int PSHipSynchronize(void);
+/* Device bytes held by live grids. */
+extern int64_t PSHipGridBytesInUse(void);
+extern int PSHipDeviceName(char *buf, int len);
+/* Free and total device memory in bytes. */
+extern int PSHipMemInfo(int64_t *free_bytes, int64_t *total_bytes);
+
+/* Event timer on the runtime stream: Begin records a start event, End a stop
+ * event; Elapsed waits and returns milliseconds between them. */
+extern int PSHipTimerBegin(void);
+extern int PSHipTimerEnd(void);
+extern int PSHipTimerElapsed(float *ms);
+
+/* ---- built-in stencil programs -------------------------------------------
+ * 7-point diffusion (reference examples/diffusion-benchmark/diffusion3d_physis.c:29-76):
+ *   out(x,y,z) = cc*c + cw*w + ce*e + cs*s + cn*n + cb*b + ct*t
+ * with the neighbour taken from the same point at a clamped face
+ * (boundary = PS_HIP_CLAMPED) or from the wrapped point
+ * (boundary = PS_HIP_PERIODIC).  coef = {ce, cw, cn, cs, ct, cb, cc}.
+ * Runs `steps` steps alternating g1->g2 and g2->g1 (PSStencilRun with two
+ * maps and count steps/2; an odd count ends with one g1->g2 map).
+ * variant: 0 = auto-tuned default, otherwise a kernel id from
+ * PSHipDiffusion7Variants. */
+enum PSHipBoundary { PS_HIP_CLAMPED = 0, PS_HIP_PERIODIC = 1 };
+extern int PSHipDiffusion7Run(__PSGrid *g1, __PSGrid *g2,
+                              const __PSDomain *dom, const float *coef,
+                              int boundary, int steps, int variant);
+/* Number of kernel variants and a short name for each (1-based ids). */
+extern int PSHipDiffusion7Variants(void);
+extern const char *PSHipDiffusion7VariantName(int variant);
+
+/* Himeno 19-point Jacobi sweep (reference examples/himeno/himenobmtxpa_physis.c:331-361).
+ * grids[] = {a0,a1,a2,a3,b0,b1,b2,c0,c1,c2,bnd,wrk1}; p0/p1 ping-pong.
+ * Runs `sweeps` sweeps over the interior [1,n-1) per axis; gosa[i] receives
+ * the FP64 sum of ss^2 of sweep i when gosa is not NULL. */
+extern int PSHipHimenoRun(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids,
+                          float omega, int sweeps, double *gosa, int variant);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHYSIS_PHYSIS_HIP_EXT_H_ */

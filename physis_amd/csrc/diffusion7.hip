@@ -1,0 +1,219 @@
+// diffusion7.hip -- the 7-point diffusion program, as physisc-hip lowers
+// reference examples/diffusion-benchmark/diffusion3d_physis.c:29-76
+// (clamped form) and tests/system_tests/test_cases/test_7-pt-periodic.c
+// (periodic form, PSGridGetPeriodic) with coefficients, plus hand-tuned
+// kernels for the same update.  Every variant evaluates
+//     cc*c + cw*w + ce*e + cs*s + cn*n + cb*b + ct*t
+// left to right in FP32 with no contraction into FMA, exactly as the
+// sequential ref target does, so results are bitwise equal to it.
+#include "runtime_hip.h"
+#include "physis/hip/skeleton.h"
+#include "physis/hip/storage.h"
+
+#include <cstdio>
+
+namespace physis_hip {
+namespace {
+
+struct Coef7 {
+  float ce, cw, cn, cs, ct, cb, cc;
+};
+
+// ---- variant family 1: the generic skeleton with an emitted body ---------
+// Plan for map(kernel, dom, g_in, g_out, ce, cw, cn, cs, ct, cb, cc).
+template <class S, bool kPeriodic>
+struct DiffusionPlan {
+  using SV = float;
+  static constexpr int NS = 1, RX = 1, RY = 1, RZ = 1;
+  static constexpr bool kWrap = kPeriodic;
+  static constexpr bool kReduce = false;
+  struct Args {
+    const typename S::stored_t *g1;
+    typename S::stored_t *g2;
+    int dim[3];
+    Coef7 k;
+  };
+  __device__ static void stage_dims(const Args &a, int &nx, int &ny, int &nz) {
+    nx = a.dim[0];
+    ny = a.dim[1];
+    nz = a.dim[2];
+  }
+  __device__ static SV stage_load(const Args &a, int, int64_t off) { return S::load(a.g1 + off); }
+  template <class C>
+  __device__ static void point(const C &ctx, const Args &a, const int x, const int y, const int z) {
+    const int nx = a.dim[0], ny = a.dim[1], nz = a.dim[2];
+    float c, w, e, n, s, b, t;
+    c = ctx.at(0, 0, 0, 0);
+    if (kPeriodic) {
+      w = ctx.at(0, -1, 0, 0);
+      e = ctx.at(0, 1, 0, 0);
+      n = ctx.at(0, 0, -1, 0);
+      s = ctx.at(0, 0, 1, 0);
+      b = ctx.at(0, 0, 0, -1);
+      t = ctx.at(0, 0, 0, 1);
+    } else {
+      if (x == 0) w = ctx.at(0, 0, 0, 0); else w = ctx.at(0, -1, 0, 0);
+      if (x == nx - 1) e = ctx.at(0, 0, 0, 0); else e = ctx.at(0, 1, 0, 0);
+      if (y == 0) n = ctx.at(0, 0, 0, 0); else n = ctx.at(0, 0, -1, 0);
+      if (y == ny - 1) s = ctx.at(0, 0, 0, 0); else s = ctx.at(0, 0, 1, 0);
+      if (z == 0) b = ctx.at(0, 0, 0, 0); else b = ctx.at(0, 0, 0, -1);
+      if (z == nz - 1) t = ctx.at(0, 0, 0, 0); else t = ctx.at(0, 0, 0, 1);
+    }
+    const Coef7 &k = a.k;
+    const float v = k.cc * c + k.cw * w + k.ce * e + k.cs * s + k.cn * n + k.cb * b + k.ct * t;
+    S::store(a.g2 + (x + y * nx + (int64_t)z * nx * ny), v);
+  }
+};
+
+// ---- variant family 2: hand-written z-march with a register queue --------
+// Thread per (x, y) column; z-1/z/z+1 of the column live in registers and
+// the four in-plane neighbours are read from HBM/L2 directly.
+template <bool kPeriodic>
+__device__ __forceinline__ int Nb(int i, int d, int n) {
+  int j = i + d;
+  if (kPeriodic) return j < 0 ? j + n : (j >= n ? j - n : j);
+  return j < 0 ? 0 : (j >= n ? n - 1 : j);
+}
+
+template <class S, bool kPeriodic, int BX, int BY>
+__global__ void __launch_bounds__(BX * BY)
+Diffusion7Direct(const typename S::stored_t *__restrict__ in, typename S::stored_t *__restrict__ out,
+                 int nx, int ny, int nz, psk::Box box, int zchunk, Coef7 k) {
+  const int x = box.lo[0] + (int)blockIdx.x * BX + (int)threadIdx.x;
+  const int y = box.lo[1] + (int)blockIdx.y * BY + (int)threadIdx.y;
+  if (x >= box.hi[0] || y >= box.hi[1]) return;
+  const int zb = box.lo[2] + (int)blockIdx.z * zchunk;
+  const int ze = min(zb + zchunk, box.hi[2]);
+  const int64_t plane = (int64_t)nx * ny;
+  const int xw = Nb<kPeriodic>(x, -1, nx), xe = Nb<kPeriodic>(x, 1, nx);
+  const int yn = Nb<kPeriodic>(y, -1, ny), ys = Nb<kPeriodic>(y, 1, ny);
+  const int64_t col = x + (int64_t)y * nx;
+  const int64_t ow = xw + (int64_t)y * nx, oe = xe + (int64_t)y * nx;
+  const int64_t on = x + (int64_t)yn * nx, os = x + (int64_t)ys * nx;
+  float bv = S::load(in + col + (int64_t)Nb<kPeriodic>(zb, -1, nz) * plane);
+  float cv = S::load(in + col + (int64_t)zb * plane);
+  for (int z = zb; z < ze; ++z) {
+    const int64_t zo = (int64_t)z * plane;
+    const float tv = S::load(in + col + (int64_t)Nb<kPeriodic>(z, 1, nz) * plane);
+    const float wv = S::load(in + ow + zo), ev = S::load(in + oe + zo);
+    const float nv = S::load(in + on + zo), sv = S::load(in + os + zo);
+    const float v = k.cc * cv + k.cw * wv + k.ce * ev + k.cs * sv + k.cn * nv + k.cb * bv + k.ct * tv;
+    S::store(out + col + zo, v);
+    bv = cv;
+    cv = tv;
+  }
+}
+
+// ---- variant table --------------------------------------------------------
+struct Variant {
+  const char *name;
+  int family;  // 1 skeleton, 2 direct
+  int bx, by;
+  int zchunk;  // 0 = whole column
+  bool remap;
+};
+
+const Variant kVariants[] = {
+    {"skeleton64x4_z64", 1, 64, 4, 64, true},
+    {"skeleton64x8_z64", 1, 64, 8, 64, true},
+    {"skeleton64x4_zall", 1, 64, 4, 0, true},
+    {"skeleton64x4_z64_noremap", 1, 64, 4, 64, false},
+    {"direct64x4_z64", 2, 64, 4, 64, false},
+    {"direct64x4_zall", 2, 64, 4, 0, false},
+    {"skeleton64x2_z32", 1, 64, 2, 32, true},
+    {"skeleton64x8_z128", 1, 64, 8, 128, true},
+};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+constexpr int kDefaultVariant = 1;
+
+template <class S, bool kPeriodic>
+hipError_t LaunchOne(const Variant &v, const void *in, void *out, const int *dim, const psk::Box &box,
+                     const Coef7 &k, hipStream_t s) {
+  using st = typename S::stored_t;
+  if (v.family == 1) {
+    typename DiffusionPlan<S, kPeriodic>::Args a;
+    a.g1 = static_cast<const st *>(in);
+    a.g2 = static_cast<st *>(out);
+    a.dim[0] = dim[0];
+    a.dim[1] = dim[1];
+    a.dim[2] = dim[2];
+    a.k = k;
+    using P = DiffusionPlan<S, kPeriodic>;
+    if (v.bx == 64 && v.by == 4) return psk::LaunchZMarch<P, 64, 4>(a, box, v.zchunk, v.remap, s);
+    if (v.bx == 64 && v.by == 8) return psk::LaunchZMarch<P, 64, 8>(a, box, v.zchunk, v.remap, s);
+    if (v.bx == 64 && v.by == 2) return psk::LaunchZMarch<P, 64, 2>(a, box, v.zchunk, v.remap, s);
+    return hipErrorInvalidValue;
+  }
+  const int ex = box.hi[0] - box.lo[0], ey = box.hi[1] - box.lo[1], ez = box.hi[2] - box.lo[2];
+  if (ex <= 0 || ey <= 0 || ez <= 0) return hipSuccess;
+  int zc = v.zchunk <= 0 || v.zchunk > ez ? ez : v.zchunk;
+  dim3 grid((ex + 63) / 64, (ey + 3) / 4, (ez + zc - 1) / zc);
+  hipLaunchKernelGGL((Diffusion7Direct<S, kPeriodic, 64, 4>), grid, dim3(64, 4, 1), 0, s,
+                     static_cast<const st *>(in), static_cast<st *>(out), dim[0], dim[1], dim[2], box,
+                     zc, k);
+  return hipGetLastError();
+}
+
+template <class S>
+hipError_t LaunchStorage(const Variant &v, bool periodic, const void *in, void *out, const int *dim,
+                         const psk::Box &box, const Coef7 &k, hipStream_t s) {
+  return periodic ? LaunchOne<S, true>(v, in, out, dim, box, k, s)
+                  : LaunchOne<S, false>(v, in, out, dim, box, k, s);
+}
+
+hipError_t LaunchStep(const Variant &v, int storage, bool periodic, const void *in, void *out,
+                      const int *dim, const psk::Box &box, const Coef7 &k, hipStream_t s) {
+  switch (storage) {
+    case PS_HIP_STORAGE_NATIVE:
+      return LaunchStorage<psk::Native<float>>(v, periodic, in, out, dim, box, k, s);
+    case PS_HIP_STORAGE_BF16:
+      return LaunchStorage<psk::BF16>(v, periodic, in, out, dim, box, k, s);
+    case PS_HIP_STORAGE_FP16:
+      return LaunchStorage<psk::FP16>(v, periodic, in, out, dim, box, k, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+}  // namespace physis_hip
+
+using namespace physis_hip;
+
+extern "C" {
+
+int PSHipDiffusion7Variants(void) { return kNumVariants; }
+
+const char *PSHipDiffusion7VariantName(int variant) {
+  if (variant < 1 || variant > kNumVariants) return nullptr;
+  return kVariants[variant - 1].name;
+}
+
+int PSHipDiffusion7Run(__PSGrid *g1, __PSGrid *g2, const __PSDomain *dom, const float *coef,
+                       int boundary, int steps, int variant) {
+  if (!g1 || !g2 || !dom || !coef || steps < 0) return PS_HIP_ERR_INVALID;
+  if (boundary != PS_HIP_CLAMPED && boundary != PS_HIP_PERIODIC) return PS_HIP_ERR_INVALID;
+  if (g1->num_dims != 3 || g2->num_dims != 3 || g1->elm_size != 4 || g2->elm_size != 4)
+    return PS_HIP_ERR_INVALID;
+  for (int d = 0; d < 3; ++d) {
+    if (g1->dim[d] != g2->dim[d]) return PS_HIP_ERR_INVALID;
+    if (dom->local_min[d] < 0 || dom->local_max[d] > g1->dim[d]) return PS_HIP_ERR_INVALID;
+  }
+  if (Rec(g1)->storage != Rec(g2)->storage) return PS_HIP_ERR_INVALID;
+  if (variant == 0) variant = kDefaultVariant;
+  if (variant < 1 || variant > kNumVariants) return PS_HIP_ERR_INVALID;
+  const Variant &v = kVariants[variant - 1];
+  Coef7 k{coef[0], coef[1], coef[2], coef[3], coef[4], coef[5], coef[6]};
+  psk::Box box = psk::BoxFromDomain(*dom, 3);
+  hipStream_t s = Runtime::Get().stream();
+  const int storage = Rec(g1)->storage;
+  const bool periodic = boundary == PS_HIP_PERIODIC;
+  for (int i = 0; i < steps; ++i) {
+    const __PSGrid *in = (i & 1) ? g2 : g1;
+    __PSGrid *out = (i & 1) ? g1 : g2;
+    if (LaunchStep(v, storage, periodic, in->p0, out->p0, g1->dim, box, k, s) != hipSuccess)
+      return PS_HIP_ERR_DEVICE;
+  }
+  return PS_HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// runtime_hip.h -- internal interface of libphysis_rt_hip (not installed).
+//
+// The public handle type __PSGrid is the first member of GridRec, so a
+// __PSGrid* returned to generated code can be turned back into the record.
+// Generated code only ever touches the public prefix (p0, dim, elm_size,
+// num_dims, num_elms, dev), exactly the fields the CUDA target reads
+// (reference translator/cuda_translator.cc:426-429, 476-489).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+
+#include "physis/physis_hip.h"
+#include "physis/physis_hip_ext.h"
+
+namespace physis_hip {
+
+[[noreturn]] void Fatal(const char *where, const char *what);
+void CheckHip(hipError_t e, const char *where);
+
+#define PSH_CHECK(call) ::physis_hip::CheckHip((call), #call)
+
+// Per-grid record.  `pub` must stay the first member.
+struct GridRec {
+  __PSGrid pub;
+  __PSGrid_dev view;       // pub.dev points here
+  int storage;             // PSHipStorage
+  size_t stored_elm_size;  // bytes per element in HBM
+  size_t bytes;            // device allocation size
+  bool user_type;          // allocated through a devNewFunc
+};
+
+inline GridRec *Rec(__PSGrid *g) { return reinterpret_cast<GridRec *>(g); }
+inline const GridRec *Rec(const __PSGrid *g) {
+  return reinterpret_cast<const GridRec *>(g);
+}
+
+class Runtime {
+ public:
+  static Runtime &Get();
+  // Device selection and stream creation; idempotent.
+  void EnsureDevice();
+  hipStream_t stream() {
+    EnsureDevice();
+    return stream_;
+  }
+  int device() const { return device_; }
+  // Device bytes currently held by grids created through this runtime.
+  size_t grid_bytes() const { return grid_bytes_; }
+  void AddGridBytes(ptrdiff_t d) { grid_bytes_ += d; }
+  // A device scratch buffer of at least `bytes`, reused across calls that
+  // are ordered on stream().  Grows, never shrinks.
+  void *Scratch(size_t bytes);
+  // Pinned host scratch of at least `bytes` (for small results).
+  void *PinnedScratch(size_t bytes);
+  bool trace() const { return trace_; }
+  void set_trace(bool t) { trace_ = t; }
+  void Shutdown();
+
+ private:
+  Runtime() = default;
+  bool inited_ = false;
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  size_t grid_bytes_ = 0;
+  void *scratch_ = nullptr;
+  size_t scratch_bytes_ = 0;
+  void *pinned_ = nullptr;
+  size_t pinned_bytes_ = 0;
+  bool trace_ = false;
+  std::mutex mu_;
+};
+
+// Device-side storage conversions for 16-bit grids (buffer_hip.hip).
+void ConvertToStorage(void *dst, const float *src, int64_t n, int storage,
+                      hipStream_t s);
+void ConvertFromStorage(float *dst, const void *src, int64_t n, int storage,
+                        hipStream_t s);
+
+size_t StoredElmSize(int elm_size, int storage);
+
+}  // namespace physis_hip

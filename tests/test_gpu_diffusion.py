@@ -1,0 +1,139 @@
+"""7-point diffusion on the GPU through the C ABI, checked against the oracle.
+
+FP32 results must be bitwise equal to the sequential ref restatement (both
+evaluate the DSL expression left to right with no FMA contraction).  At the
+full 512^3 size the checks are a bitwise comparison over 2 steps and mass
+conservation over many steps.  16-bit storage is checked against the FP64
+oracle within a stated tolerance.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+K_BENCH = [0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.4]          # ce, cw, cn, cs, ct, cb, cc
+K_ASYM = [0.11, 0.07, 0.05, 0.13, 0.02, 0.17, 0.45]
+
+
+def run_hip(rt, field, coef, boundary, steps, variant=0, dom=None, storage=0):
+    nz, ny, nx = field.shape
+    g1 = rt.Grid((nx, ny, nz), storage=storage)
+    g2 = rt.Grid((nx, ny, nz), storage=storage)
+    g1.copyin(field)
+    g2.copyin(np.zeros_like(field))
+    d = rt.domain3d(*dom) if dom is not None else None
+    rt.diffusion7_run(g1, g2, coef, boundary, steps, dom=d, variant=variant)
+    a, b = g1.copyout(), g2.copyout()
+    g1.free()
+    g2.free()
+    return a, b
+
+
+def variant_ids(rt):
+    return list(rt.diffusion7_variants().keys())
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_64cubed_100_steps_bitwise_all_variants(rt, boundary):
+    field = O.uniform_field((64, 64, 64), seed=1)
+    want1, want2 = O.diffusion7(field, K_BENCH, periodic=bool(boundary), steps=100)
+    for v in variant_ids(rt):
+        print("variant", v, rt.diffusion7_variants()[v], flush=True)
+        got1, got2 = run_hip(rt, field, K_BENCH, boundary, 100, variant=v)
+        assert np.array_equal(got1, want1), "variant %d" % v
+        assert np.array_equal(got2, want2), "variant %d" % v
+
+
+@pytest.mark.parametrize("shape", [(37, 23, 19), (65, 5, 3), (1, 1, 1), (2, 3, 2), (130, 9, 70)])
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_ragged_shapes_bitwise(rt, shape, boundary):
+    field = O.uniform_field(shape, seed=11)
+    want1, want2 = O.diffusion7(field, K_ASYM, periodic=bool(boundary), steps=7)
+    for v in variant_ids(rt):
+        got1, got2 = run_hip(rt, field, K_ASYM, boundary, 7, variant=v)
+        assert np.array_equal(got1, want1) and np.array_equal(got2, want2), "variant %d" % v
+
+
+def test_subdomain_bitwise(rt):
+    shape = (40, 33, 21)
+    dom = (3, 37, 1, 30, 2, 20)
+    field = O.uniform_field(shape, seed=4)
+    want1, want2 = O.diffusion7(field, K_ASYM, periodic=False, steps=4, dom=dom)
+    for v in variant_ids(rt):
+        got1, got2 = run_hip(rt, field, K_ASYM, 0, 4, variant=v, dom=dom)
+        assert np.array_equal(got1, want1) and np.array_equal(got2, want2), "variant %d" % v
+
+
+def test_empty_domain_is_a_no_op(rt):
+    field = O.uniform_field((8, 8, 8), seed=4)
+    got1, got2 = run_hip(rt, field, K_BENCH, 0, 3, dom=(4, 4, 0, 8, 0, 8))
+    assert np.array_equal(got1, field) and np.all(got2 == 0)
+
+
+def test_neumann_cond_fixture_on_gpu(rt):
+    """The reference test test_7-pt-neumann-cond (N=32, 10 steps) on the GPU."""
+    from tests.test_oracle import GOLDEN, neumann_coefficients, sha
+    n = 32
+    field = np.arange(n ** 3, dtype=np.float32).reshape(n, n, n)
+    got1, _ = run_hip(rt, field, neumann_coefficients(n), 0, 10)
+    assert sha(got1) == GOLDEN["test_7-pt-neumann-cond"]["sha256"]
+
+
+def test_512cubed_two_steps_bitwise(rt):
+    n = 512
+    print("512^3 bitwise, 2 steps", flush=True)
+    g1 = rt.Grid((n, n, n))
+    g2 = rt.Grid((n, n, n))
+    g1.fill_uniform(7)
+    field = O.uniform_field((n, n, n), seed=7)
+    assert np.array_equal(g1.copyout(), field)
+    for boundary in (0, 1):
+        g1.copyin(field)
+        rt.diffusion7_run(g1, g2, K_BENCH, boundary, 2)
+        want1, _ = O.diffusion7(field, K_BENCH, periodic=bool(boundary), steps=2)
+        assert np.array_equal(g1.copyout(), want1), "boundary %d" % boundary
+    g1.free()
+    g2.free()
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_512cubed_mass_conservation(rt, boundary):
+    n = 512
+    g1 = rt.Grid((n, n, n))
+    g2 = rt.Grid((n, n, n))
+    g1.fill_uniform(3)
+    before = g1.reduce64(rt.PS_SUM)
+    rt.diffusion7_run(g1, g2, K_BENCH, boundary, 200)
+    after = g1.reduce64(rt.PS_SUM)
+    # FP32 rounding per update; the operator itself conserves mass exactly.
+    assert abs(after - before) / before < 1e-5
+    assert abs(before / n ** 3 - 0.5) < 1e-3
+    g1.free()
+    g2.free()
+
+
+@pytest.mark.parametrize("storage,tol", [(1, 2e-2), (2, 2e-3)])
+def test_16bit_storage_vs_fp64(rt, storage, tol):
+    shape = (48, 40, 32)
+    field = O.uniform_field(shape, seed=9)
+    want, _ = O.diffusion7(field.astype(np.float64), K_BENCH, periodic=False, steps=20, dtype=np.float64)
+    got, _ = run_hip(rt, field, K_BENCH, 0, 20, storage=storage)
+    err = np.max(np.abs(got - want))
+    print("storage", storage, "max abs error vs FP64", err, flush=True)
+    assert err < tol
+
+
+def test_invalid_arguments_are_rejected(rt):
+    g1 = rt.Grid((8, 8, 8))
+    g2 = rt.Grid((8, 8, 4))
+    with pytest.raises(rt.PhysisError):
+        rt.diffusion7_run(g1, g2, K_BENCH, 0, 1)
+    g3 = rt.Grid((8, 8, 8))
+    with pytest.raises(rt.PhysisError):
+        rt.diffusion7_run(g1, g3, K_BENCH, 5, 1)
+    with pytest.raises(rt.PhysisError):
+        rt.diffusion7_run(g1, g3, K_BENCH, 0, 1, dom=rt.domain3d(0, 9, 0, 8, 0, 8))
+    for g in (g1, g2, g3):
+        g.free()

@@ -1,0 +1,86 @@
+"""Himeno 19-point Jacobi on the GPU vs the oracle.
+
+FP32 pressure fields are bitwise equal to the sequential ref restatement
+(same expression order, no FMA contraction).  The FP64 residual sums differ
+from the oracle's sequential sum only by summation order (relative 1e-12).
+16-bit storage is compared with the FP64 oracle within a stated tolerance.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def himeno_inputs(shape, seed):
+    """Config-3 inputs: coefficients uniform in [0.9, 1.1] (a3 in [0.15,
+    0.18]), bnd = 1, wrk1 = 0, p = (z/(nz-1))^2."""
+    nx, ny, nz = shape
+    coefs = []
+    for i in range(12):
+        if i == 3:
+            c = O.uniform_field(shape, seed=seed + i, lo=0.15, hi=0.18)
+        elif i == 10:
+            c = np.ones(tuple(reversed(shape)), np.float32)
+        elif i == 11:
+            c = np.zeros(tuple(reversed(shape)), np.float32)
+        else:
+            c = O.uniform_field(shape, seed=seed + i, lo=0.9, hi=1.1)
+        coefs.append(c)
+    z = np.arange(nz, dtype=np.float32)
+    p = np.broadcast_to((z * z / np.float32((nz - 1) * (nz - 1)))[:, None, None], (nz, ny, nx)).astype(np.float32)
+    return p, coefs
+
+
+def run_hip(rt, p, coefs, omega, sweeps, storage=0, variant=0):
+    nz, ny, nx = p.shape
+    shape = (nx, ny, nz)
+    p0, p1 = rt.Grid(shape, storage=storage), rt.Grid(shape, storage=storage)
+    p0.copyin(p)
+    p1.copyin(p)
+    grids = []
+    for c in coefs:
+        g = rt.Grid(shape, storage=storage)
+        g.copyin(c)
+        grids.append(g)
+    gosa = rt.himeno_run(p0, p1, grids, omega, sweeps, variant=variant)
+    out0, out1 = p0.copyout(), p1.copyout()
+    for g in grids + [p0, p1]:
+        g.free()
+    return out0, out1, gosa
+
+
+@pytest.mark.parametrize("shape", [(65, 33, 33), (19, 7, 5), (3, 3, 3), (129, 17, 9)])
+def test_himeno_bitwise_vs_oracle(rt, shape):
+    p, coefs = himeno_inputs(shape, seed=100)
+    want0, want1, wgosa = O.himeno(p, coefs, 0.8, 6)
+    for variant in (1, 2, 3):
+        print("shape", shape, "variant", variant, flush=True)
+        got0, got1, gosa = run_hip(rt, p, coefs, 0.8, 6, variant=variant)
+        assert np.array_equal(got0, want0) and np.array_equal(got1, want1)
+        np.testing.assert_allclose(gosa, wgosa, rtol=1e-12, atol=1e-300)
+
+
+def test_himeno_config3_shape_400_sweeps_vs_fp64(rt):
+    shape = (257, 129, 129)
+    p, coefs = himeno_inputs(shape, seed=200)
+    got0, _, gosa = run_hip(rt, p, coefs, 0.8, 40)
+    want0, _, wgosa = O.himeno(p.astype(np.float64), [c.astype(np.float64) for c in coefs], 0.8, 40,
+                               dtype=np.float64)
+    rel = np.max(np.abs(got0 - want0)) / np.max(np.abs(want0))
+    print("FP32 storage: max rel error vs FP64 after 40 sweeps", rel, flush=True)
+    assert rel < 1e-5
+    np.testing.assert_allclose(gosa, wgosa, rtol=1e-4)
+
+
+@pytest.mark.parametrize("storage,tol", [(1, 3e-2), (2, 5e-3)])
+def test_himeno_16bit_storage_vs_fp64(rt, storage, tol):
+    shape = (65, 33, 33)
+    p, coefs = himeno_inputs(shape, seed=300)
+    got0, _, gosa = run_hip(rt, p, coefs, 0.8, 10, storage=storage)
+    want0, _, _ = O.himeno(p.astype(np.float64), [c.astype(np.float64) for c in coefs], 0.8, 10,
+                           dtype=np.float64)
+    err = np.max(np.abs(got0 - want0))
+    print("storage", storage, "max abs error vs FP64", err, flush=True)
+    assert err < tol
