@@ -107,6 +107,18 @@ def main():
     g2 = rt.Grid((n, n, n))
     g1.fill_uniform(1)
 
+    # Yardstick: a plain streaming copy of one grid (read + write n^3*4 B).
+    timer = rt.Timer()
+    for _ in range(3):
+        rt.stream_copy(g2, g1)
+    reps = 20
+    timer.begin()
+    for _ in range(reps):
+        rt.stream_copy(g2, g1)
+    timer.end()
+    copy_gbs = 2 * 4 * n ** 3 * reps / (timer.elapsed_ms() / 1e3) / 1e9
+    print("stream copy: %.1f GB/s" % copy_gbs, file=sys.stderr, flush=True)
+
     if args.sweep:
         for v, name in rt.diffusion7_variants().items():
             for boundary in (0, 1):
@@ -151,7 +163,8 @@ def main():
                    "grid": [n, n, n], "boundary": "clamped", "steps_timed": args.steps,
                    "variant": args.variant,
                    "periodic_glups": round(n ** 3 * args.steps / results[1][0] / 1e9, 2),
-                   "hbm_roofline_pct": round(100 * achieved / HBM_PEAK_GBS, 2)},
+                   "hbm_roofline_pct": round(100 * achieved / HBM_PEAK_GBS, 2),
+                   "stream_copy_gbs": round(copy_gbs, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,

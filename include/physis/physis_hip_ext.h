@@ -59,6 +59,10 @@ extern void *PSHipGridDevicePtr(const __PSGrid *g);
  * i the linear (x fastest) index.  Asynchronous. */
 extern int PSHipGridFillUniform(__PSGrid *g, uint64_t seed, float lo, float hi);
 
+/* Streaming device copy of src into dst (same byte size, a multiple of 16)
+ * with one plain kernel: the HBM bandwidth yardstick.  Asynchronous. */
+extern int PSHipStreamCopy(__PSGrid *dst, const __PSGrid *src);
+
 /* Reduction with an FP64 result.  SUM and PROD accumulate in FP64 in a
  * fixed order (bitwise reproducible); MAX and MIN are exact. */
 extern int PSHipReduceGrid(double *out, enum PSReduceOp op, __PSGrid *g);

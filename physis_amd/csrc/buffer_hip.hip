@@ -40,6 +40,17 @@ fill_uniform(typename S::stored_t *dst, int64_t n, uint64_t seed, float lo, floa
   for (; i < n; i += stride) S::store_cached(dst + i, (typename S::value_t)(lo + span * UnitF32(seed, i)));
 }
 
+// Plain streaming copy (16 bytes per lane per iteration): the bandwidth
+// yardstick that stencil kernels and HBM counter readings are scaled by.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256)
+stream_copy(v4f *__restrict__ dst, const v4f *__restrict__ src, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) __builtin_nontemporal_store(src[i], dst + i);
+}
+
 unsigned BlocksFor(int64_t n) {
   int64_t b = (n + 255) / 256;
   return (unsigned)(b < 8192 ? (b < 1 ? 1 : b) : 8192);
@@ -74,6 +85,18 @@ void ConvertFromStorage(float *dst, const void *src, int64_t n, int storage, hip
 }  // namespace physis_hip
 
 using namespace physis_hip;
+
+extern "C" int PSHipStreamCopy(__PSGrid *dst, const __PSGrid *src) {
+  if (!dst || !src || !dst->p0 || !src->p0) return PS_HIP_ERR_INVALID;
+  const GridRec *rs = Rec(src);
+  if (rs->bytes != Rec(dst)->bytes || rs->bytes % 16 != 0) return PS_HIP_ERR_INVALID;
+  const int64_t n = (int64_t)(rs->bytes / 16);
+  int64_t b = (n + 255) / 256;
+  if (b > 256 * 64) b = 256 * 64;
+  hipLaunchKernelGGL(stream_copy, dim3((unsigned)b), dim3(256), 0, Runtime::Get().stream(),
+                     static_cast<v4f *>(dst->p0), static_cast<const v4f *>(src->p0), n);
+  return hipGetLastError() == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
+}
 
 extern "C" int PSHipGridFillUniform(__PSGrid *g, uint64_t seed, float lo, float hi) {
   if (!g || !g->p0) return PS_HIP_ERR_INVALID;

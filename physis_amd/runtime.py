@@ -82,6 +82,7 @@ def lib():
         if hasattr(L, "PSHipHimenoRun"):
             L.PSHipHimenoRun.argtypes = [_GP, _GP, ctypes.POINTER(_GP), ctypes.c_float, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.PSHipStreamCopy.argtypes = [_GP, _GP]
         if hasattr(L, "PSHipGridFillUniform"):
             L.PSHipGridFillUniform.argtypes = [_GP, ctypes.c_uint64, ctypes.c_float, ctypes.c_float]
         _lib = L
@@ -196,6 +197,11 @@ class Grid:
                 _lib.PSHipGridFree(self.handle)
         except Exception:
             pass
+
+
+def stream_copy(dst, src):
+    """One plain streaming copy kernel dst <- src (bandwidth yardstick)."""
+    check(lib().PSHipStreamCopy(dst.handle, src.handle), "PSHipStreamCopy")
 
 
 def diffusion7_run(g1, g2, coef, boundary, steps, dom=None, variant=0):
