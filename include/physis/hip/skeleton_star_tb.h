@@ -1,0 +1,185 @@
+// skeleton_star_tb.h -- two time steps per HBM pass for radius-1 star stencils.
+//
+// Fuses the two maps of a ping-pong PSStencilRun (map(k, dom, A, B, ...) then
+// map(k, dom, B, A, ...), same kernel and scalars, domain = whole grid) into
+// one z-march: u1 = step(u0) is kept on chip and only u2 = step(u1) is
+// written, so one read and one write of HBM advance the field two steps.
+// u2 goes to a third buffer, never back into u0: workgroups march z at
+// different paces and still read u0 planes that a faster neighbour has
+// passed.
+//
+// A workgroup spans whole rows (nx <= 256*NWX, one lane per four x points)
+// and BY rows; it computes u1 on its BY rows and writes u2 on the inner
+// BY-2 rows (tiles overlap by two rows in y).  Per z step k:
+//   publish u0 plane k (rows y0-1 .. y0+BY) and u1 plane k-1 to LDS
+//   (double-buffered by step parity: one barrier per step), then
+//   u1(k)   = body(u0 planes k-1, k, k+1)          registers + LDS rows
+//   u2(k-1) = body(u1 planes k-2, k-1, k)          registers + LDS rows
+// Rows have four pad floats at each end.  Periodic plans fill the pads with
+// the wrapped row ends; for clamped plans the pads are never selected by the
+// body (it reads the point itself at a clamped face).  Intermediate points
+// outside the grid are evaluated at their wrapped/clamped coordinates.
+// Each u1 value is computed by the same body in FP32 as in the one-step
+// path, so the fused result is bitwise equal to two separate steps.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "physis/hip/skeleton.h"
+#include "physis/hip/skeleton_star.h"
+
+namespace psk {
+
+struct TBGeometry {
+  int tiles_y;  // y tiles, each writing BY-2 rows
+  int zchunk;
+  int remap;
+};
+
+template <class Plan, int NWX, int BY>
+__global__ void __launch_bounds__(64 * NWX * BY)
+ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
+  using S = typename Plan::Storage;
+  using st = typename S::stored_t;
+  constexpr bool W = Plan::kWrap;
+  constexpr int WX = 256 * NWX, ROW = WX + 8;
+  static_assert(BY >= 3, "need at least one output row");
+  __shared__ float L0[2][BY + 2][ROW];
+  __shared__ float L1[2][BY][ROW];
+
+  const int lane = threadIdx.x & 63, wx = threadIdx.x >> 6, wy = threadIdx.y;
+  const int tile = geo.remap ? XcdTile(blockIdx.x, geo.tiles_y) : (int)blockIdx.x;
+  int nx, ny, nz;
+  Plan::stage_dims(a1, nx, ny, nz);
+  const int ya = tile * (BY - 2) - 1;
+  const int y = ya + wy;
+  const int zb = (int)blockIdx.y * geo.zchunk;
+  const int ze = min(zb + geo.zchunk, nz);
+  const int xs = wx * 256 + lane * 4;
+  const bool has = xs < nx;
+  const int64_t plane = (int64_t)nx * ny;
+  const st *in = Plan::stage_ptr(a1);
+  st *out = Plan::out_ptr(a2);
+  const int yf = FixIndex<W>(y, ny);
+  const int64_t row = (int64_t)yf * nx + xs;
+  const bool halo_lo = wy == 0, halo_hi = wy == BY - 1;
+  const int64_t hrow_lo = (int64_t)FixIndex<W>(ya - 1, ny) * nx + xs;
+  const int64_t hrow_hi = (int64_t)FixIndex<W>(ya + BY, ny) * nx + xs;
+  const bool out_row = wy >= 1 && wy <= BY - 2 && y < ny;
+  // Pad writers: the lane holding the row's last four points fills the left
+  // pad, the lane holding the first four the right pad.
+  const bool pad_l = W && xs == nx - 4, pad_r = W && xs == 0;
+
+  auto zoff = [&](int z) { return (int64_t)FixIndex<W>(z, nz) * plane; };
+  auto ld = [&](int64_t off, float (&v)[4]) {
+    if (has) {
+      Load4<S>(in + off, v);
+    } else {
+      v[0] = v[1] = v[2] = v[3] = 0.f;
+    }
+  };
+  auto put = [&](float *r, const float (&v)[4]) {
+    if (!has) return;
+    *reinterpret_cast<v4f_t *>(r + xs + 4) = v4f_t{v[0], v[1], v[2], v[3]};
+    if (pad_l) *reinterpret_cast<v4f_t *>(r) = v4f_t{v[0], v[1], v[2], v[3]};
+    if (pad_r) *reinterpret_cast<v4f_t *>(r + nx + 4) = v4f_t{v[0], v[1], v[2], v[3]};
+  };
+  auto get4 = [&](const float *r, float (&v)[4]) {
+    const v4f_t q = *reinterpret_cast<const v4f_t *>(r + xs + 4);
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+  };
+
+  float b0[4], c0[4], t0[4], p0[4] = {}, hlo[4] = {}, hhi[4] = {}, nlo[4] = {}, nhi[4] = {};
+  float b1[4] = {}, c1[4] = {}, t1[4] = {};
+  ld(zoff(zb - 2) + row, b0);
+  ld(zoff(zb - 1) + row, c0);
+  ld(zoff(zb) + row, t0);
+  if (halo_lo) ld(zoff(zb - 1) + hrow_lo, hlo);
+  if (halo_hi) ld(zoff(zb - 1) + hrow_hi, hhi);
+
+  for (int k = zb - 1; k <= ze; ++k) {
+    const int buf = (k + 2) & 1;
+    put(&L0[buf][wy + 1][0], c0);
+    if (halo_lo) put(&L0[buf][0][0], hlo);
+    if (halo_hi) put(&L0[buf][BY + 1][0], hhi);
+    put(&L1[buf][wy][0], c1);
+    if (k < ze) {
+      ld(zoff(k + 2) + row, p0);
+      if (halo_lo) ld(zoff(k + 1) + hrow_lo, nlo);
+      if (halo_hi) ld(zoff(k + 1) + hrow_hi, nhi);
+    }
+    __syncthreads();
+    if (has) {
+      // u1 at plane k, this wave's row.
+      float nv[4], sv[4];
+      get4(&L0[buf][wy][0], nv);
+      get4(&L0[buf][wy + 2][0], sv);
+      const float wl = L0[buf][wy + 1][xs + 3], er = L0[buf][wy + 1][xs + 8];
+      const int kf = FixIndex<W>(k, nz);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        StarCtx c;
+        c.c = c0[j];
+        c.w = j > 0 ? c0[j - 1] : wl;
+        c.e = j < 3 ? c0[j + 1] : er;
+        c.n = nv[j];
+        c.s = sv[j];
+        c.b = b0[j];
+        c.t = t0[j];
+        t1[j] = Plan::compute(c, a1, xs + j, yf, kf);
+      }
+      // u2 at plane k-1 on the output rows.
+      if (out_row && k - 1 >= zb) {
+        float n1[4], s1[4], v[4];
+        get4(&L1[buf][wy - 1][0], n1);
+        get4(&L1[buf][wy + 1][0], s1);
+        const float w1 = L1[buf][wy][xs + 3], e1 = L1[buf][wy][xs + 8];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          StarCtx c;
+          c.c = c1[j];
+          c.w = j > 0 ? c1[j - 1] : w1;
+          c.e = j < 3 ? c1[j + 1] : e1;
+          c.n = n1[j];
+          c.s = s1[j];
+          c.b = b1[j];
+          c.t = t1[j];
+          v[j] = Plan::compute(c, a2, xs + j, y, k - 1);
+        }
+        Store4<S>(out + (int64_t)(k - 1) * plane + (int64_t)y * nx + xs, v);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      b0[j] = c0[j];
+      c0[j] = t0[j];
+      t0[j] = p0[j];
+      hlo[j] = nlo[j];
+      hhi[j] = nhi[j];
+      b1[j] = c1[j];
+      c1[j] = t1[j];
+    }
+  }
+}
+
+// Whether a grid can take the fused two-step path with NWX waves per row.
+template <int NWX>
+inline bool StarTB2Supported(int nx, int ny, int nz) {
+  return nx % 4 == 0 && nx <= 256 * NWX && nx >= 4 && ny >= 1 && nz >= 1;
+}
+
+template <class Plan, int NWX, int BY>
+inline hipError_t LaunchZMarchStarTB2(const typename Plan::Args &a1, const typename Plan::Args &a2, int ny,
+                                      int nz, int zchunk, bool remap, hipStream_t stream) {
+  TBGeometry geo;
+  geo.tiles_y = (ny + BY - 3) / (BY - 2);
+  if (zchunk <= 0 || zchunk > nz) zchunk = nz;
+  geo.zchunk = zchunk;
+  geo.remap = remap ? 1 : 0;
+  dim3 grid(geo.tiles_y, (nz + zchunk - 1) / zchunk, 1);
+  hipLaunchKernelGGL((ZMarchStar4TB2<Plan, NWX, BY>), grid, dim3(64 * NWX, BY, 1), 0, stream, a1, a2, geo);
+  return hipGetLastError();
+}
+
+}  // namespace psk

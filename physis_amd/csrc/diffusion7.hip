@@ -8,6 +8,8 @@
 // sequential ref target does, so results are bitwise equal to it.
 #include "runtime_hip.h"
 #include "physis/hip/skeleton.h"
+#include "physis/hip/skeleton_star.h"
+#include "physis/hip/skeleton_star_tb.h"
 #include "physis/hip/storage.h"
 
 #include <cstdio>
@@ -39,8 +41,17 @@ struct DiffusionPlan {
     nz = a.dim[2];
   }
   __device__ static SV stage_load(const Args &a, int, int64_t off) { return S::load(a.g1 + off); }
+  // Star-skeleton interface: single input grid, one emitted value per point.
+  using Storage = S;
+  __device__ static const typename S::stored_t *stage_ptr(const Args &a) { return a.g1; }
+  __device__ static typename S::stored_t *out_ptr(const Args &a) { return a.g2; }
   template <class C>
   __device__ static void point(const C &ctx, const Args &a, const int x, const int y, const int z) {
+    const int nx = a.dim[0], ny = a.dim[1];
+    S::store(a.g2 + (x + y * nx + (int64_t)z * nx * ny), compute(ctx, a, x, y, z));
+  }
+  template <class C>
+  __device__ static float compute(const C &ctx, const Args &a, const int x, const int y, const int z) {
     const int nx = a.dim[0], ny = a.dim[1], nz = a.dim[2];
     float c, w, e, n, s, b, t;
     c = ctx.at(0, 0, 0, 0);
@@ -60,8 +71,7 @@ struct DiffusionPlan {
       if (z == nz - 1) t = ctx.at(0, 0, 0, 0); else t = ctx.at(0, 0, 0, 1);
     }
     const Coef7 &k = a.k;
-    const float v = k.cc * c + k.cw * w + k.ce * e + k.cs * s + k.cn * n + k.cb * b + k.ct * t;
-    S::store(a.g2 + (x + y * nx + (int64_t)z * nx * ny), v);
+    return k.cc * c + k.cw * w + k.ce * e + k.cs * s + k.cn * n + k.cb * b + k.ct * t;
   }
 };
 
@@ -107,7 +117,7 @@ Diffusion7Direct(const typename S::stored_t *__restrict__ in, typename S::stored
 // ---- variant table --------------------------------------------------------
 struct Variant {
   const char *name;
-  int family;  // 1 skeleton, 2 direct
+  int family;  // 1 LDS-ring skeleton, 2 direct, 3 vectorised star skeleton
   int bx, by;
   int zchunk;  // 0 = whole column
   bool remap;
@@ -122,15 +132,96 @@ const Variant kVariants[] = {
     {"direct64x4_zall", 2, 64, 4, 0, false},
     {"skeleton64x2_z32", 1, 64, 2, 32, true},
     {"skeleton64x8_z128", 1, 64, 8, 128, true},
+    {"star4_by4_z64", 3, 256, 4, 64, true},
+    {"star4_by8_z64", 3, 256, 8, 64, true},
+    {"star4_by2_z64", 3, 256, 2, 64, true},
+    {"star4_by4_z128", 3, 256, 4, 128, true},
+    {"star4_by4_z32", 3, 256, 4, 32, true},
+    {"star4_by8_z128", 3, 256, 8, 128, true},
+    {"star4_by4_z64_noremap", 3, 256, 4, 64, false},
+    {"star4_by1_z64", 3, 256, 1, 64, true},
+    // Family 4: two fused steps per pass; bx = waves per row (rows <= 256*bx).
+    {"tb2_w2_by8_z32", 4, 2, 8, 32, true},
+    {"tb2_w2_by8_z64", 4, 2, 8, 64, true},
+    {"tb2_w2_by6_z32", 4, 2, 6, 32, true},
+    {"tb2_w2_by4_z32", 4, 2, 4, 32, true},
+    {"tb2_w2_by8_z16", 4, 2, 8, 16, true},
+    {"tb2_w1_by16_z32", 4, 1, 16, 32, true},
+    {"tb2_w1_by8_z32", 4, 1, 8, 32, true},
+    {"tb2_w4_by4_z32", 4, 4, 4, 32, true},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
-constexpr int kDefaultVariant = 1;
+constexpr int kDefaultVariant = 9;
+// One-step variant used by family-4 runs for unfused steps.
+constexpr int kStarVariant = 9;
+
+// Launches one fused pair of steps src -> (on chip) -> dst with variant v
+// (family 4).  Returns hipErrorNotSupported if the grid does not qualify.
+template <class S, bool kPeriodic>
+hipError_t LaunchPair(const Variant &v, const void *src, void *dst, const int *dim, const Coef7 &k,
+                      hipStream_t s) {
+  using P = DiffusionPlan<S, kPeriodic>;
+  using st = typename S::stored_t;
+  typename P::Args a1, a2;
+  a1.g1 = static_cast<const st *>(src);
+  a1.g2 = nullptr;
+  a2.g1 = nullptr;
+  a2.g2 = static_cast<st *>(dst);
+  for (int d = 0; d < 3; ++d) a1.dim[d] = a2.dim[d] = dim[d];
+  a1.k = a2.k = k;
+#define PSH_TB(NWX, BY)                                                                     \
+  if (v.bx == NWX && v.by == BY) {                                                          \
+    if (!psk::StarTB2Supported<NWX>(dim[0], dim[1], dim[2])) return hipErrorNotSupported; \
+    return psk::LaunchZMarchStarTB2<P, NWX, BY>(a1, a2, dim[1], dim[2], v.zchunk, v.remap, s); \
+  }
+  PSH_TB(2, 8)
+  PSH_TB(2, 6)
+  PSH_TB(2, 4)
+  PSH_TB(1, 16)
+  PSH_TB(1, 8)
+  PSH_TB(4, 4)
+#undef PSH_TB
+  return hipErrorInvalidValue;
+}
+
+hipError_t LaunchPairStorage(const Variant &v, int storage, bool periodic, const void *src, void *dst,
+                             const int *dim, const Coef7 &k, hipStream_t s) {
+  switch (storage) {
+    case PS_HIP_STORAGE_NATIVE:
+      return periodic ? LaunchPair<psk::Native<float>, true>(v, src, dst, dim, k, s)
+                      : LaunchPair<psk::Native<float>, false>(v, src, dst, dim, k, s);
+    case PS_HIP_STORAGE_BF16:
+      return periodic ? LaunchPair<psk::BF16, true>(v, src, dst, dim, k, s)
+                      : LaunchPair<psk::BF16, false>(v, src, dst, dim, k, s);
+    case PS_HIP_STORAGE_FP16:
+      return periodic ? LaunchPair<psk::FP16, true>(v, src, dst, dim, k, s)
+                      : LaunchPair<psk::FP16, false>(v, src, dst, dim, k, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+// Whether family-4 variant v can fuse steps on this grid and domain.
+bool PairApplies(const Variant &v, const int *dim, const psk::Box &box) {
+  for (int d = 0; d < 3; ++d)
+    if (box.lo[d] != 0 || box.hi[d] != dim[d]) return false;  // whole-grid domain only
+  switch (v.bx) {
+    case 1: return psk::StarTB2Supported<1>(dim[0], dim[1], dim[2]);
+    case 2: return psk::StarTB2Supported<2>(dim[0], dim[1], dim[2]);
+    case 4: return psk::StarTB2Supported<4>(dim[0], dim[1], dim[2]);
+  }
+  return false;
+}
 
 template <class S, bool kPeriodic>
 hipError_t LaunchOne(const Variant &v, const void *in, void *out, const int *dim, const psk::Box &box,
                      const Coef7 &k, hipStream_t s) {
   using st = typename S::stored_t;
-  if (v.family == 1) {
+  if (v.family == 3 && !psk::StarSupported(dim[0])) {
+    // Rows that are not whole 16-byte vectors run on the LDS-ring skeleton.
+    static const Variant fallback = {"skeleton64x4_z64", 1, 64, 4, 64, true};
+    return LaunchOne<S, kPeriodic>(fallback, in, out, dim, box, k, s);
+  }
+  if (v.family == 1 || v.family == 3) {
     typename DiffusionPlan<S, kPeriodic>::Args a;
     a.g1 = static_cast<const st *>(in);
     a.g2 = static_cast<st *>(out);
@@ -139,6 +230,15 @@ hipError_t LaunchOne(const Variant &v, const void *in, void *out, const int *dim
     a.dim[2] = dim[2];
     a.k = k;
     using P = DiffusionPlan<S, kPeriodic>;
+    if (v.family == 3) {
+      switch (v.by) {
+        case 1: return psk::LaunchZMarchStar<P, 1>(a, box, v.zchunk, v.remap, s);
+        case 2: return psk::LaunchZMarchStar<P, 2>(a, box, v.zchunk, v.remap, s);
+        case 4: return psk::LaunchZMarchStar<P, 4>(a, box, v.zchunk, v.remap, s);
+        case 8: return psk::LaunchZMarchStar<P, 8>(a, box, v.zchunk, v.remap, s);
+      }
+      return hipErrorInvalidValue;
+    }
     if (v.bx == 64 && v.by == 4) return psk::LaunchZMarch<P, 64, 4>(a, box, v.zchunk, v.remap, s);
     if (v.bx == 64 && v.by == 8) return psk::LaunchZMarch<P, 64, 8>(a, box, v.zchunk, v.remap, s);
     if (v.bx == 64 && v.by == 2) return psk::LaunchZMarch<P, 64, 2>(a, box, v.zchunk, v.remap, s);
@@ -207,6 +307,32 @@ int PSHipDiffusion7Run(__PSGrid *g1, __PSGrid *g2, const __PSDomain *dom, const 
   hipStream_t s = Runtime::Get().stream();
   const int storage = Rec(g1)->storage;
   const bool periodic = boundary == PS_HIP_PERIODIC;
+  int first = 0;        // first map index left to run one step at a time
+  const void *src = g1->p0;
+  if (v.family == 4) {
+    const Variant &one = kVariants[kStarVariant - 1];
+    const int fused = steps / 2 - 1;  // the last pair stays unfused so g2 ends as in the ref target
+    if (fused > 0 && PairApplies(v, g1->dim, box)) {
+      // Fused pairs ping-pong between g1 and a third buffer; the last pair then
+      // reads whichever holds the field: map(src -> g2), map(g2 -> g1).
+      void *aux = Runtime::Get().Aux(Rec(g1)->bytes);
+      if (!aux) return PS_HIP_ERR_NOMEM;
+      for (int i = 0; i < fused; ++i) {
+        void *dst = (i & 1) ? g1->p0 : aux;
+        if (LaunchPairStorage(v, storage, periodic, src, dst, g1->dim, k, s) != hipSuccess)
+          return PS_HIP_ERR_DEVICE;
+        src = dst;
+      }
+      first = 2 * fused;
+    }
+    for (int i = first; i < steps; ++i) {
+      const void *in = (i == first) ? src : ((i - first) & 1 ? g2->p0 : g1->p0);
+      void *out = ((i - first) & 1) ? g1->p0 : g2->p0;
+      if (LaunchStep(one, storage, periodic, in, out, g1->dim, box, k, s) != hipSuccess)
+        return PS_HIP_ERR_DEVICE;
+    }
+    return PS_HIP_OK;
+  }
   for (int i = 0; i < steps; ++i) {
     const __PSGrid *in = (i & 1) ? g2 : g1;
     __PSGrid *out = (i & 1) ? g1 : g2;

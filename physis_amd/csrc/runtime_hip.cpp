@@ -75,14 +75,35 @@ void *Runtime::PinnedScratch(size_t bytes) {
   return pinned_;
 }
 
+void *Runtime::Aux(size_t bytes) {
+  EnsureDevice();
+  if (bytes > aux_bytes_) {
+    if (aux_) {
+      CheckHip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+      CheckHip(hipFree(aux_), "hipFree(aux)");
+      aux_ = nullptr;
+      aux_bytes_ = 0;
+    }
+    if (hipMalloc(&aux_, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      aux_ = nullptr;
+      return nullptr;
+    }
+    aux_bytes_ = bytes;
+  }
+  return aux_;
+}
+
 void Runtime::Shutdown() {
   if (!inited_) return;
   (void)hipStreamSynchronize(stream_);
   if (scratch_) (void)hipFree(scratch_);
   if (pinned_) (void)hipHostFree(pinned_);
+  if (aux_) (void)hipFree(aux_);
   scratch_ = nullptr;
   pinned_ = nullptr;
-  scratch_bytes_ = pinned_bytes_ = 0;
+  aux_ = nullptr;
+  scratch_bytes_ = pinned_bytes_ = aux_bytes_ = 0;
   (void)hipStreamDestroy(stream_);
   stream_ = nullptr;
   inited_ = false;

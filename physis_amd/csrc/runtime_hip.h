@@ -57,6 +57,9 @@ class Runtime {
   void *Scratch(size_t bytes);
   // Pinned host scratch of at least `bytes` (for small results).
   void *PinnedScratch(size_t bytes);
+  // A device buffer of at least `bytes` that stencil programs use as a third
+  // grid buffer (fused time steps); separate from Scratch().
+  void *Aux(size_t bytes);
   bool trace() const { return trace_; }
   void set_trace(bool t) { trace_ = t; }
   void Shutdown();
@@ -71,6 +74,8 @@ class Runtime {
   size_t scratch_bytes_ = 0;
   void *pinned_ = nullptr;
   size_t pinned_bytes_ = 0;
+  void *aux_ = nullptr;
+  size_t aux_bytes_ = 0;
   bool trace_ = false;
   std::mutex mu_;
 };

@@ -172,12 +172,13 @@ class Grid:
 
     def reduce(self, op):
         """PSReduce(&v, op, g): result in the grid's element type."""
+        # getattr: a literal __name inside a class body would be name-mangled.
         if self.dtype == np.float32:
             v = ctypes.c_float()
-            lib().__PSReduceGridFloat(ctypes.byref(v), op, self.handle)
+            getattr(lib(), "__PSReduceGridFloat")(ctypes.byref(v), op, self.handle)
         else:
             v = ctypes.c_double()
-            lib().__PSReduceGridDouble(ctypes.byref(v), op, self.handle)
+            getattr(lib(), "__PSReduceGridDouble")(ctypes.byref(v), op, self.handle)
         return v.value
 
     def reduce64(self, op):

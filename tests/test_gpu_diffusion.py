@@ -46,7 +46,8 @@ def test_64cubed_100_steps_bitwise_all_variants(rt, boundary):
         assert np.array_equal(got2, want2), "variant %d" % v
 
 
-@pytest.mark.parametrize("shape", [(37, 23, 19), (65, 5, 3), (1, 1, 1), (2, 3, 2), (130, 9, 70)])
+@pytest.mark.parametrize("shape", [(37, 23, 19), (65, 5, 3), (1, 1, 1), (2, 3, 2), (130, 9, 70),
+                                   (36, 23, 19), (4, 5, 3), (260, 7, 9), (512, 3, 5), (8, 1, 1)])
 @pytest.mark.parametrize("boundary", [0, 1])
 def test_ragged_shapes_bitwise(rt, shape, boundary):
     field = O.uniform_field(shape, seed=11)
@@ -81,19 +82,28 @@ def test_neumann_cond_fixture_on_gpu(rt):
     assert sha(got1) == GOLDEN["test_7-pt-neumann-cond"]["sha256"]
 
 
-def test_512cubed_two_steps_bitwise(rt):
+def fused_variants(rt):
+    return [v for v, name in rt.diffusion7_variants().items() if name.startswith("tb2")]
+
+
+def test_512cubed_six_steps_bitwise(rt):
+    """Full size: the default kernel and the fused two-step kernels, 6 steps
+    (two fused pairs plus the final unfused pair)."""
     n = 512
-    print("512^3 bitwise, 2 steps", flush=True)
     g1 = rt.Grid((n, n, n))
     g2 = rt.Grid((n, n, n))
     g1.fill_uniform(7)
     field = O.uniform_field((n, n, n), seed=7)
     assert np.array_equal(g1.copyout(), field)
+    variants = [0] + fused_variants(rt)[:2]
     for boundary in (0, 1):
-        g1.copyin(field)
-        rt.diffusion7_run(g1, g2, K_BENCH, boundary, 2)
-        want1, _ = O.diffusion7(field, K_BENCH, periodic=bool(boundary), steps=2)
-        assert np.array_equal(g1.copyout(), want1), "boundary %d" % boundary
+        want1, want2 = O.diffusion7(field, K_BENCH, periodic=bool(boundary), steps=6)
+        for v in variants:
+            print("512^3 bitwise, 6 steps, boundary", boundary, "variant", v, flush=True)
+            g1.copyin(field)
+            rt.diffusion7_run(g1, g2, K_BENCH, boundary, 6, variant=v)
+            assert np.array_equal(g1.copyout(), want1), "boundary %d variant %d" % (boundary, v)
+            assert np.array_equal(g2.copyout(), want2), "boundary %d variant %d" % (boundary, v)
     g1.free()
     g2.free()
 
