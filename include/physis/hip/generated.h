@@ -1,0 +1,166 @@
+// generated.h -- support code included by every file physisc-hip emits.
+//
+// Grid references passed to kernels, offset helpers with the reference's
+// indexing rules (x fastest; periodic wrap (i + n) % n, reference
+// include/physis/physis_cuda.h:158-231), storage dispatch, launch error
+// checks and the ping-pong runner that fuses two maps per HBM pass.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "physis/physis_hip.h"
+#include "physis/physis_hip_ext.h"
+#include "physis/hip/skeleton.h"
+#include "physis/hip/skeleton_star.h"
+#include "physis/hip/skeleton_star_tb.h"
+#include "physis/hip/storage.h"
+
+namespace psk {
+
+// A grid as a kernel sees it: device pointer and extents (unused axes 1).
+template <class S>
+struct GridRef {
+  typename S::stored_t *p;
+  int dim[3];
+  __device__ __forceinline__ typename S::value_t load(int64_t off) const { return S::load(p + off); }
+  __device__ __forceinline__ void store(int64_t off, typename S::value_t v) const { S::store(p + off, v); }
+};
+
+template <class S>
+inline GridRef<S> MakeRef(const __PSGrid *g) {
+  GridRef<S> r;
+  r.p = static_cast<typename S::stored_t *>(g->p0);
+  for (int d = 0; d < 3; ++d) r.dim[d] = d < g->num_dims ? g->dim[d] : 1;
+  return r;
+}
+
+template <class R>
+__device__ __forceinline__ int64_t Off1(const R &, int i) { return i; }
+template <class R>
+__device__ __forceinline__ int64_t Off2(const R &g, int i, int j) { return i + (int64_t)j * g.dim[0]; }
+template <class R>
+__device__ __forceinline__ int64_t Off3(const R &g, int i, int j, int k) {
+  return i + (int64_t)j * g.dim[0] + (int64_t)k * g.dim[0] * g.dim[1];
+}
+template <class R>
+__device__ __forceinline__ int64_t OffP1(const R &g, int i) { return (i + g.dim[0]) % g.dim[0]; }
+template <class R>
+__device__ __forceinline__ int64_t OffP2(const R &g, int i, int j) {
+  return (i + g.dim[0]) % g.dim[0] + (int64_t)((j + g.dim[1]) % g.dim[1]) * g.dim[0];
+}
+template <class R>
+__device__ __forceinline__ int64_t OffP3(const R &g, int i, int j, int k) {
+  return (i + g.dim[0]) % g.dim[0] + (int64_t)((j + g.dim[1]) % g.dim[1]) * g.dim[0] +
+         (int64_t)((k + g.dim[2]) % g.dim[2]) * g.dim[0] * g.dim[1];
+}
+
+inline void CheckLaunch(hipError_t e, const char *kernel) {
+  if (e != hipSuccess) {
+    fprintf(stderr, "physis-hip: launch of %s failed: %s\n", kernel, hipGetErrorString(e));
+    PSAbort(1);
+  }
+}
+
+// Storage shared by the float grids of a map (PSHipStorage); aborts on a mix.
+inline int FloatStorage(const __PSGrid *const *grids, int n) {
+  int st = PS_HIP_STORAGE_NATIVE;
+  bool first = true;
+  for (int i = 0; i < n; ++i) {
+    int s = PSHipGridStorage(grids[i]);
+    if (first) st = s;
+    else if (s != st) {
+      fprintf(stderr, "physis-hip: float grids of one map must share a storage kind\n");
+      PSAbort(1);
+    }
+    first = false;
+  }
+  return st;
+}
+
+// Launch shape for the one-step LDS-ring skeleton by map rank.
+template <class P, int RANK>
+inline hipError_t LaunchRing(const typename P::Args &a, const Box &box, hipStream_t s) {
+  if (RANK == 1) return LaunchZMarch<P, 64, 1>(a, box, 0, true, s);
+  if (RANK == 2) return LaunchZMarch<P, 64, 4>(a, box, 0, true, s);
+  return LaunchZMarch<P, 64, 4>(a, box, 64, true, s);
+}
+
+// One step of a star plan: the vectorised skeleton when rows are whole
+// 16-byte vectors, else the LDS ring.
+template <class P>
+inline hipError_t LaunchStarStep(const typename P::Args &a, const Box &box, int nx, hipStream_t s) {
+  if (StarSupported(nx)) return LaunchZMarchStar<P, 4>(a, box, 64, true, s);
+  return LaunchZMarch<P, 64, 4>(a, box, 64, true, s);
+}
+
+// Two fused steps src -> dst for a star plan; false if the grid does not fit
+// a supported row width.
+template <class P>
+inline bool LaunchFusedPair(const typename P::Args &a1, const typename P::Args &a2, int nx, int ny, int nz,
+                            hipStream_t s) {
+  hipError_t e;
+  if (StarTB2Supported<1>(nx, ny, nz)) e = LaunchZMarchStarTB2<P, 1, 16>(a1, a2, ny, nz, 32, true, s);
+  else if (StarTB2Supported<2>(nx, ny, nz)) e = LaunchZMarchStarTB2<P, 2, 8>(a1, a2, ny, nz, 32, true, s);
+  else if (StarTB2Supported<4>(nx, ny, nz)) e = LaunchZMarchStarTB2<P, 4, 4>(a1, a2, ny, nz, 32, true, s);
+  else return false;
+  CheckLaunch(e, "fused pair");
+  return true;
+}
+
+// PSStencilRun(map(k, A -> B), map(k, B -> A), iter) for a star plan whose
+// two maps differ only in the swapped grids, over the whole grid: iter-1
+// fused pairs ping-pong between A and a third buffer, then the last pair runs
+// unfused (src -> B, B -> A) so both grids end exactly as with 2*iter
+// separate steps.  ab: args of map 1 (A -> B); ba: args of map 2 (B -> A).
+template <class P>
+inline void RunPingPong(typename P::Args ab, typename P::Args ba, int iter, int nx, int ny, int nz,
+                        size_t grid_bytes, hipStream_t s) {
+  using st = typename P::Storage::stored_t;
+  Box box;
+  const int ext[3] = {nx, ny, nz};
+  for (int d = 0; d < 3; ++d) {
+    box.lo[d] = 0;
+    box.hi[d] = ext[d];
+  }
+  st *A = P::out_ptr(ba);
+  st *B = P::out_ptr(ab);
+  const st *src = A;
+  int fused = iter - 1;
+  if (fused > 0 && (StarTB2Supported<1>(nx, ny, nz) || StarTB2Supported<2>(nx, ny, nz) ||
+                    StarTB2Supported<4>(nx, ny, nz))) {
+    st *aux = static_cast<st *>(__PSHipAuxBuffer(grid_bytes));
+    if (!aux) {
+      fprintf(stderr, "physis-hip: no memory for the fused-step buffer\n");
+      PSAbort(1);
+    }
+    for (int i = 0; i < fused; ++i) {
+      st *dst = (i & 1) ? A : aux;
+      typename P::Args a1 = ab, a2 = ba;
+      P::set_in(a1, src);
+      P::set_out(a2, dst);
+      LaunchFusedPair<P>(a1, a2, nx, ny, nz, s);
+      src = dst;
+    }
+  } else {
+    fused = 0;
+  }
+  for (int i = fused; i < iter; ++i) {
+    typename P::Args a1 = ab;
+    P::set_in(a1, src);
+    CheckLaunch(LaunchStarStep<P>(a1, box, nx, s), "step");
+    CheckLaunch(LaunchStarStep<P>(ba, box, nx, s), "step");
+    src = A;
+  }
+}
+
+}  // namespace psk
+
+// PSGrid{1,2,3}D<Type>New(n0[, n1[, n2]][, attr]) as physisc-hip lowers it.
+static inline __PSGrid *__PSGridNewHip(int elm_size, int rank, int attr, int d0, int d1, int d2) {
+  PSVectorInt d = {d0, d1, d2};
+  if (attr & (PS_GRID_STORAGE_BF16 | PS_GRID_STORAGE_FP16)) return __PSGridNewAttr(elm_size, rank, d, attr);
+  return __PSGridNew(elm_size, rank, d, NULL);
+}

@@ -142,22 +142,26 @@ ZMarch(typename Plan::Args a, Box box, Geometry geo) {
   const int64_t plane_elems = (int64_t)nx * ny;
 
   auto load_plane = [&](int p) {
-    const int64_t zo = (int64_t)FixIndex<Plan::kWrap>(p, nz) * plane_elems;
+    if constexpr (NS > 0) {
+      const int64_t zo = (int64_t)FixIndex<Plan::kWrap>(p, nz) * plane_elems;
 #pragma unroll
-    for (int s = 0; s < NS; ++s)
+      for (int s = 0; s < NS; ++s)
 #pragma unroll
-      for (int j = 0; j < PER; ++j)
-        if (PLANE % NT == 0 || tid + j * NT < PLANE) buf[s][j] = Plan::stage_load(a, s, goff[j] + zo);
+        for (int j = 0; j < PER; ++j)
+          if (PLANE % NT == 0 || tid + j * NT < PLANE) buf[s][j] = Plan::stage_load(a, s, goff[j] + zo);
+    }
   };
   auto store_plane = [&](int p) {
-    const int sl = (p - zb + RZ) % NSLOT;
+    if constexpr (NS > 0) {
+      const int sl = (p - zb + RZ) % NSLOT;
 #pragma unroll
-    for (int s = 0; s < NS; ++s)
+      for (int s = 0; s < NS; ++s)
 #pragma unroll
-      for (int j = 0; j < PER; ++j) {
-        const int e = tid + j * NT;
-        if (PLANE % NT == 0 || e < PLANE) lds[(s * NSLOT + sl) * PLANE + e] = buf[s][j];
-      }
+        for (int j = 0; j < PER; ++j) {
+          const int e = tid + j * NT;
+          if (PLANE % NT == 0 || e < PLANE) lds[(s * NSLOT + sl) * PLANE + e] = buf[s][j];
+        }
+    }
   };
 
   if constexpr (NS > 0) {

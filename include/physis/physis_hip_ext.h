@@ -67,6 +67,11 @@ extern int PSHipStreamCopy(__PSGrid *dst, const __PSGrid *src);
  * fixed order (bitwise reproducible); MAX and MIN are exact. */
 extern int PSHipReduceGrid(double *out, enum PSReduceOp op, __PSGrid *g);
 
+/* A device buffer of at least `bytes`, owned by the runtime and reused by
+ * every caller on its stream (fused time steps use it as a third grid
+ * buffer).  NULL if it cannot be allocated. */
+extern void *__PSHipAuxBuffer(size_t bytes);
+
 /* Blocks until the runtime stream is idle; returns a PSHipStatus. */
 extern int PSHipSynchronize(void);
 /* Device bytes held by live grids. */

@@ -444,6 +444,8 @@ int PSHipGridStoredElmSize(const __PSGrid *g) {
 }
 void *PSHipGridDevicePtr(const __PSGrid *g) { return g ? g->p0 : nullptr; }
 
+void *__PSHipAuxBuffer(size_t bytes) { return Runtime::Get().Aux(bytes); }
+
 int PSHipSynchronize(void) {
   return hipStreamSynchronize(Runtime::Get().stream()) == hipSuccess ? PS_HIP_OK
                                                                       : PS_HIP_ERR_DEVICE;

@@ -1,0 +1,73 @@
+"""Translate-and-build driver for Physis DSL programs on the MI355X target.
+
+The reference builds a DSL program as `physisc-cuda foo.c` followed by nvcc
+and a link against libphysis_rt_cuda.a (docs/compilation.md:11-53).  Here:
+
+    python -m physis_amd.phc foo.c -o foo            # translate + hipcc + link
+    python -m physis_amd.phc foo.c --emit-only       # just write foo.hip
+
+The executable links libphysis_rt_hip.so from physis_amd/lib through an
+rpath relative to the executable when it is placed under physis_amd/, else
+an absolute rpath.
+"""
+import argparse
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+PHYSISC = os.path.join(HERE, "bin", "physisc-hip")
+LIB_DIR = os.path.join(HERE, "lib")
+INCLUDE = os.path.join(ROOT, "include")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-w"]
+
+
+def translate(src, out_hip, config=None, defines=()):
+    if not os.path.exists(PHYSISC):
+        raise RuntimeError("physisc-hip is not built (make -C physis_amd/translator)")
+    cmd = [PHYSISC, src, "-o", out_hip]
+    if config:
+        cmd += ["--config", config]
+    for d in defines:
+        cmd.append("-D" + d)
+    subprocess.run(cmd, check=True)
+    return out_hip
+
+
+def compile_hip(hip, exe):
+    out_dir = os.path.dirname(os.path.abspath(exe))
+    rel = os.path.relpath(LIB_DIR, out_dir)
+    rpath = "$ORIGIN/" + rel if os.path.abspath(out_dir).startswith(HERE) else LIB_DIR
+    cmd = [HIPCC] + HIP_FLAGS + ["-I", INCLUDE, "-x", "hip", hip, "-L", LIB_DIR, "-lphysis_rt_hip",
+                                 "-Wl,-rpath," + rpath, "-o", exe]
+    subprocess.run(cmd, check=True)
+    return exe
+
+
+def build(src, exe, config=None, defines=(), keep_hip=None):
+    hip = keep_hip or (os.path.splitext(exe)[0] + ".hip")
+    translate(src, hip, config, defines)
+    return compile_hip(hip, exe)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("source")
+    ap.add_argument("-o", "--output")
+    ap.add_argument("--config")
+    ap.add_argument("-D", dest="defines", action="append", default=[])
+    ap.add_argument("--emit-only", action="store_true")
+    args = ap.parse_args(argv)
+    stem = os.path.splitext(os.path.basename(args.source))[0]
+    if args.emit_only:
+        translate(args.source, args.output or stem + ".hip", args.config, args.defines)
+        return 0
+    build(args.source, args.output or stem, args.config, args.defines)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

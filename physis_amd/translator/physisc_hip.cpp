@@ -1,0 +1,1073 @@
+// physisc-hip -- translates Physis DSL C programs to HIP for MI355X (gfx950).
+//
+// The HIP counterpart of the reference's `physisc --cuda` (translator/
+// physisc.cc:449-690 driving cuda_translator.cc and cuda_runtime_builder.cc).
+// ROSE is not available, so the front end is a restricted one for the DSL
+// subset (docs/programming.md:139-194): the system C preprocessor expands the
+// source, a tokenizer and brace matcher find the top-level functions, and
+// the stencil kernels (the functions passed to PSStencilMap) are analysed
+// token by token.
+//
+// Lowering (names follow the reference's generated code):
+//   PSGrid<r>D<T>New(n..[, attr])   -> __PSGridNewHip(sizeof(T), r, attr, n..)
+//                                      (reference_translator.cc:186-231)
+//   PSStencilMap(k, dom, args..)     -> __PSStencilMap_k(dom, args..) returning
+//                                      struct __PSStencil_k (reference_runtime_builder.cc:398-550)
+//   PSStencilRun(maps..[, iter])     -> __PSStencilRun_<id>(iter, maps..), iter
+//                                      defaulting to 1 (reference_translator.cc:1249-1283)
+//   PSReduce(&v, op, g)              -> __PSReduceGridFloat/Double(&v, op, g)
+//                                      (reference_translator.cc:1315-1351)
+//   kernel k                         -> template <class SF> struct __PSPlan_k, a
+//                                      Plan for the skeletons in include/physis/hip
+//   PSGridGet(g, x+dx, y+dy, z+dz)   -> LDS/register tap for the staged grid,
+//                                      else a load at the computed offset
+//   PSGridGetPeriodic(...)           -> same with (i + n) % n wrapping
+//   PSGridEmit(g, v)                 -> store at the current point
+//   PSGridDim(g, d)                  -> g.dim[d]
+// Stencil-range analysis (reference stencil_analysis.cc:248,
+// runtime_builder.cc:78-118) picks the staged grid and its halo radius; a
+// kernel that reads one float grid only at the 7-point star offsets and emits
+// one value becomes a star plan (vectorised skeleton, and two maps of a
+// ping-pong PSStencilRun fuse into one pass per two steps).
+//
+// Not supported (rejected with a message): user-defined point types,
+// red-black maps, kernel reductions, grids captured from globals.
+//
+// Usage: physisc-hip [--config FILE] [-o OUT] [-I DIR]... [-D NAME[=V]]... INPUT.c
+// Config keys (key = value lines): TRACE_KERNEL, KERNEL_ERROR_CHECK (bools),
+// FUSE_STEPS (bool, default true).
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cctype>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <algorithm>
+#include <fstream>
+#include <map>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+[[noreturn]] void Die(const std::string &msg) {
+  fprintf(stderr, "physisc-hip: error: %s\n", msg.c_str());
+  exit(1);
+}
+
+// ---------------------------------------------------------------- tokens --
+struct Tok {
+  enum Kind { kIdent, kNum, kStr, kChar, kPunct } kind;
+  std::string s;
+};
+
+bool IsIdentStart(char c) { return isalpha((unsigned char)c) || c == '_'; }
+bool IsIdentChar(char c) { return isalnum((unsigned char)c) || c == '_'; }
+
+std::vector<Tok> Tokenize(const std::string &src) {
+  static const char *kPuncts[] = {"<<=", ">>=", "...", "->", "++", "--", "<<", ">>", "<=", ">=", "==", "!=",
+                                  "&&",  "||",  "+=",  "-=", "*=", "/=", "%=", "&=", "|=", "^=", "##", "::"};
+  std::vector<Tok> out;
+  size_t i = 0, n = src.size();
+  while (i < n) {
+    char c = src[i];
+    if (isspace((unsigned char)c)) {
+      ++i;
+      continue;
+    }
+    if (c == '/' && i + 1 < n && src[i + 1] == '/') {
+      while (i < n && src[i] != '\n') ++i;
+      continue;
+    }
+    if (c == '/' && i + 1 < n && src[i + 1] == '*') {
+      size_t e = src.find("*/", i + 2);
+      i = e == std::string::npos ? n : e + 2;
+      continue;
+    }
+    if (c == '#') {  // leftover directive (e.g. #pragma): keep the line verbatim
+      size_t e = src.find('\n', i);
+      if (e == std::string::npos) e = n;
+      out.push_back({Tok::kPunct, src.substr(i, e - i) + "\n"});
+      i = e;
+      continue;
+    }
+    if (IsIdentStart(c)) {
+      size_t j = i;
+      while (j < n && IsIdentChar(src[j])) ++j;
+      out.push_back({Tok::kIdent, src.substr(i, j - i)});
+      i = j;
+      continue;
+    }
+    if (isdigit((unsigned char)c) || (c == '.' && i + 1 < n && isdigit((unsigned char)src[i + 1]))) {
+      size_t j = i;
+      while (j < n && (IsIdentChar(src[j]) || src[j] == '.' ||
+                       ((src[j] == '+' || src[j] == '-') && (src[j - 1] == 'e' || src[j - 1] == 'E') &&
+                        !(src[i] == '0' && i + 1 < n && (src[i + 1] == 'x' || src[i + 1] == 'X')))))
+        ++j;
+      out.push_back({Tok::kNum, src.substr(i, j - i)});
+      i = j;
+      continue;
+    }
+    if (c == '"' || c == '\'') {
+      size_t j = i + 1;
+      while (j < n && src[j] != c) j += (src[j] == '\\') ? 2 : 1;
+      out.push_back({c == '"' ? Tok::kStr : Tok::kChar, src.substr(i, j + 1 - i)});
+      i = j + 1;
+      continue;
+    }
+    bool matched = false;
+    for (const char *p : kPuncts) {
+      size_t l = strlen(p);
+      if (src.compare(i, l, p) == 0) {
+        out.push_back({Tok::kPunct, p});
+        i += l;
+        matched = true;
+        break;
+      }
+    }
+    if (!matched) {
+      out.push_back({Tok::kPunct, std::string(1, c)});
+      ++i;
+    }
+  }
+  return out;
+}
+
+size_t MatchClose(const std::vector<Tok> &t, size_t open) {
+  const std::string o = t[open].s;
+  const std::string c = o == "(" ? ")" : o == "[" ? "]" : "}";
+  int depth = 0;
+  for (size_t i = open; i < t.size(); ++i) {
+    if (t[i].kind != Tok::kPunct) continue;
+    if (t[i].s == o) ++depth;
+    else if (t[i].s == c && --depth == 0) return i;
+  }
+  Die("unbalanced '" + o + "'");
+}
+
+// Splits (b, e) at top-level commas into [begin, end) ranges.
+std::vector<std::pair<size_t, size_t>> SplitArgs(const std::vector<Tok> &t, size_t b, size_t e) {
+  std::vector<std::pair<size_t, size_t>> out;
+  if (b >= e) return out;
+  int depth = 0;
+  size_t start = b;
+  for (size_t i = b; i < e; ++i) {
+    if (t[i].kind != Tok::kPunct) continue;
+    const std::string &s = t[i].s;
+    if (s == "(" || s == "[" || s == "{") ++depth;
+    else if (s == ")" || s == "]" || s == "}") --depth;
+    else if (s == "," && depth == 0) {
+      out.push_back({start, i});
+      start = i + 1;
+    }
+  }
+  out.push_back({start, e});
+  return out;
+}
+
+std::string Text(const std::vector<Tok> &t, size_t b, size_t e) {
+  std::string out;
+  for (size_t i = b; i < e; ++i) {
+    if (!out.empty()) {
+      char p = out.back();
+      bool need = (IsIdentChar(p) || p == '.') && (IsIdentChar(t[i].s[0]) || t[i].s[0] == '.');
+      if (need || (t[i].kind != Tok::kPunct) || (t[i].s != ")" && t[i].s != "," && t[i].s != ";" &&
+                                                 t[i].s != "]" && p != '(' && p != '['))
+        out += ' ';
+    }
+    out += t[i].s;
+    if (t[i].s.size() > 1 && t[i].s[0] == '#') out += "";
+  }
+  return out;
+}
+
+// ------------------------------------------------------------ grid types --
+struct GridType {
+  int rank = 0;
+  std::string elem;  // float double int long
+};
+
+bool ParseGridType(const std::string &id, GridType *gt) {
+  // PSGrid<r>D<Float|Double|Int|Long>
+  if (id.size() < 10 || id.compare(0, 6, "PSGrid") != 0 || !isdigit((unsigned char)id[6]) || id[7] != 'D')
+    return false;
+  std::string rest = id.substr(8);
+  static const std::map<std::string, std::string> kElem = {
+      {"Float", "float"}, {"Double", "double"}, {"Int", "int"}, {"Long", "long"}};
+  auto it = kElem.find(rest);
+  if (it == kElem.end()) return false;
+  gt->rank = id[6] - '0';
+  gt->elem = it->second;
+  return gt->rank >= 1 && gt->rank <= 3;
+}
+
+bool ParseGridNew(const std::string &id, GridType *gt) {
+  if (id.size() < 4 || id.compare(id.size() - 3, 3, "New") != 0) return false;
+  return ParseGridType(id.substr(0, id.size() - 3), gt);
+}
+
+std::string StorageOf(const GridType &g) {
+  return g.elem == "float" ? "SF" : "psk::Native<" + g.elem + ">";
+}
+
+// ------------------------------------------------------------- functions --
+struct Param {
+  std::string type;  // text of the type tokens
+  std::string name;
+  bool is_grid = false;
+  GridType gt;
+};
+
+struct Func {
+  std::string name;
+  size_t begin;   // first token of the definition (return type)
+  size_t lparen, rparen, lbrace, rbrace;
+  std::vector<Param> params;
+};
+
+std::vector<Func> FindFunctions(const std::vector<Tok> &t) {
+  std::vector<Func> out;
+  size_t stmt_begin = 0;
+  for (size_t i = 0; i < t.size(); ++i) {
+    const Tok &k = t[i];
+    if (k.kind == Tok::kPunct && (k.s == ";" || (k.s.size() > 1 && k.s[0] == '#'))) {
+      stmt_begin = i + 1;
+      continue;
+    }
+    if (k.kind == Tok::kPunct && k.s == "{") {
+      // A top-level brace: function body if preceded by ") " (with optional
+      // attributes), else a struct/union/enum/initializer body.
+      size_t close = MatchClose(t, i);
+      size_t j = i;
+      while (j > stmt_begin && t[j - 1].kind == Tok::kIdent &&
+             (t[j - 1].s == "const" || t[j - 1].s == "__attribute__"))
+        --j;
+      if (j > stmt_begin && t[j - 1].s == ")") {
+        // find the matching '(' of the parameter list
+        int depth = 0;
+        size_t rp = j - 1, lp = rp;
+        for (size_t q = rp + 1; q-- > stmt_begin;) {
+          if (t[q].s == ")") ++depth;
+          else if (t[q].s == "(" && --depth == 0) {
+            lp = q;
+            break;
+          }
+        }
+        if (lp > stmt_begin && t[lp - 1].kind == Tok::kIdent) {
+          Func f;
+          f.name = t[lp - 1].s;
+          f.begin = stmt_begin;
+          f.lparen = lp;
+          f.rparen = rp;
+          f.lbrace = i;
+          f.rbrace = close;
+          for (auto &a : SplitArgs(t, lp + 1, rp)) {
+            if (a.first >= a.second) continue;
+            if (a.second - a.first == 1 && t[a.first].s == "void") continue;
+            Param p;
+            size_t name_at = a.second - 1;
+            while (name_at > a.first && t[name_at].kind != Tok::kIdent) --name_at;  // skip [] etc.
+            p.name = t[name_at].s;
+            p.type = Text(t, a.first, name_at);
+            for (size_t q = a.first; q < name_at; ++q)
+              if (ParseGridType(t[q].s, &p.gt)) p.is_grid = true;
+            f.params.push_back(p);
+          }
+          out.push_back(f);
+        }
+      }
+      i = close;
+      // A struct definition continues until ';'; a function ends here.
+      if (!out.empty() && out.back().rbrace == close) stmt_begin = close + 1;
+      continue;
+    }
+  }
+  return out;
+}
+
+// ----------------------------------------------------------------- config --
+struct Config {
+  bool trace = false;
+  bool error_check = false;
+  bool fuse = true;
+};
+
+Config LoadConfig(const std::string &path) {
+  Config c;
+  if (path.empty()) return c;
+  std::ifstream in(path);
+  if (!in) Die("cannot read config " + path);
+  std::string line;
+  while (std::getline(in, line)) {
+    size_t h = line.find_first_of("#-");
+    if (h != std::string::npos && (line[h] == '#' || (h + 1 < line.size() && line[h + 1] == '-')))
+      line = line.substr(0, h);
+    size_t eq = line.find('=');
+    if (eq == std::string::npos) continue;
+    auto trim = [](std::string s) {
+      size_t b = s.find_first_not_of(" \t\r"), e = s.find_last_not_of(" \t\r,;");
+      return b == std::string::npos ? std::string() : s.substr(b, e - b + 1);
+    };
+    std::string k = trim(line.substr(0, eq)), v = trim(line.substr(eq + 1));
+    bool on = v == "true" || v == "1" || v == "TRUE" || v == "True";
+    if (k == "TRACE_KERNEL") c.trace = on;
+    else if (k == "KERNEL_ERROR_CHECK" || k == "CUDA_KERNEL_ERROR_CHECK") c.error_check = on;
+    else if (k == "FUSE_STEPS") c.fuse = on;
+  }
+  return c;
+}
+
+// ----------------------------------------------------------------- kernels --
+struct Get {
+  std::string grid;
+  bool periodic = false;
+  bool relative = false;
+  int off[3] = {0, 0, 0};
+};
+
+struct Kernel {
+  const Func *f = nullptr;
+  int rank = 0;
+  std::vector<std::string> idx;           // index parameter names
+  std::vector<Param> grids, scalars;       // in parameter order
+  std::vector<Param> all;                  // all non-index params in order
+  std::map<std::string, GridType> gtype;
+  std::vector<Get> gets;
+  std::set<std::string> emitted;
+  int emit_count = 0;
+  std::string staged;   // grid staged in LDS ("" = none)
+  int rx = 0, ry = 0, rz = 0;
+  bool wrap = false;
+  bool star = false;    // vectorised star plan (compute() available)
+  std::string out_grid; // the single emitted grid when star
+};
+
+// Parses "x", "x + 3", "x - 1", "2 + x", "(x-1)" against index name `name`.
+bool RelativeIndex(const std::vector<Tok> &t, size_t b, size_t e, const std::string &name, int *off) {
+  while (e - b >= 2 && t[b].s == "(" && MatchClose(t, b) == e - 1) {
+    ++b;
+    --e;
+  }
+  if (e - b == 1 && t[b].kind == Tok::kIdent && t[b].s == name) {
+    *off = 0;
+    return true;
+  }
+  auto num = [&](size_t i, int *v) {
+    if (t[i].kind != Tok::kNum) return false;
+    char *end = nullptr;
+    long x = strtol(t[i].s.c_str(), &end, 0);
+    if (*end != '\0') return false;
+    *v = (int)x;
+    return true;
+  };
+  int v;
+  if (e - b == 3 && t[b].kind == Tok::kIdent && t[b].s == name && (t[b + 1].s == "+" || t[b + 1].s == "-") &&
+      num(b + 2, &v)) {
+    *off = t[b + 1].s == "+" ? v : -v;
+    return true;
+  }
+  if (e - b == 3 && num(b, &v) && t[b + 1].s == "+" && t[b + 2].kind == Tok::kIdent && t[b + 2].s == name) {
+    *off = v;
+    return true;
+  }
+  return false;
+}
+
+class Translator {
+ public:
+  Translator(std::vector<Tok> toks, Config cfg, std::string input)
+      : t_(std::move(toks)), cfg_(cfg), input_(std::move(input)) {}
+
+  std::string Run(const std::vector<std::string> &includes);
+
+ private:
+  void CollectGridVars();
+  void AnalyzeKernel(Kernel &k);
+  std::string RewriteBody(const Kernel &k, size_t b, size_t e, bool compute_form);
+  std::string EmitPlan(const Kernel &k);
+  std::string EmitMapAndLaunch(const Kernel &k);
+  std::string RewriteHost(size_t b, size_t e);
+  std::string EmitRun(const std::vector<std::pair<std::string, std::string>> &maps, int id);
+
+  std::vector<Tok> t_;
+  Config cfg_;
+  std::string input_;
+  std::vector<Func> funcs_;
+  std::map<std::string, Kernel> kernels_;
+  std::map<std::string, GridType> gridvars_;
+  std::vector<std::string> runs_;  // emitted run functions
+  int run_id_ = 0;
+};
+
+void Translator::CollectGridVars() {
+  for (size_t i = 0; i + 1 < t_.size(); ++i) {
+    GridType gt;
+    if (t_[i].kind == Tok::kIdent && ParseGridType(t_[i].s, &gt)) {
+      size_t j = i + 1;
+      while (j < t_.size() && (t_[j].s == "*" || t_[j].s == "const")) ++j;
+      while (j < t_.size() && t_[j].kind == Tok::kIdent) {
+        gridvars_[t_[j].s] = gt;
+        // more declarators: ", name"
+        size_t q = j + 1;
+        if (q < t_.size() && t_[q].s == "=") {
+          int depth = 0;
+          while (q < t_.size()) {
+            const std::string &s = t_[q].s;
+            if (s == "(" || s == "[" || s == "{") ++depth;
+            else if (s == ")" || s == "]" || s == "}") { if (--depth < 0) break; }
+            else if ((s == "," || s == ";") && depth == 0) break;
+            ++q;
+          }
+        }
+        if (q < t_.size() && t_[q].s == "," && q + 1 < t_.size() && t_[q + 1].kind == Tok::kIdent) {
+          j = q + 1;
+          continue;
+        }
+        break;
+      }
+    }
+  }
+}
+
+void Translator::AnalyzeKernel(Kernel &k) {
+  const Func &f = *k.f;
+  size_t p = 0;
+  for (; p < f.params.size(); ++p) {
+    const Param &pr = f.params[p];
+    if (pr.is_grid) break;
+    if (pr.type.find("int") == std::string::npos) break;
+    k.idx.push_back(pr.name);
+  }
+  k.rank = (int)k.idx.size();
+  if (k.rank < 1 || k.rank > 3) Die("kernel " + f.name + ": expected 1-3 leading int index parameters");
+  for (; p < f.params.size(); ++p) {
+    const Param &pr = f.params[p];
+    k.all.push_back(pr);
+    if (pr.is_grid) {
+      k.grids.push_back(pr);
+      k.gtype[pr.name] = pr.gt;
+    } else {
+      if (pr.type.find("PSGrid") != std::string::npos)
+        Die("kernel " + f.name + ": user-defined point types are not supported by this target");
+      k.scalars.push_back(pr);
+    }
+  }
+  // Scan accesses.
+  std::map<std::string, int> nonzero;  // gets at non-zero offsets per grid
+  std::map<std::string, bool> any_abs;
+  for (size_t i = f.lbrace; i < f.rbrace; ++i) {
+    if (t_[i].kind != Tok::kIdent || i + 1 >= f.rbrace || t_[i + 1].s != "(") continue;
+    const std::string &name = t_[i].s;
+    if (name == "PSGridGet" || name == "PSGridGetPeriodic") {
+      size_t close = MatchClose(t_, i + 1);
+      auto args = SplitArgs(t_, i + 2, close);
+      if (args.empty()) Die("kernel " + f.name + ": empty PSGridGet");
+      std::string g = Text(t_, args[0].first, args[0].second);
+      auto it = k.gtype.find(g);
+      if (it == k.gtype.end()) Die("kernel " + f.name + ": PSGridGet on unknown grid '" + g + "'");
+      if ((int)args.size() != it->second.rank + 1)
+        Die("kernel " + f.name + ": PSGridGet on '" + g + "' has the wrong number of indices");
+      Get acc;
+      acc.grid = g;
+      acc.periodic = name == "PSGridGetPeriodic";
+      acc.relative = it->second.rank <= k.rank;
+      for (int d = 0; d < it->second.rank && acc.relative; ++d)
+        acc.relative = RelativeIndex(t_, args[d + 1].first, args[d + 1].second, k.idx[d], &acc.off[d]);
+      k.gets.push_back(acc);
+      if (!acc.relative) any_abs[g] = true;
+    } else if (name == "PSGridEmit") {
+      size_t close = MatchClose(t_, i + 1);
+      auto args = SplitArgs(t_, i + 2, close);
+      std::string g = Text(t_, args[0].first, args[0].second);
+      if (!k.gtype.count(g)) Die("kernel " + f.name + ": PSGridEmit on unknown grid '" + g + "'");
+      k.emitted.insert(g);
+      ++k.emit_count;
+    } else if (name == "PSGridEmitUtype" || name == "PSGridSet") {
+      Die("kernel " + f.name + ": " + name + " is not supported by this target");
+    }
+  }
+  // Staging: the grid of the kernel's rank and element type read at the most
+  // non-zero relative offsets (radius <= 2, not emitted).
+  std::map<std::string, int> count;
+  for (const Get &g : k.gets) {
+    if (!g.relative) continue;
+    const GridType &gt = k.gtype[g.grid];
+    if (gt.rank != k.rank || k.emitted.count(g.grid)) continue;
+    if (g.off[0] || g.off[1] || g.off[2]) count[g.grid]++;
+  }
+  int best = 0;
+  for (auto &c : count) {
+    int r = 0;
+    for (const Get &g : k.gets)
+      if (g.grid == c.first && g.relative)
+        for (int d = 0; d < 3; ++d) r = std::max(r, std::abs(g.off[d]));
+    if (r <= 2 && c.second > best) {
+      best = c.second;
+      k.staged = c.first;
+    }
+  }
+  if (!k.staged.empty()) {
+    for (const Get &g : k.gets) {
+      if (g.grid != k.staged || !g.relative) continue;
+      k.rx = std::max(k.rx, std::abs(g.off[0]));
+      k.ry = std::max(k.ry, std::abs(g.off[1]));
+      k.rz = std::max(k.rz, std::abs(g.off[2]));
+      if (g.periodic) k.wrap = true;
+    }
+    if (k.rank < 3) k.rz = 0;
+    if (k.rank < 2) k.ry = 0;
+  }
+  // Star form: rank 3, staged float grid with offsets in the 7-point star,
+  // no absolute reads of it, one unconditional emit as the last statement.
+  if (k.rank == 3 && !k.staged.empty() && k.gtype[k.staged].elem == "float" && !any_abs[k.staged] &&
+      k.emitted.size() == 1 && k.emit_count == 1) {
+    bool in_star = true;
+    for (const Get &g : k.gets) {
+      if (g.grid != k.staged) continue;
+      int nz = (g.off[0] != 0) + (g.off[1] != 0) + (g.off[2] != 0);
+      if (nz > 1 || std::abs(g.off[0]) > 1 || std::abs(g.off[1]) > 1 || std::abs(g.off[2]) > 1) in_star = false;
+    }
+    // The emit statement must sit at the top level of the body, followed
+    // only by an optional "return;".
+    size_t emit_at = 0;
+    int depth = 0;
+    bool top = false;
+    for (size_t i = f.lbrace + 1; i < f.rbrace; ++i) {
+      const std::string &s = t_[i].s;
+      if (s == "{") ++depth;
+      else if (s == "}") --depth;
+      else if (s == "PSGridEmit") {
+        emit_at = i;
+        top = depth == 0;
+      } else if (s == "return" && i + 1 < f.rbrace && t_[i + 1].s != ";") {
+        top = false;  // value returns are not part of the DSL
+      }
+    }
+    bool tail_ok = false;
+    if (top && emit_at) {
+      size_t close = MatchClose(t_, emit_at + 1);
+      size_t q = close + 1;
+      if (q < f.rbrace && t_[q].s == ";") ++q;
+      if (q < f.rbrace && t_[q].s == "return" && q + 1 < f.rbrace && t_[q + 1].s == ";") q += 2;
+      tail_ok = q == f.rbrace;
+      // no early "return;" before the emit
+      for (size_t i = f.lbrace + 1; i < emit_at; ++i)
+        if (t_[i].s == "return") tail_ok = false;
+    }
+    const GridType &og = k.gtype[*k.emitted.begin()];
+    if (in_star && tail_ok && (k.rx | k.ry | k.rz) <= 1 && og.elem == "float" && og.rank == 3) {
+      k.star = true;
+      k.out_grid = *k.emitted.begin();
+    }
+  }
+}
+
+std::string Translator::RewriteBody(const Kernel &k, size_t b, size_t e, bool compute_form) {
+  std::string out;
+  int pdepth = 0;
+  auto add = [&](const std::string &s) {
+    if (!out.empty() && IsIdentChar(out.back()) && !s.empty() && IsIdentChar(s[0])) out += ' ';
+    out += s;
+  };
+  for (size_t i = b; i < e; ++i) {
+    const Tok &tk = t_[i];
+    if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" &&
+        (tk.s == "PSGridGet" || tk.s == "PSGridGetPeriodic" || tk.s == "PSGridEmit" || tk.s == "PSGridDim")) {
+      size_t close = MatchClose(t_, i + 1);
+      auto args = SplitArgs(t_, i + 2, close);
+      std::string g = Text(t_, args[0].first, args[0].second);
+      const GridType &gt = k.gtype.at(g);
+      std::vector<std::string> a;
+      for (size_t q = 1; q < args.size(); ++q) a.push_back(RewriteBody(k, args[q].first, args[q].second, false));
+      std::ostringstream r;
+      if (tk.s == "PSGridDim") {
+        r << "(" << g << ".dim[" << a[0] << "])";
+      } else if (tk.s == "PSGridEmit") {
+        std::string at = "psk::Off" + std::to_string(gt.rank) + "(" + g;
+        for (int d = 0; d < gt.rank; ++d) at += ", " + k.idx[std::min(d, k.rank - 1)];
+        at += ")";
+        if (compute_form) r << "return (" << gt.elem << ")(" << a[0] << ")";
+        else r << g << ".store(" << at << ", (" << gt.elem << ")(" << a[0] << "))";
+      } else {
+        Get acc;
+        acc.relative = gt.rank <= k.rank;
+        for (int d = 0; d < gt.rank && acc.relative; ++d)
+          acc.relative = RelativeIndex(t_, args[d + 1].first, args[d + 1].second, k.idx[d], &acc.off[d]);
+        if (g == k.staged && acc.relative) {
+          r << "((" << gt.elem << ")__ctx.at(0, " << acc.off[0] << ", " << acc.off[1] << ", " << acc.off[2]
+            << "))";
+        } else {
+          r << g << ".load(psk::Off" << (tk.s == "PSGridGetPeriodic" ? "P" : "") << gt.rank << "(" << g;
+          for (auto &s : a) r << ", " << s;
+          r << "))";
+        }
+      }
+      add(r.str());
+      i = close;
+      continue;
+    }
+    if (compute_form && tk.kind == Tok::kIdent && tk.s == "return" && i + 1 < e && t_[i + 1].s == ";") {
+      ++i;  // the trailing "return;" of a star kernel
+      continue;
+    }
+    if (tk.kind == Tok::kPunct && tk.s == "(") ++pdepth;
+    if (tk.kind == Tok::kPunct && tk.s == ")") --pdepth;
+    if (tk.kind == Tok::kPunct && ((tk.s == ";" && pdepth == 0) || tk.s == "{" || tk.s == "}")) {
+      add(tk.s);
+      out += "\n";
+      continue;
+    }
+    if (!out.empty() && out.back() != '\n' && out.back() != '(' && tk.s != ")" && tk.s != "," && tk.s != ";")
+      out += ' ';
+    add(tk.s);
+  }
+  return out;
+}
+
+std::string Translator::EmitPlan(const Kernel &k) {
+  const Func &f = *k.f;
+  std::ostringstream o;
+  const std::string P = "__PSPlan_" + f.name;
+  std::string sv = k.staged.empty() ? "float" : k.gtype.at(k.staged).elem;
+  o << "// Plan for kernel " << f.name << " (rank " << k.rank << ", staged grid: "
+    << (k.staged.empty() ? "none" : k.staged) << ", radius " << k.rx << "/" << k.ry << "/" << k.rz
+    << (k.star ? ", star form" : "") << ")\n";
+  o << "template <class SF>\nstruct " << P << " {\n";
+  o << "  using SV = " << sv << ";\n";
+  o << "  static constexpr int NS = " << (k.staged.empty() ? 0 : 1) << ", RX = " << k.rx << ", RY = " << k.ry
+    << ", RZ = " << k.rz << ";\n";
+  o << "  static constexpr bool kWrap = " << (k.wrap ? "true" : "false") << ";\n";
+  o << "  static constexpr bool kReduce = false;\n";
+  o << "  struct Args {\n";
+  for (const Param &p : k.all) {
+    if (p.is_grid) o << "    psk::GridRef<" << StorageOf(p.gt) << "> " << p.name << ";\n";
+    else o << "    " << p.type << " " << p.name << ";\n";
+  }
+  o << "  };\n";
+  if (!k.staged.empty()) {
+    const GridType &sg = k.gtype.at(k.staged);
+    o << "  using Storage = " << StorageOf(sg) << ";\n";
+    o << "  __device__ static void stage_dims(const Args &a, int &nx, int &ny, int &nz) {\n"
+      << "    nx = a." << k.staged << ".dim[0]; ny = a." << k.staged << ".dim[1]; nz = a." << k.staged
+      << ".dim[2];\n  }\n";
+    o << "  __device__ static SV stage_load(const Args &a, int, int64_t off) { return a." << k.staged
+      << ".load(off); }\n";
+    o << "  __host__ __device__ static const typename Storage::stored_t *stage_ptr(const Args &a) { return a."
+      << k.staged << ".p; }\n";
+    o << "  static void set_in(Args &a, const typename Storage::stored_t *p) { a." << k.staged
+      << ".p = const_cast<typename Storage::stored_t *>(p); }\n";
+  }
+  if (k.star) {
+    o << "  __host__ __device__ static typename Storage::stored_t *out_ptr(const Args &a) { return a."
+      << k.out_grid << ".p; }\n";
+    o << "  static void set_out(Args &a, typename Storage::stored_t *p) { a." << k.out_grid << ".p = p; }\n";
+  }
+  auto prologue = [&]() {
+    std::ostringstream q;
+    for (const Param &p : k.all) {
+      if (p.is_grid) q << "    const auto &" << p.name << " = __a." << p.name << ";\n";
+      else q << "    " << p.type << " " << p.name << " = __a." << p.name << ";\n";
+    }
+    for (const Param &p : k.all) q << "    (void)" << p.name << ";\n";
+    return q.str();
+  };
+  std::string params = "const C &__ctx, const Args &__a";
+  const char *dummies[3] = {"__i0", "__i1", "__i2"};
+  for (int d = 0; d < 3; ++d) params += ", const int " + (d < k.rank ? k.idx[d] : std::string(dummies[d]));
+  o << "  template <class C>\n  __device__ static void point(" << params << ") {\n";
+  o << "    (void)__ctx;\n" << prologue();
+  o << "    {\n" << RewriteBody(k, f.lbrace + 1, f.rbrace, false) << "    }\n  }\n";
+  if (k.star) {
+    o << "  template <class C>\n  __device__ static float compute(" << params << ") {\n";
+    o << "    (void)__ctx;\n" << prologue();
+    o << "    {\n" << RewriteBody(k, f.lbrace + 1, f.rbrace, true) << "    }\n  }\n";
+  }
+  o << "};\n\n";
+  return o.str();
+}
+
+std::string Translator::EmitMapAndLaunch(const Kernel &k) {
+  const Func &f = *k.f;
+  std::ostringstream o;
+  const std::string S = "__PSStencil_" + f.name, P = "__PSPlan_" + f.name;
+  // The stencil object (reference_runtime_builder.cc:398-448): domain, each
+  // grid with its id, then the scalars.
+  o << "struct " << S << " {\n  __PSDomain dom;\n";
+  for (const Param &p : k.all) {
+    if (p.is_grid) o << "  __PSGrid *" << p.name << ";\n  int " << p.name << "_index;\n";
+    else o << "  " << p.type << " " << p.name << ";\n";
+  }
+  o << "};\n\n";
+  o << "static struct " << S << " __PSStencilMap_" << f.name << "(__PSDomain dom";
+  for (const Param &p : k.all) o << ", " << (p.is_grid ? "__PSGrid *" : p.type + " ") << p.name;
+  o << ") {\n  struct " << S << " s;\n  s.dom = dom;\n";
+  for (const Param &p : k.all) {
+    if (p.is_grid) o << "  s." << p.name << " = " << p.name << ";\n  s." << p.name << "_index = __PSGridGetID("
+                     << p.name << ");\n";
+    else o << "  s." << p.name << " = " << p.name << ";\n";
+  }
+  o << "  return s;\n}\n\n";
+  // Typed args.
+  o << "template <class SF>\nstatic typename " << P << "<SF>::Args __PSArgs_" << f.name << "(const struct " << S
+    << " *s) {\n  typename " << P << "<SF>::Args a;\n";
+  for (const Param &p : k.all) {
+    if (p.is_grid) o << "  a." << p.name << " = psk::MakeRef<" << (p.gt.elem == "float" ? "SF" : StorageOf(p.gt))
+                     << ">(s->" << p.name << ");\n";
+    else o << "  a." << p.name << " = s->" << p.name << ";\n";
+  }
+  o << "  return a;\n}\n\n";
+  // One map application.
+  o << "template <class SF>\nstatic void __PSLaunchT_" << f.name << "(const struct " << S
+    << " *s, hipStream_t st) {\n";
+  o << "  typename " << P << "<SF>::Args a = __PSArgs_" << f.name << "<SF>(s);\n";
+  o << "  psk::Box box = psk::BoxFromDomain(s->dom, " << k.rank << ");\n";
+  if (k.star)
+    o << "  psk::CheckLaunch(psk::LaunchStarStep<" << P << "<SF>>(a, box, a." << k.staged << ".dim[0], st), \""
+      << f.name << "\");\n";
+  else
+    o << "  psk::CheckLaunch(psk::LaunchRing<" << P << "<SF>, " << k.rank << ">(a, box, st), \"" << f.name
+      << "\");\n";
+  o << "}\n\n";
+  // Storage dispatch.
+  std::vector<std::string> floats;
+  for (const Param &p : k.all)
+    if (p.is_grid && p.gt.elem == "float") floats.push_back(p.name);
+  o << "static int __PSStorage_" << f.name << "(const struct " << S << " *s) {\n";
+  if (floats.empty()) {
+    o << "  (void)s;\n  return PS_HIP_STORAGE_NATIVE;\n";
+  } else {
+    o << "  const __PSGrid *g[] = {";
+    for (size_t i = 0; i < floats.size(); ++i) o << (i ? ", " : "") << "s->" << floats[i];
+    o << "};\n  return psk::FloatStorage(g, " << floats.size() << ");\n";
+  }
+  o << "}\n\n";
+  o << "static void __PSLaunch_" << f.name << "(const struct " << S << " *s) {\n";
+  o << "  hipStream_t st = (hipStream_t)__PSGetStream();\n";
+  o << "  switch (__PSStorage_" << f.name << "(s)) {\n";
+  o << "    case PS_HIP_STORAGE_BF16: __PSLaunchT_" << f.name << "<psk::BF16>(s, st); break;\n";
+  o << "    case PS_HIP_STORAGE_FP16: __PSLaunchT_" << f.name << "<psk::FP16>(s, st); break;\n";
+  o << "    default: __PSLaunchT_" << f.name << "<psk::Native<float>>(s, st); break;\n";
+  o << "  }\n";
+  if (cfg_.error_check) o << "  __PSCheckDeviceError(\"" << f.name << "\");\n";
+  o << "}\n\n";
+  if (k.star) {
+    // Ping-pong fusion: two maps of this kernel that differ only by swapping
+    // the staged and the emitted grid, over the whole grid.
+    o << "static int __PSFusable_" << f.name << "(const struct " << S << " *a, const struct " << S << " *b) {\n";
+    o << "  if (a->" << k.staged << " != b->" << k.out_grid << " || a->" << k.out_grid << " != b->" << k.staged
+      << " || a->" << k.staged << " == a->" << k.out_grid << ") return 0;\n";
+    for (const Param &p : k.all) {
+      if (p.name == k.staged || p.name == k.out_grid) continue;
+      if (p.is_grid) o << "  if (a->" << p.name << " != b->" << p.name << ") return 0;\n";
+      else o << "  if (memcmp(&a->" << p.name << ", &b->" << p.name << ", sizeof(a->" << p.name
+             << ")) != 0) return 0;\n";
+    }
+    o << "  const __PSGrid *g = a->" << k.staged << ";\n";
+    o << "  for (int d = 0; d < 3; ++d) {\n"
+      << "    if (a->dom.local_min[d] != 0 || a->dom.local_max[d] != g->dim[d]) return 0;\n"
+      << "    if (b->dom.local_min[d] != 0 || b->dom.local_max[d] != g->dim[d]) return 0;\n  }\n";
+    o << "  return g->dim[0] % 4 == 0 && g->dim[0] <= 1024;\n}\n\n";
+    o << "template <class SF>\nstatic void __PSRunPingPongT_" << f.name << "(const struct " << S
+      << " *a, const struct " << S << " *b, int iter, hipStream_t st) {\n";
+    o << "  const __PSGrid *g = a->" << k.staged << ";\n";
+    o << "  size_t bytes = (size_t)g->num_elms * PSHipGridStoredElmSize(g);\n";
+    o << "  psk::RunPingPong<" << P << "<SF>>(__PSArgs_" << f.name << "<SF>(a), __PSArgs_" << f.name
+      << "<SF>(b), iter, g->dim[0], g->dim[1], g->dim[2], bytes, st);\n}\n\n";
+    o << "static void __PSRunPingPong_" << f.name << "(const struct " << S << " *a, const struct " << S
+      << " *b, int iter) {\n";
+    o << "  hipStream_t st = (hipStream_t)__PSGetStream();\n";
+    o << "  switch (__PSStorage_" << f.name << "(a)) {\n";
+    o << "    case PS_HIP_STORAGE_BF16: __PSRunPingPongT_" << f.name << "<psk::BF16>(a, b, iter, st); break;\n";
+    o << "    case PS_HIP_STORAGE_FP16: __PSRunPingPongT_" << f.name << "<psk::FP16>(a, b, iter, st); break;\n";
+    o << "    default: __PSRunPingPongT_" << f.name << "<psk::Native<float>>(a, b, iter, st); break;\n";
+    o << "  }\n";
+    if (cfg_.error_check) o << "  __PSCheckDeviceError(\"" << f.name << "\");\n";
+    o << "}\n\n";
+  }
+  return o.str();
+}
+
+std::string Translator::EmitRun(const std::vector<std::pair<std::string, std::string>> &maps, int id) {
+  // maps: (kernel name, argument text of the map call)
+  std::ostringstream o;
+  o << "static float __PSStencilRun_" << id << "(int iter";
+  for (size_t i = 0; i < maps.size(); ++i) o << ", struct __PSStencil_" << maps[i].first << " s" << i;
+  o << ") {\n";
+  if (cfg_.trace) {
+    o << "  __PSTraceStencilPre(\"";
+    for (auto &m : maps) o << m.first;
+    o << "\");\n  __PSStopwatch __st;\n  __PSStopwatchStart(&__st);\n";
+  }
+  bool fusable = cfg_.fuse && maps.size() == 2 && maps[0].first == maps[1].first &&
+                 kernels_.at(maps[0].first).star;
+  if (fusable) {
+    const std::string &kn = maps[0].first;
+    o << "  if (iter > 1 && __PSFusable_" << kn << "(&s0, &s1)) {\n";
+    o << "    __PSRunPingPong_" << kn << "(&s0, &s1, iter);\n  } else {\n";
+  }
+  o << "  for (int i = 0; i < iter; ++i) {\n";
+  for (size_t i = 0; i < maps.size(); ++i) o << "    __PSLaunch_" << maps[i].first << "(&s" << i << ");\n";
+  o << "  }\n";
+  if (fusable) o << "  }\n";
+  if (cfg_.trace) {
+    o << "  (void)hipStreamSynchronize((hipStream_t)__PSGetStream());\n";
+    o << "  float f = __PSStopwatchStop(&__st);\n  __PSTraceStencilPost(f);\n  return f;\n";
+  } else {
+    o << "  return 0.0f;\n";
+  }
+  o << "}\n\n";
+  (void)id;
+  return o.str();
+}
+
+std::string Translator::RewriteHost(size_t b, size_t e) {
+  std::string out;
+  int pdepth = 0;
+  auto add = [&](const std::string &s) {
+    if (!out.empty()) {
+      char p = out.back();
+      if ((IsIdentChar(p) && IsIdentChar(s[0])) || (p != '\n' && p != '(' && p != '[' && s != ")" && s != "," &&
+                                                    s != ";" && s != "]" && s != "(" && s != "[" && s != "."))
+        out += ' ';
+    }
+    out += s;
+  };
+  for (size_t i = b; i < e; ++i) {
+    const Tok &tk = t_[i];
+    if (tk.kind == Tok::kPunct && tk.s.size() > 1 && tk.s[0] == '#') {
+      if (!out.empty() && out.back() != '\n') out += "\n";
+      out += tk.s;
+      continue;
+    }
+    GridType gt;
+    if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" && ParseGridNew(tk.s, &gt)) {
+      size_t close = MatchClose(t_, i + 1);
+      auto args = SplitArgs(t_, i + 2, close);
+      if ((int)args.size() < gt.rank) Die(tk.s + ": expected " + std::to_string(gt.rank) + " extents");
+      std::string attr = (int)args.size() > gt.rank ? RewriteHost(args[gt.rank].first, args[gt.rank].second) : "0";
+      std::string r = "__PSGridNewHip(sizeof(" + gt.elem + "), " + std::to_string(gt.rank) + ", " + attr;
+      for (int d = 0; d < 3; ++d)
+        r += ", " + (d < gt.rank ? "(" + RewriteHost(args[d].first, args[d].second) + ")" : std::string("1"));
+      r += ")";
+      add(r);
+      i = close;
+      continue;
+    }
+    if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" && tk.s == "PSReduce") {
+      size_t close = MatchClose(t_, i + 1);
+      auto args = SplitArgs(t_, i + 2, close);
+      if (args.size() != 3) Die("PSReduce with a kernel is not supported by this target");
+      std::string g = Text(t_, args[2].first, args[2].second);
+      auto it = gridvars_.find(g);
+      if (it == gridvars_.end()) Die("PSReduce: unknown grid '" + g + "'");
+      std::string fn;
+      if (it->second.elem == "float") fn = "__PSReduceGridFloat";
+      else if (it->second.elem == "double") fn = "__PSReduceGridDouble";
+      else Die("PSReduce: unsupported element type of '" + g + "'");
+      add(fn + "(" + RewriteHost(args[0].first, args[0].second) + ", " +
+          RewriteHost(args[1].first, args[1].second) + ", " + g + ")");
+      i = close;
+      continue;
+    }
+    if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" &&
+        (tk.s == "PSStencilMapRedBlack" || tk.s == "PSStencilMapRed" || tk.s == "PSStencilMapBlack"))
+      Die(tk.s + " is not supported by this target");
+    if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" && tk.s == "PSStencilRun") {
+      size_t close = MatchClose(t_, i + 1);
+      auto args = SplitArgs(t_, i + 2, close);
+      std::vector<std::pair<std::string, std::string>> maps;
+      std::string iter = "1";
+      for (size_t a = 0; a < args.size(); ++a) {
+        size_t ab = args[a].first, ae = args[a].second;
+        if (t_[ab].s == "PSStencilMap" && ab + 1 < ae && t_[ab + 1].s == "(") {
+          size_t mc = MatchClose(t_, ab + 1);
+          auto margs = SplitArgs(t_, ab + 2, mc);
+          std::string kn = Text(t_, margs[0].first, margs[0].second);
+          if (!kernels_.count(kn)) Die("PSStencilMap of unknown kernel '" + kn + "'");
+          std::string call = "__PSStencilMap_" + kn + "(";
+          for (size_t q = 1; q < margs.size(); ++q)
+            call += (q > 1 ? ", " : "") + RewriteHost(margs[q].first, margs[q].second);
+          call += ")";
+          maps.push_back({kn, call});
+        } else if (a + 1 == args.size()) {
+          iter = RewriteHost(ab, ae);  // trailing count (reference run.cc:60-72)
+        } else {
+          Die("PSStencilRun arguments must be PSStencilMap(...) calls");
+        }
+      }
+      int id = run_id_++;
+      runs_.push_back(EmitRun(maps, id));
+      std::string call = "__PSStencilRun_" + std::to_string(id) + "(" + iter;
+      for (auto &m : maps) call += ", " + m.second;
+      call += ")";
+      add(call);
+      i = close;
+      continue;
+    }
+    if (tk.kind == Tok::kPunct && tk.s == "(") ++pdepth;
+    if (tk.kind == Tok::kPunct && tk.s == ")") --pdepth;
+    if (tk.kind == Tok::kPunct && ((tk.s == ";" && pdepth == 0) || tk.s == "{" || tk.s == "}")) {
+      add(tk.s);
+      out += "\n";
+      continue;
+    }
+    add(tk.s);
+  }
+  return out;
+}
+
+std::string Translator::Run(const std::vector<std::string> &includes) {
+  funcs_ = FindFunctions(t_);
+  CollectGridVars();
+  std::map<std::string, const Func *> byname;
+  for (const Func &f : funcs_) byname[f.name] = &f;
+  // Kernels: first argument of every PSStencilMap.
+  for (size_t i = 0; i + 2 < t_.size(); ++i) {
+    if (t_[i].kind == Tok::kIdent && t_[i + 1].s == "(" &&
+        (t_[i].s == "PSStencilMap" || t_[i].s == "PSStencilMapRedBlack" || t_[i].s == "PSStencilMapRed" ||
+         t_[i].s == "PSStencilMapBlack")) {
+      if (t_[i].s != "PSStencilMap") Die(t_[i].s + " is not supported by this target");
+      const std::string &kn = t_[i + 2].s;
+      auto it = byname.find(kn);
+      if (it == byname.end()) Die("kernel '" + kn + "' is not defined in this file");
+      if (!kernels_.count(kn)) {
+        Kernel k;
+        k.f = it->second;
+        AnalyzeKernel(k);
+        kernels_[kn] = k;
+      }
+    }
+  }
+  // Emit: header, user code with kernels replaced by the generated code
+  // (inserted where the first kernel was defined).
+  std::ostringstream o;
+  o << "// Generated by physisc-hip from " << input_ << ". Do not edit.\n";
+  o << "#define PHYSIS_HIP 1\n";
+  for (const std::string &inc : includes) o << inc << "\n";
+  o << "#include \"physis/physis.h\"\n#include \"physis/hip/generated.h\"\n\n";
+  size_t first_kernel = t_.size();
+  std::set<size_t> kernel_starts;
+  for (auto &kv : kernels_) {
+    first_kernel = std::min(first_kernel, kv.second.f->begin);
+    kernel_starts.insert(kv.second.f->begin);
+  }
+  std::string generated;
+  for (auto &kv : kernels_) generated += EmitPlan(kv.second) + EmitMapAndLaunch(kv.second);
+  // Host code, skipping kernel definitions (and their prototypes are harmless).
+  std::string host_before, host_after;
+  size_t pos = 0;
+  std::vector<std::pair<size_t, size_t>> skip;
+  for (auto &kv : kernels_) skip.push_back({kv.second.f->begin, kv.second.f->rbrace + 1});
+  std::sort(skip.begin(), skip.end());
+  std::string *dst = &host_before;
+  for (auto &sk : skip) {
+    *dst += RewriteHost(pos, sk.first);
+    if (sk.first == first_kernel) dst = &host_after;
+    pos = sk.second;
+  }
+  *dst += RewriteHost(pos, t_.size());
+  o << host_before << "\n" << generated;
+  for (const std::string &r : runs_) o << r;
+  o << host_after << "\n";
+  return o.str();
+}
+
+std::string ReadFile(const std::string &p) {
+  std::ifstream in(p);
+  if (!in) Die("cannot read " + p);
+  std::stringstream ss;
+  ss << in.rdbuf();
+  return ss.str();
+}
+
+// Runs the system C preprocessor over `src` with every #include replaced by
+// an empty stand-in, so only the program's own macros are expanded; returns
+// the expanded text and the include lines (re-emitted verbatim).
+std::string Preprocess(const std::string &path, const std::vector<std::string> &defs,
+                       std::vector<std::string> *includes) {
+  std::string src = ReadFile(path);
+  std::string tmpl = "/tmp/physisc-hip-XXXXXX";
+  std::vector<char> dir(tmpl.begin(), tmpl.end());
+  dir.push_back('\0');
+  if (!mkdtemp(dir.data())) Die("mkdtemp failed");
+  std::string tmp(dir.data());
+  std::istringstream lines(src);
+  std::string line, stripped;
+  while (std::getline(lines, line)) {
+    size_t b = line.find_first_not_of(" \t");
+    if (b != std::string::npos && line[b] == '#') {
+      size_t d = line.find_first_not_of(" \t", b + 1);
+      if (d != std::string::npos && line.compare(d, 7, "include") == 0) {
+        size_t q1 = line.find_first_of("<\"", d + 7);
+        if (q1 != std::string::npos) {
+          char close = line[q1] == '<' ? '>' : '"';
+          size_t q2 = line.find(close, q1 + 1);
+          std::string name = line.substr(q1 + 1, q2 - q1 - 1);
+          if (name != "physis/physis.h") includes->push_back(line.substr(b));
+          // empty stand-in under tmp/
+          std::string full = tmp + "/" + name;
+          for (size_t s = full.find('/', tmp.size() + 1); s != std::string::npos; s = full.find('/', s + 1))
+            mkdir(full.substr(0, s).c_str(), 0700);
+          FILE *f = fopen(full.c_str(), "w");
+          if (f) fclose(f);
+        }
+      }
+    }
+    stripped += line + "\n";
+  }
+  std::string in_path = tmp + "/input.c";
+  {
+    std::ofstream f(in_path);
+    f << stripped;
+  }
+  std::string cmd = "gcc -E -P -nostdinc -DPHYSIS_USER -I" + tmp;
+  for (const std::string &d : defs) cmd += " '-D" + d + "'";
+  cmd += " " + in_path + " 2>&1";
+  FILE *p = popen(cmd.c_str(), "r");
+  if (!p) Die("cannot run the C preprocessor");
+  std::string out;
+  char buf[4096];
+  size_t n;
+  while ((n = fread(buf, 1, sizeof buf, p)) > 0) out.append(buf, n);
+  int rc = pclose(p);
+  std::string rm = "rm -rf '" + tmp + "'";
+  if (system(rm.c_str()) != 0) fprintf(stderr, "physisc-hip: warning: could not remove %s\n", tmp.c_str());
+  if (rc != 0) Die("preprocessing failed:\n" + out);
+  return out;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  std::string input, output, config;
+  std::vector<std::string> defs;
+  for (int i = 1; i < argc; ++i) {
+    std::string a = argv[i];
+    if (a == "-o" && i + 1 < argc) output = argv[++i];
+    else if (a == "--config" && i + 1 < argc) config = argv[++i];
+    else if (a.compare(0, 2, "-D") == 0) defs.push_back(a.size() > 2 ? a.substr(2) : argv[++i]);
+    else if (a == "--hip") continue;
+    else if (a == "-h" || a == "--help") {
+      printf("usage: physisc-hip [--config FILE] [-o OUT] [-D NAME[=V]]... INPUT.c\n");
+      return 0;
+    } else if (!a.empty() && a[0] == '-') Die("unknown option " + a);
+    else input = a;
+  }
+  if (input.empty()) Die("no input file");
+  if (output.empty()) {
+    std::string base = input.substr(input.find_last_of('/') + 1);
+    size_t dot = base.rfind('.');
+    output = (dot == std::string::npos ? base : base.substr(0, dot)) + ".hip";
+  }
+  std::vector<std::string> includes;
+  std::string expanded = Preprocess(input, defs, &includes);
+  Translator tr(Tokenize(expanded), LoadConfig(config), input);
+  std::string code = tr.Run(includes);
+  std::ofstream out(output);
+  if (!out) Die("cannot write " + output);
+  out << code;
+  return 0;
+}

@@ -1,0 +1,56 @@
+"""Builds the reference's DSL system tests for the MI355X target.
+
+Each tests/system_tests/test_cases/test_*.c of the reference checkout is
+translated with physisc-hip and compiled with hipcc into tests/_corpus_bin/
+(ignored by git; the executables travel with the tree to the GPU machine,
+where tests/test_gpu_corpus.py runs them and compares their stdout with the
+SHA-256 of the reference's own manual.ref outputs in manual_ref.json).
+Programs this target does not support (user-defined point types, red-black
+maps, the multi-file module test) are listed in SKIPPED with the reason.
+"""
+import json
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+CASES = "/root/reference/tests/system_tests/test_cases"
+OUT = os.path.join(ROOT, "tests", "_corpus_bin")
+
+SKIPPED = {
+    "user-defined": "user-defined point types (PSGrid3DPoint...) are not lowered by physisc-hip",
+    "redblack": "PSStencilMapRedBlack/Red/Black are not lowered by physisc-hip",
+    "module": "multi-file module test (no main in one unit)",
+}
+
+
+def main():
+    if not os.path.isdir(CASES):
+        print("reference corpus not present; nothing to build", file=sys.stderr)
+        return 0
+    sys.path.insert(0, ROOT)
+    from physis_amd import phc
+    os.makedirs(OUT, exist_ok=True)
+    built, failed = [], []
+    for name in sorted(os.listdir(CASES)):
+        if not name.startswith("test_") or not name.endswith(".c") or ".manual." in name:
+            continue
+        stem = name[:-2]
+        if any(k in stem for k in SKIPPED):
+            continue
+        exe = os.path.join(OUT, stem)
+        try:
+            phc.build(os.path.join(CASES, name), exe)
+            os.remove(exe + ".hip")
+            built.append(stem)
+        except subprocess.CalledProcessError:
+            failed.append(stem)
+    with open(os.path.join(OUT, "manifest.json"), "w") as f:
+        json.dump({"built": built, "failed": failed}, f, indent=1)
+    print("built %d, failed %d %s" % (len(built), len(failed), failed), file=sys.stderr)
+    return 1 if failed else 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

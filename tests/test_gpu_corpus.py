@@ -1,0 +1,60 @@
+"""The reference's DSL system tests, translated by physisc-hip, run on the GPU.
+
+The executables are built from the reference checkout by
+tests/golden/build_corpus.py into tests/_corpus_bin/.  As in the reference's
+run_system_tests.sh (run_system_tests.sh.cmake:808-856), a program passes if
+its stdout equals the output of the reference's hand-written ref oracle
+(compared through the SHA-256 in tests/golden/manual_ref.json); the
+self-checking programs (test_01, 02, 03, 08-14, multi-kernels) must exit 0.
+"""
+import hashlib
+import json
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BIN = os.path.join(HERE, "_corpus_bin")
+GOLDEN = json.load(open(os.path.join(HERE, "golden", "manual_ref.json")))
+
+# Programs whose output is compared with a manual.ref oracle.
+COMPARED = ["test_7-pt", "test_7-pt-periodic", "test_7-pt-neumann-cond", "test_7-pt-multi-iterations",
+            "test_7-pt-double-type", "test_7-pt-int-type", "test_7-pt-type-mix", "test_15", "test_16",
+            "test_27-pt", "test_27-pt-periodic", "test_3-pt-periodic", "test_3-pt-1d", "test_9-pt-2d",
+            "test_asymmetric", "test_asymmetric-periodic", "test_stencil-hole", "test_mixed-dim",
+            "test_mixed-dim2", "test_mixed-dim3"]
+SELF_CHECKING = ["test_01", "test_02", "test_03", "test_08", "test_09", "test_10", "test_11", "test_12",
+                 "test_13", "test_14", "test_multi-kernels"]
+
+
+def exe(name):
+    path = os.path.join(BIN, name)
+    if not os.path.exists(path):
+        pytest.skip("%s not built (tests/golden/build_corpus.py needs the reference checkout)" % name)
+    return path
+
+
+@pytest.mark.parametrize("name", COMPARED)
+def test_output_matches_reference_oracle(name):
+    print("running", name, flush=True)
+    out = subprocess.run([exe(name)], capture_output=True, timeout=120)
+    assert out.returncode == 0, out.stderr.decode()[-2000:]
+    text = out.stdout
+    g = GOLDEN[name]
+    if hashlib.sha256(text).hexdigest() != g["sha256"]:
+        lines = text.decode().splitlines()
+        assert len(lines) == g["lines"], "line count %d != %d" % (len(lines), g["lines"])
+        for i, want in enumerate(g["samples"]):
+            got = lines[i * g["sample_every"]]
+            assert got == want, "line %d: %s != %s" % (i * g["sample_every"], got, want)
+        pytest.fail("output differs from the reference oracle outside the sampled lines")
+
+
+@pytest.mark.parametrize("name", SELF_CHECKING)
+def test_self_checking_program_passes(name):
+    print("running", name, flush=True)
+    out = subprocess.run([exe(name)], capture_output=True, timeout=120)
+    assert out.returncode == 0, (out.stdout.decode()[-1000:], out.stderr.decode()[-2000:])
