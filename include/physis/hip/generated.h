@@ -92,7 +92,7 @@ inline hipError_t LaunchRing(const typename P::Args &a, const Box &box, hipStrea
 // 16-byte vectors, else the LDS ring.
 template <class P>
 inline hipError_t LaunchStarStep(const typename P::Args &a, const Box &box, int nx, hipStream_t s) {
-  if (StarSupported(nx)) return LaunchZMarchStar<P, 4>(a, box, 64, true, s);
+  if (StarSupported(nx)) return LaunchZMarchStar<P, 8>(a, box, 128, true, s);
   return LaunchZMarch<P, 64, 4>(a, box, 64, true, s);
 }
 

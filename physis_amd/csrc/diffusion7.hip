@@ -151,9 +151,11 @@ const Variant kVariants[] = {
     {"tb2_w4_by4_z32", 4, 4, 4, 32, true},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
-constexpr int kDefaultVariant = 9;
+// Measured on MI355X at 512^3 FP32: star4_by8_z128 701 GLUPS one step per
+// pass; tb2_w2_by8_z32 ~1100 GLUPS two steps per pass (bench.py --sweep).
+constexpr int kDefaultVariant = 17;
 // One-step variant used by family-4 runs for unfused steps.
-constexpr int kStarVariant = 9;
+constexpr int kStarVariant = 14;
 
 // Launches one fused pair of steps src -> (on chip) -> dst with variant v
 // (family 4).  Returns hipErrorNotSupported if the grid does not qualify.
