@@ -75,6 +75,52 @@ def cpu_baseline(n):
             "sample": "%d steps of %d^3 FP32 7-pt clamped diffusion, ref-target restatement, 1 thread" % (steps, n)}
 
 
+def batched_bench(rt, n_grids=256, steps=200, seed=2024):
+    """Config 4: n_grids independent FP32 7-point grids, edges log-uniform in
+    [32, 256] per axis (seeded), 200 steps each.  Batched (tuned launch shape,
+    one hipGraph for the run) vs a sequential loop of plain single-grid runs
+    (PSHipDiffusion7Run per grid) over the same grids."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    shapes = [tuple(int(round(v)) for v in np.exp(rng.uniform(np.log(32), np.log(256), 3))) for _ in range(n_grids)]
+    pairs = [(rt.Grid(s), rt.Grid(s)) for s in shapes]
+    lups = sum(s[0] * s[1] * s[2] for s in shapes) * steps
+
+    def refill():
+        for i, (a, _) in enumerate(pairs):
+            a.fill_uniform(1000 + i)
+        rt.synchronize()
+
+    refill()
+    t0 = time.perf_counter()
+    for a, b in pairs:
+        rt.diffusion7_run(a, b, COEF, 0, steps)
+    rt.synchronize()
+    seq = time.perf_counter() - t0
+    refill()
+    batch = rt.Batch(pairs, [COEF] * n_grids, rt.CLAMPED)
+    t0 = time.perf_counter()
+    tune_ms = batch.tune(8)
+    tune_s = time.perf_counter() - t0
+    batch.run(steps)          # captures the graph (and runs it once)
+    rt.synchronize()
+    refill()
+    t0 = time.perf_counter()
+    batch.run(steps)          # graph replay: the timed run
+    rt.synchronize()
+    bat = time.perf_counter() - t0
+    cfg = batch.config_name()
+    batch.destroy()
+    for a, b in pairs:
+        a.free()
+        b.free()
+    return {"workload": "%d grids, edges log-uniform [32,256] per axis, seed %d, %d steps, FP32 clamped"
+                        % (n_grids, seed, steps),
+            "lups": lups, "batched_glups": round(lups / bat / 1e9, 1), "batched_s": round(bat, 4),
+            "sequential_glups": round(lups / seq / 1e9, 1), "sequential_s": round(seq, 4),
+            "speedup": round(seq / bat, 2), "tuned_config": cfg, "tune_s": round(tune_s, 2)}
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "diffusion7_512_traffic.json")
     try:
@@ -93,6 +139,7 @@ def main():
     ap.add_argument("--variant", type=int, default=0, help="kernel variant id (0 = default)")
     ap.add_argument("--sweep", action="store_true", help="also time every kernel variant (stderr)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched many-grid leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the stencil path on a single MI355X "
@@ -140,6 +187,10 @@ def main():
             file=sys.stderr, flush=True)
     g1.free()
     g2.free()
+    batched = None
+    if not args.no_batched:
+        batched = batched_bench(rt)
+        print("batched: %s" % json.dumps(batched), file=sys.stderr, flush=True)
 
     wall, dev = results[0]
     glups = n ** 3 * args.steps / wall / 1e9
@@ -164,7 +215,8 @@ def main():
                    "variant": args.variant,
                    "periodic_glups": round(n ** 3 * args.steps / results[1][0] / 1e9, 2),
                    "hbm_roofline_pct": round(100 * achieved / HBM_PEAK_GBS, 2),
-                   "stream_copy_gbs": round(copy_gbs, 1)},
+                   "stream_copy_gbs": round(copy_gbs, 1),
+                   "batched": batched},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,

@@ -1,0 +1,219 @@
+// skeleton_stride.h -- star-stencil tiles for rows of any length.
+//
+// A wave owns one whole row of up to 64*NP points: lane i holds x = i + 64k
+// (k < NP), so every load and store instruction touches 64 consecutive
+// elements whatever the row's alignment.  A workgroup holds BY rows of one
+// z plane.  As in the 16-byte skeletons, z-1/z/z+1 of a lane's points are a
+// register queue with the next plane's load in flight, and each step
+// publishes the row to an LDS buffer (double-buffered by step parity, four
+// pad floats at each end) from which x-1/x+1 and y-1/y+1 are read.  Periodic
+// plans write the wrapped row ends into the pads; clamped plans never select
+// them.
+//
+// These are tile-level device functions: the single-grid launchers and the
+// batched launchers (many grids in one launch) call them per work item.
+//   StrideStep   one map:  out(z) = body(in planes z-1, z, z+1)
+//   StrideTB2    two maps fused: u1 = body(u0) on chip, out = body(u1)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "physis/hip/skeleton.h"
+#include "physis/hip/skeleton_star.h"
+
+namespace psk {
+
+template <int NP, int BY>
+struct StrideLds {
+  static constexpr int kRow = 64 * NP + 8;
+  float step[2][BY + 2][kRow];   // StrideStep rows
+};
+
+template <int NP, int BY>
+struct StrideTB2Lds {
+  static constexpr int kRow = 64 * NP + 8;
+  float u0[2][BY + 2][kRow];
+  float u1[2][BY][kRow];
+};
+
+// Publishes NP points of a row into LDS row r (pads filled for wrap plans).
+template <bool W, int NP>
+__device__ __forceinline__ void StridePut(float *r, const float (&v)[NP], int lane, int nx) {
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    const int x = lane + 64 * k;
+    if (x < nx) {
+      r[x + 4] = v[k];
+      if (W && x == nx - 1) r[3] = v[k];
+      if (W && x == 0) r[nx + 4] = v[k];
+    }
+  }
+}
+
+template <class S, int NP>
+__device__ __forceinline__ void StrideLoad(const typename S::stored_t *row, float (&v)[NP], int lane, int nx) {
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    const int x = lane + 64 * k;
+    v[k] = x < nx ? S::load(row + x) : 0.f;
+  }
+}
+
+// One map over rows [y0, y0+BY) x planes [zb, ze) of a grid nx x ny x nz
+// (nx <= 64*NP), x over the whole row.  Out-of-range rows are skipped.
+template <class Plan, int NP, int BY>
+__device__ __forceinline__ void StrideStep(const typename Plan::Args &a, int nx, int ny, int nz, int y0, int zb,
+                                           int ze, StrideLds<NP, BY> &L) {
+  using S = typename Plan::Storage;
+  using st = typename S::stored_t;
+  constexpr bool W = Plan::kWrap;
+  const int lane = threadIdx.x, wy = threadIdx.y;
+  const int y = y0 + wy;
+  const int64_t plane = (int64_t)nx * ny;
+  const st *in = Plan::stage_ptr(a);
+  st *out = Plan::out_ptr(a);
+  const int64_t row = (int64_t)FixIndex<W>(y, ny) * nx;
+  const bool halo_lo = wy == 0, halo_hi = wy == BY - 1;
+  const int64_t hlo_row = (int64_t)FixIndex<W>(y0 - 1, ny) * nx;
+  const int64_t hhi_row = (int64_t)FixIndex<W>(y0 + BY, ny) * nx;
+  const bool active = y < ny;
+  auto zo = [&](int z) { return (int64_t)FixIndex<W>(z, nz) * plane; };
+
+  float b[NP], c[NP], t[NP], p[NP] = {}, hlo[NP] = {}, hhi[NP] = {}, nlo[NP] = {}, nhi[NP] = {};
+  StrideLoad<S, NP>(in + zo(zb - 1) + row, b, lane, nx);
+  StrideLoad<S, NP>(in + zo(zb) + row, c, lane, nx);
+  StrideLoad<S, NP>(in + zo(zb + 1) + row, t, lane, nx);
+  if (halo_lo) StrideLoad<S, NP>(in + zo(zb) + hlo_row, hlo, lane, nx);
+  if (halo_hi) StrideLoad<S, NP>(in + zo(zb) + hhi_row, hhi, lane, nx);
+  for (int z = zb; z < ze; ++z) {
+    if (z + 1 < ze) {
+      StrideLoad<S, NP>(in + zo(z + 2) + row, p, lane, nx);
+      if (halo_lo) StrideLoad<S, NP>(in + zo(z + 1) + hlo_row, nlo, lane, nx);
+      if (halo_hi) StrideLoad<S, NP>(in + zo(z + 1) + hhi_row, nhi, lane, nx);
+    }
+    auto &R = L.step[z & 1];
+    StridePut<W, NP>(R[wy + 1], c, lane, nx);
+    if (halo_lo) StridePut<W, NP>(R[0], hlo, lane, nx);
+    if (halo_hi) StridePut<W, NP>(R[BY + 1], hhi, lane, nx);
+    __syncthreads();
+    if (active) {
+      st *o = out + (int64_t)z * plane + (int64_t)y * nx;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        const int x = lane + 64 * k;
+        if (x < nx) {
+          StarCtx cx;
+          cx.c = c[k];
+          cx.w = R[wy + 1][x + 3];
+          cx.e = R[wy + 1][x + 5];
+          cx.n = R[wy][x + 4];
+          cx.s = R[wy + 2][x + 4];
+          cx.b = b[k];
+          cx.t = t[k];
+          S::store(o + x, Plan::compute(cx, a, x, y, z));
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      b[k] = c[k];
+      c[k] = t[k];
+      t[k] = p[k];
+      hlo[k] = nlo[k];
+      hhi[k] = nhi[k];
+    }
+  }
+}
+
+// Two fused maps: u1 = body(u0 from stage_ptr(a1)) on rows [y0-1+1 ...]:
+// the tile computes u1 on rows [ya, ya+BY) and writes u2 = body(u1) through
+// out_ptr(a2) on rows [ya+1, ya+BY-1), planes [zb, ze).
+template <class Plan, int NP, int BY>
+__device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx,
+                                          int ny, int nz, int ya, int zb, int ze, StrideTB2Lds<NP, BY> &L) {
+  using S = typename Plan::Storage;
+  using st = typename S::stored_t;
+  constexpr bool W = Plan::kWrap;
+  const int lane = threadIdx.x, wy = threadIdx.y;
+  const int y = ya + wy;
+  const int yf = FixIndex<W>(y, ny);
+  const int64_t plane = (int64_t)nx * ny;
+  const st *in = Plan::stage_ptr(a1);
+  st *out = Plan::out_ptr(a2);
+  const int64_t row = (int64_t)yf * nx;
+  const bool halo_lo = wy == 0, halo_hi = wy == BY - 1;
+  const int64_t hlo_row = (int64_t)FixIndex<W>(ya - 1, ny) * nx;
+  const int64_t hhi_row = (int64_t)FixIndex<W>(ya + BY, ny) * nx;
+  const bool out_row = wy >= 1 && wy <= BY - 2 && y < ny;
+  auto zo = [&](int z) { return (int64_t)FixIndex<W>(z, nz) * plane; };
+
+  float b0[NP], c0[NP], t0[NP], p0[NP] = {}, hlo[NP] = {}, hhi[NP] = {}, nlo[NP] = {}, nhi[NP] = {};
+  float b1[NP] = {}, c1[NP] = {}, t1[NP] = {};
+  StrideLoad<S, NP>(in + zo(zb - 2) + row, b0, lane, nx);
+  StrideLoad<S, NP>(in + zo(zb - 1) + row, c0, lane, nx);
+  StrideLoad<S, NP>(in + zo(zb) + row, t0, lane, nx);
+  if (halo_lo) StrideLoad<S, NP>(in + zo(zb - 1) + hlo_row, hlo, lane, nx);
+  if (halo_hi) StrideLoad<S, NP>(in + zo(zb - 1) + hhi_row, hhi, lane, nx);
+  for (int k = zb - 1; k <= ze; ++k) {
+    const int buf = (k + 2) & 1;
+    auto &R0 = L.u0[buf];
+    auto &R1 = L.u1[buf];
+    StridePut<W, NP>(R0[wy + 1], c0, lane, nx);
+    if (halo_lo) StridePut<W, NP>(R0[0], hlo, lane, nx);
+    if (halo_hi) StridePut<W, NP>(R0[BY + 1], hhi, lane, nx);
+    StridePut<W, NP>(R1[wy], c1, lane, nx);
+    if (k < ze) {
+      StrideLoad<S, NP>(in + zo(k + 2) + row, p0, lane, nx);
+      if (halo_lo) StrideLoad<S, NP>(in + zo(k + 1) + hlo_row, nlo, lane, nx);
+      if (halo_hi) StrideLoad<S, NP>(in + zo(k + 1) + hhi_row, nhi, lane, nx);
+    }
+    __syncthreads();
+    const int kf = FixIndex<W>(k, nz);
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      const int x = lane + 64 * q;
+      if (x < nx) {
+        StarCtx cx;
+        cx.c = c0[q];
+        cx.w = R0[wy + 1][x + 3];
+        cx.e = R0[wy + 1][x + 5];
+        cx.n = R0[wy][x + 4];
+        cx.s = R0[wy + 2][x + 4];
+        cx.b = b0[q];
+        cx.t = t0[q];
+        t1[q] = Plan::compute(cx, a1, x, yf, kf);
+      }
+    }
+    if (out_row && k - 1 >= zb) {
+      st *o = out + (int64_t)(k - 1) * plane + (int64_t)y * nx;
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        const int x = lane + 64 * q;
+        if (x < nx) {
+          StarCtx cx;
+          cx.c = c1[q];
+          cx.w = R1[wy][x + 3];
+          cx.e = R1[wy][x + 5];
+          cx.n = R1[wy - 1][x + 4];
+          cx.s = R1[wy + 1][x + 4];
+          cx.b = b1[q];
+          cx.t = t1[q];
+          S::store(o + x, Plan::compute(cx, a2, x, y, k - 1));
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      b0[q] = c0[q];
+      c0[q] = t0[q];
+      t0[q] = p0[q];
+      hlo[q] = nlo[q];
+      hhi[q] = nhi[q];
+      b1[q] = c1[q];
+      c1[q] = t1[q];
+    }
+  }
+}
+
+}  // namespace psk

@@ -1,0 +1,480 @@
+// batched.hip -- many uneven grids advanced together (7-point diffusion).
+//
+// A batch is a set of independent grid pairs (A_i, B_i) of any shape with
+// their own coefficients.  Each time step is ONE launch covering every grid:
+// the launch's workgroups are persistent and stride over a work list of
+// (grid, row tile, z chunk) items sorted largest grid first.  With fused
+// steps on (temporal degree 2) each launch advances every grid two steps
+// through a per-grid third buffer, as the single-grid path does, and the last
+// pair runs unfused so every B_i ends as the reference leaves it.  All the
+// launches of a run are captured once into a hipGraph and replayed as one
+// graph launch.
+//
+// The launch shape (points per lane NP, rows per workgroup BY, z chunk,
+// temporal degree) is picked by an in-process autotuner that times the
+// candidates on the batch itself (grids restored afterwards) and caches the
+// winner per batch signature.
+//
+// No reference counterpart: the reference runs one grid per PSStencilRun.
+#include "runtime_hip.h"
+#include "physis/hip/skeleton.h"
+#include "physis/hip/skeleton_star.h"
+#include "physis/hip/skeleton_stride.h"
+#include "physis/hip/storage.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+namespace physis_hip {
+namespace {
+
+struct Coef7 {
+  float ce, cw, cn, cs, ct, cb, cc;
+};
+
+// The diffusion body (reference examples/diffusion-benchmark/diffusion3d_physis.c:29-58;
+// periodic form test_7-pt-periodic.c:12-24), one value per point.
+template <bool kPeriodic>
+struct BatchPlan {
+  using Storage = psk::Native<float>;
+  static constexpr bool kWrap = kPeriodic;
+  struct Args {
+    const float *in;
+    float *out;
+    int dim[3];
+    Coef7 k;
+  };
+  __host__ __device__ static const float *stage_ptr(const Args &a) { return a.in; }
+  __host__ __device__ static float *out_ptr(const Args &a) { return a.out; }
+  template <class C>
+  __device__ static float compute(const C &ctx, const Args &a, const int x, const int y, const int z) {
+    const int nx = a.dim[0], ny = a.dim[1], nz = a.dim[2];
+    float c, w, e, n, s, b, t;
+    c = ctx.at(0, 0, 0, 0);
+    if (kPeriodic) {
+      w = ctx.at(0, -1, 0, 0);
+      e = ctx.at(0, 1, 0, 0);
+      n = ctx.at(0, 0, -1, 0);
+      s = ctx.at(0, 0, 1, 0);
+      b = ctx.at(0, 0, 0, -1);
+      t = ctx.at(0, 0, 0, 1);
+    } else {
+      if (x == 0) w = ctx.at(0, 0, 0, 0); else w = ctx.at(0, -1, 0, 0);
+      if (x == nx - 1) e = ctx.at(0, 0, 0, 0); else e = ctx.at(0, 1, 0, 0);
+      if (y == 0) n = ctx.at(0, 0, 0, 0); else n = ctx.at(0, 0, -1, 0);
+      if (y == ny - 1) s = ctx.at(0, 0, 0, 0); else s = ctx.at(0, 0, 1, 0);
+      if (z == 0) b = ctx.at(0, 0, 0, 0); else b = ctx.at(0, 0, 0, -1);
+      if (z == nz - 1) t = ctx.at(0, 0, 0, 0); else t = ctx.at(0, 0, 0, 1);
+    }
+    const Coef7 &k = a.k;
+    return k.cc * c + k.cw * w + k.ce * e + k.cs * s + k.cn * n + k.cb * b + k.ct * t;
+  }
+};
+
+struct Item {
+  int grid, y0, zb, ze;
+};
+
+template <class P, int NP, int BY>
+__global__ void __launch_bounds__(64 * BY)
+BatchStep(const typename P::Args *args, const Item *items, int nitems) {
+  __shared__ psk::StrideLds<NP, BY> L;
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const Item it = items[w];
+    const typename P::Args a = args[it.grid];
+    psk::StrideStep<P, NP, BY>(a, a.dim[0], a.dim[1], a.dim[2], it.y0, it.zb, it.ze, L);
+    __syncthreads();  // the next item reuses the LDS rows
+  }
+}
+
+template <class P, int NP, int BY>
+__global__ void __launch_bounds__(64 * BY)
+BatchTB2(const typename P::Args *in_args, const typename P::Args *out_args, const Item *items, int nitems) {
+  __shared__ psk::StrideTB2Lds<NP, BY> L;
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const Item it = items[w];
+    const typename P::Args a1 = in_args[it.grid];
+    const typename P::Args a2 = out_args[it.grid];
+    psk::StrideTB2<P, NP, BY>(a1, a2, a1.dim[0], a1.dim[1], a1.dim[2], it.y0, it.zb, it.ze, L);
+    __syncthreads();
+  }
+}
+
+// A launch configuration of the batch.
+struct Config {
+  int np;      // points per lane (row width 64*np)
+  int by;      // rows per workgroup
+  int zchunk;  // planes per work item
+  int degree;  // 1: one step per launch; 2: two fused steps
+  bool operator<(const Config &o) const {
+    return std::tie(np, by, zchunk, degree) < std::tie(o.np, o.by, o.zchunk, o.degree);
+  }
+};
+
+std::string ConfigName(const Config &c) {
+  return "np" + std::to_string(c.np) + "_by" + std::to_string(c.by) + "_z" + std::to_string(c.zchunk) + "_t" +
+         std::to_string(c.degree);
+}
+
+using KernelFn = const void *;
+
+template <bool kPer>
+KernelFn StepKernel(int np, int by) {
+  using P = BatchPlan<kPer>;
+#define PSH_K(NP, BY) \
+  if (np == NP && by == BY) return reinterpret_cast<KernelFn>(&BatchStep<P, NP, BY>);
+  PSH_K(1, 4) PSH_K(1, 8) PSH_K(2, 4) PSH_K(2, 8) PSH_K(4, 4) PSH_K(4, 8)
+#undef PSH_K
+  return nullptr;
+}
+
+template <bool kPer>
+KernelFn TB2Kernel(int np, int by) {
+  using P = BatchPlan<kPer>;
+#define PSH_K(NP, BY) \
+  if (np == NP && by == BY) return reinterpret_cast<KernelFn>(&BatchTB2<P, NP, BY>);
+  PSH_K(1, 8) PSH_K(2, 8) PSH_K(4, 8) PSH_K(1, 16) PSH_K(2, 16) PSH_K(4, 16)
+#undef PSH_K
+  return nullptr;
+}
+
+// A device array of T, reallocated on growth.
+template <class T>
+struct DevArray {
+  T *p = nullptr;
+  size_t n = 0;
+  void Assign(const std::vector<T> &v, hipStream_t s) {
+    if (v.size() > n) {
+      if (p) CheckHip(hipFree(p), "hipFree");
+      CheckHip(hipMalloc(&p, v.size() * sizeof(T)), "hipMalloc(batch table)");
+      n = v.size();
+    }
+    if (!v.empty()) CheckHip(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+  }
+  ~DevArray() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// One batch of grid pairs.
+struct Batch {
+  std::vector<__PSGrid *> a, b;
+  std::vector<Coef7> coef;
+  std::vector<float *> aux;  // per-grid third buffer (fused steps)
+  bool periodic = false;
+  int max_nx = 0;
+  size_t bytes = 0;
+  // Per-config work lists and arg tables, and cached graphs per (config, steps).
+  struct Plan {
+    DevArray<Item> items;
+    int nitems = 0;
+    int blocks = 0;
+  };
+  std::map<Config, Plan> plans;
+  DevArray<BatchPlan<false>::Args> t_ab, t_ba, t_a_aux_in, t_aux_a_out, t_a_aux_out, t_aux_in, t_aux_b;
+  bool tables_ready = false;
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+  };
+  std::map<std::pair<Config, int>, Graph> graphs;
+  Config tuned{0, 0, 0, 0};
+
+  ~Batch() {
+    for (auto &g : graphs)
+      if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+    for (float *p : aux)
+      if (p) (void)hipFree(p);
+  }
+
+  int Count() const { return (int)a.size(); }
+
+  void BuildTables(hipStream_t s) {
+    if (tables_ready) return;
+    using Args = BatchPlan<false>::Args;  // identical layout for both boundary forms
+    std::vector<Args> ab, ba, a_in, aux_out_a, aux_out, aux_in, aux_b;
+    for (int i = 0; i < Count(); ++i) {
+      Args x;
+      for (int d = 0; d < 3; ++d) x.dim[d] = a[i]->dim[d];
+      x.k = coef[i];
+      float *A = static_cast<float *>(a[i]->p0), *B = static_cast<float *>(b[i]->p0), *X = aux[i];
+      x.in = A; x.out = B; ab.push_back(x);          // map 1: A -> B
+      x.in = B; x.out = A; ba.push_back(x);          // map 2: B -> A
+      x.in = A; x.out = nullptr; a_in.push_back(x);  // fused input A
+      x.in = nullptr; x.out = A; aux_out_a.push_back(x);  // fused output A
+      x.in = nullptr; x.out = X; aux_out.push_back(x);    // fused output aux
+      x.in = X; x.out = nullptr; aux_in.push_back(x);     // fused input aux
+      x.in = X; x.out = B; aux_b.push_back(x);            // last pair from aux: aux -> B
+    }
+    t_ab.Assign(ab, s);
+    t_ba.Assign(ba, s);
+    t_a_aux_in.Assign(a_in, s);
+    t_aux_a_out.Assign(aux_out_a, s);
+    t_a_aux_out.Assign(aux_out, s);
+    t_aux_in.Assign(aux_in, s);
+    t_aux_b.Assign(aux_b, s);
+    tables_ready = true;
+  }
+
+  bool Supports(const Config &c) const {
+    if (64 * c.np < max_nx) return false;
+    if (c.degree == 2) return TB2Kernel<false>(c.np, c.by) != nullptr;
+    return StepKernel<false>(c.np, c.by) != nullptr;
+  }
+
+  Plan &PlanFor(const Config &c, hipStream_t s) {
+    auto it = plans.find(c);
+    if (it != plans.end()) return it->second;
+    Plan &p = plans[c];
+    // Largest grids first; items of one grid adjacent.
+    std::vector<int> order(Count());
+    for (int i = 0; i < Count(); ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return a[x]->num_elms > a[y]->num_elms; });
+    std::vector<Item> items;
+    const int rows_out = c.degree == 2 ? c.by - 2 : c.by;
+    for (int g : order) {
+      const int ny = a[g]->dim[1], nz = a[g]->dim[2];
+      const int zc = c.zchunk <= 0 || c.zchunk > nz ? nz : c.zchunk;
+      for (int ty = 0; ty * rows_out < ny; ++ty)
+        for (int zb = 0; zb < nz; zb += zc) {
+          Item it;
+          it.grid = g;
+          it.y0 = c.degree == 2 ? ty * rows_out - 1 : ty * rows_out;
+          it.zb = zb;
+          it.ze = std::min(zb + zc, nz);
+          items.push_back(it);
+        }
+    }
+    p.items.Assign(items, s);
+    p.nitems = (int)items.size();
+    // Persistent grid: as many workgroups as fit on the device at once.
+    KernelFn k = c.degree == 2 ? TB2Kernel<false>(c.np, c.by) : StepKernel<false>(c.np, c.by);
+    int per_cu = 1, cus = 1;
+    CheckHip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64 * c.by, 0), "occupancy");
+    hipDeviceProp_t prop;
+    CheckHip(hipGetDeviceProperties(&prop, Runtime::Get().device()), "props");
+    cus = prop.multiProcessorCount;
+    p.blocks = std::max(1, std::min(p.nitems, std::max(1, per_cu) * cus));
+    return p;
+  }
+
+  // Queues `steps` steps of every grid with configuration c.
+  hipError_t Enqueue(const Config &c, int steps, hipStream_t s) {
+    BuildTables(s);
+    Plan &p = PlanFor(c, s);
+    if (p.nitems == 0 || steps <= 0) return hipSuccess;
+    dim3 grid(p.blocks), block(64, c.by);
+    auto step = [&](const BatchPlan<false>::Args *t) {
+      KernelFn k = periodic ? StepKernel<true>(c.np, c.by) : StepKernel<false>(c.np, c.by);
+      const void *tv = t;
+      const Item *iv = p.items.p;
+      int n = p.nitems;
+      void *args[] = {&tv, &iv, &n};
+      return hipLaunchKernel(k, grid, block, args, 0, s);
+    };
+    auto pair = [&](const BatchPlan<false>::Args *tin, const BatchPlan<false>::Args *tout) {
+      KernelFn k = periodic ? TB2Kernel<true>(c.np, c.by) : TB2Kernel<false>(c.np, c.by);
+      const void *ti = tin, *to = tout;
+      const Item *iv = p.items.p;
+      int n = p.nitems;
+      void *args[] = {&ti, &to, &iv, &n};
+      return hipLaunchKernel(k, grid, block, args, 0, s);
+    };
+    hipError_t e = hipSuccess;
+    int first = 0;
+    bool from_aux = false;
+    if (c.degree == 2 && steps >= 4) {
+      const int fused = steps / 2 - 1;
+      for (int i = 0; i < fused && e == hipSuccess; ++i) {
+        // A -> aux, aux -> A, ...
+        e = (i & 1) ? pair(t_aux_in.p, t_aux_a_out.p) : pair(t_a_aux_in.p, t_a_aux_out.p);
+      }
+      from_aux = fused & 1;
+      first = 2 * fused;
+    }
+    for (int i = first; i < steps && e == hipSuccess; ++i) {
+      const bool even = ((i - first) & 1) == 0;
+      if (even) e = step((i == first && from_aux) ? t_aux_b.p : t_ab.p);
+      else e = step(t_ba.p);
+    }
+    return e;
+  }
+
+  hipError_t Run(const Config &c, int steps, bool use_graph, hipStream_t s) {
+    if (!use_graph) return Enqueue(c, steps, s);
+    auto key = std::make_pair(c, steps);
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+      BuildTables(s);
+      PlanFor(c, s);
+      CheckHip(hipStreamSynchronize(s), "sync before capture");
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) return e;
+      hipError_t le = Enqueue(c, steps, s);
+      e = hipStreamEndCapture(s, &g);
+      if (le != hipSuccess) return le;
+      if (e != hipSuccess) return e;
+      hipGraphExec_t exec = nullptr;
+      e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return e;
+      it = graphs.emplace(key, Graph{exec}).first;
+    }
+    return hipGraphLaunch(it->second.exec, s);
+  }
+};
+
+std::vector<Config> Candidates(const Batch &b) {
+  std::vector<Config> out;
+  for (int deg : {1, 2})
+    for (int np : {1, 2, 4})
+      for (int by : {4, 8, 16})
+        for (int zc : {16, 32, 64}) {
+          Config c{np, by, zc, deg};
+          if (!b.Supports(c)) continue;
+          // Narrowest row that fits only (wider rows idle lanes).
+          if (np > 1 && 64 * (np / 2) >= b.max_nx) continue;
+          out.push_back(c);
+        }
+  return out;
+}
+
+Config DefaultConfig(const Batch &b) {
+  int np = b.max_nx <= 64 ? 1 : b.max_nx <= 128 ? 2 : 4;
+  return Config{np, 8, 32, 2};
+}
+
+}  // namespace
+}  // namespace physis_hip
+
+using namespace physis_hip;
+
+extern "C" {
+
+int PSHipBatchCreate(void **out, __PSGrid *const *a, __PSGrid *const *b, int n, const float *coef,
+                     int boundary) {
+  if (!out || !a || !b || !coef || n <= 0) return PS_HIP_ERR_INVALID;
+  if (boundary != PS_HIP_CLAMPED && boundary != PS_HIP_PERIODIC) return PS_HIP_ERR_INVALID;
+  *out = nullptr;
+  Batch *bt = new Batch();
+  bt->periodic = boundary == PS_HIP_PERIODIC;
+  for (int i = 0; i < n; ++i) {
+    const __PSGrid *x = a[i], *y = b[i];
+    if (!x || !y || x->num_dims != 3 || y->num_dims != 3 || x->elm_size != 4 || y->elm_size != 4 ||
+        Rec(x)->storage != PS_HIP_STORAGE_NATIVE || Rec(y)->storage != PS_HIP_STORAGE_NATIVE ||
+        x->dim[0] != y->dim[0] || x->dim[1] != y->dim[1] || x->dim[2] != y->dim[2] || x->dim[0] > 256) {
+      delete bt;
+      return PS_HIP_ERR_INVALID;
+    }
+    bt->a.push_back(a[i]);
+    bt->b.push_back(b[i]);
+    bt->coef.push_back(Coef7{coef[7 * i], coef[7 * i + 1], coef[7 * i + 2], coef[7 * i + 3], coef[7 * i + 4],
+                             coef[7 * i + 5], coef[7 * i + 6]});
+    bt->max_nx = std::max(bt->max_nx, x->dim[0]);
+    float *aux = nullptr;
+    if (hipMalloc(&aux, Rec(x)->bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      delete bt;
+      return PS_HIP_ERR_NOMEM;
+    }
+    bt->aux.push_back(aux);
+    bt->bytes += Rec(x)->bytes;
+  }
+  bt->tuned = DefaultConfig(*bt);
+  *out = bt;
+  return PS_HIP_OK;
+}
+
+int PSHipBatchDestroy(void *h) {
+  if (!h) return PS_HIP_ERR_INVALID;
+  (void)hipStreamSynchronize(Runtime::Get().stream());
+  delete static_cast<Batch *>(h);
+  return PS_HIP_OK;
+}
+
+// Runs `steps` steps of every grid of the batch (map A->B, B->A, ...; after an
+// even count A holds the field).  use_graph: capture once, replay afterwards.
+int PSHipBatchRun(void *h, int steps, int use_graph) {
+  if (!h || steps < 0) return PS_HIP_ERR_INVALID;
+  Batch *b = static_cast<Batch *>(h);
+  hipError_t e = b->Run(b->tuned, steps, use_graph != 0, Runtime::Get().stream());
+  return e == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
+}
+
+// Sets the launch configuration explicitly (np, by, zchunk, degree).
+int PSHipBatchConfigure(void *h, int np, int by, int zchunk, int degree) {
+  if (!h) return PS_HIP_ERR_INVALID;
+  Batch *b = static_cast<Batch *>(h);
+  Config c{np, by, zchunk, degree};
+  if ((degree != 1 && degree != 2) || !b->Supports(c)) return PS_HIP_ERR_UNSUPPORTED;
+  b->tuned = c;
+  return PS_HIP_OK;
+}
+
+int PSHipBatchConfigName(void *h, char *buf, int len) {
+  if (!h || !buf || len <= 0) return PS_HIP_ERR_INVALID;
+  snprintf(buf, len, "%s", ConfigName(static_cast<Batch *>(h)->tuned).c_str());
+  return PS_HIP_OK;
+}
+
+// Times every candidate configuration for `steps` steps (graph launches) on
+// the batch itself and keeps the fastest; the grids' contents are saved
+// before and restored after.  best_ms receives the winner's time.
+int PSHipBatchTune(void *h, int steps, float *best_ms) {
+  if (!h || steps <= 0) return PS_HIP_ERR_INVALID;
+  Batch *b = static_cast<Batch *>(h);
+  Runtime &rt = Runtime::Get();
+  hipStream_t s = rt.stream();
+  std::vector<void *> save_a(b->Count()), save_b(b->Count());
+  for (int i = 0; i < b->Count(); ++i) {
+    size_t by = Rec(b->a[i])->bytes;
+    if (hipMalloc(&save_a[i], by) != hipSuccess || hipMalloc(&save_b[i], by) != hipSuccess) {
+      (void)hipGetLastError();
+      for (void *p : save_a) if (p) (void)hipFree(p);
+      for (void *p : save_b) if (p) (void)hipFree(p);
+      return PS_HIP_ERR_NOMEM;
+    }
+    CheckHip(hipMemcpyAsync(save_a[i], b->a[i]->p0, by, hipMemcpyDeviceToDevice, s), "save");
+    CheckHip(hipMemcpyAsync(save_b[i], b->b[i]->p0, by, hipMemcpyDeviceToDevice, s), "save");
+  }
+  hipEvent_t e0, e1;
+  CheckHip(hipEventCreate(&e0), "event");
+  CheckHip(hipEventCreate(&e1), "event");
+  float best = 1e30f;
+  Config best_c = b->tuned;
+  for (const Config &c : Candidates(*b)) {
+    if (b->Run(c, steps, true, s) != hipSuccess) continue;  // warm: capture + first replay
+    CheckHip(hipEventRecord(e0, s), "record");
+    if (b->Run(c, steps, true, s) != hipSuccess) continue;
+    CheckHip(hipEventRecord(e1, s), "record");
+    CheckHip(hipEventSynchronize(e1), "sync");
+    float ms = 0.f;
+    CheckHip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    if (rt.trace()) fprintf(stderr, "physis-hip: batch tune %s: %.3f ms\n", ConfigName(c).c_str(), ms);
+    if (ms < best) {
+      best = ms;
+      best_c = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (int i = 0; i < b->Count(); ++i) {
+    size_t by = Rec(b->a[i])->bytes;
+    CheckHip(hipMemcpyAsync(b->a[i]->p0, save_a[i], by, hipMemcpyDeviceToDevice, s), "restore");
+    CheckHip(hipMemcpyAsync(b->b[i]->p0, save_b[i], by, hipMemcpyDeviceToDevice, s), "restore");
+  }
+  CheckHip(hipStreamSynchronize(s), "sync");
+  for (int i = 0; i < b->Count(); ++i) {
+    (void)hipFree(save_a[i]);
+    (void)hipFree(save_b[i]);
+  }
+  b->tuned = best_c;
+  if (best_ms) *best_ms = best;
+  return PS_HIP_OK;
+}
+
+}  // extern "C"

@@ -232,6 +232,55 @@ def himeno_run(p0, p1, coef_grids, omega, sweeps, with_gosa=True, variant=0):
     return np.array(gosa[:sweeps], np.float64)
 
 
+class Batch:
+    """Many independent 7-point diffusion grid pairs advanced together: one
+    launch per step (or per two fused steps) for all grids, captured into a
+    hipGraph.  pairs: list of (g1, g2) Grids (nx <= 256); coefs: one 7-tuple
+    (ce, cw, cn, cs, ct, cb, cc) per pair."""
+
+    def __init__(self, pairs, coefs, boundary=CLAMPED):
+        L = lib()
+        for name in ("PSHipBatchCreate", "PSHipBatchRun", "PSHipBatchTune", "PSHipBatchConfigure",
+                     "PSHipBatchConfigName", "PSHipBatchDestroy"):
+            getattr(L, name).restype = ctypes.c_int
+        L.PSHipBatchCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(_GP), ctypes.POINTER(_GP),
+                                       ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
+        L.PSHipBatchRun.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.PSHipBatchTune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+        L.PSHipBatchConfigure.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 4
+        L.PSHipBatchConfigName.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
+        L.PSHipBatchDestroy.argtypes = [ctypes.c_void_p]
+        n = len(pairs)
+        self.pairs = pairs
+        a = (_GP * n)(*[p[0].handle for p in pairs])
+        b = (_GP * n)(*[p[1].handle for p in pairs])
+        k = (ctypes.c_float * (7 * n))(*[float(v) for c in coefs for v in c])
+        h = ctypes.c_void_p()
+        check(L.PSHipBatchCreate(ctypes.byref(h), a, b, n, k, boundary), "PSHipBatchCreate")
+        self.handle = h
+
+    def run(self, steps, graph=True):
+        check(lib().PSHipBatchRun(self.handle, steps, 1 if graph else 0), "PSHipBatchRun")
+
+    def configure(self, np_, by, zchunk, degree):
+        check(lib().PSHipBatchConfigure(self.handle, np_, by, zchunk, degree), "PSHipBatchConfigure")
+
+    def tune(self, steps):
+        ms = ctypes.c_float()
+        check(lib().PSHipBatchTune(self.handle, steps, ctypes.byref(ms)), "PSHipBatchTune")
+        return ms.value
+
+    def config_name(self):
+        buf = ctypes.create_string_buffer(64)
+        check(lib().PSHipBatchConfigName(self.handle, buf, 64), "PSHipBatchConfigName")
+        return buf.value.decode()
+
+    def destroy(self):
+        if self.handle:
+            check(lib().PSHipBatchDestroy(self.handle), "PSHipBatchDestroy")
+            self.handle = None
+
+
 class Timer:
     """HIP events on the runtime stream."""
 

@@ -1,0 +1,91 @@
+"""Batched multi-grid diffusion on the GPU: every grid of the batch must end
+bitwise equal to the oracle run on that grid alone, for every launch
+configuration (one and two steps per launch), with and without hipGraph
+capture, and across repeated graph replays with eager work in between."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def make_batch(rt, shapes, seed, boundary):
+    rng = np.random.default_rng(seed)
+    pairs, fields, coefs = [], [], []
+    for i, shp in enumerate(shapes):
+        f = O.uniform_field(shp, seed=seed * 1000 + i)
+        g1, g2 = rt.Grid(shp), rt.Grid(shp)
+        g1.copyin(f)
+        g2.copyin(np.zeros_like(f))
+        nb = rng.uniform(0.02, 0.15, 6).astype(np.float32)
+        c = [float(v) for v in nb] + [float(np.float32(1.0) - np.float32(nb.sum()))]
+        pairs.append((g1, g2))
+        fields.append(f)
+        coefs.append(c)
+    return pairs, fields, coefs, rt.Batch(pairs, coefs, boundary)
+
+
+SHAPES = [(5, 7, 3), (70, 9, 11), (33, 40, 6), (64, 64, 64), (1, 1, 1), (17, 3, 29), (128, 5, 4), (2, 31, 8),
+          (101, 12, 9), (8, 8, 8), (66, 2, 2), (127, 6, 5)]
+
+
+def check_against_oracle(pairs, fields, coefs, boundary, steps):
+    for (g1, g2), f, c in zip(pairs, fields, coefs):
+        w1, w2 = O.diffusion7(f, c, periodic=bool(boundary), steps=steps)
+        assert np.array_equal(g1.copyout(), w1), "grid %s" % (g1.shape,)
+        assert np.array_equal(g2.copyout(), w2), "grid %s" % (g2.shape,)
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+@pytest.mark.parametrize("cfg", [(2, 4, 16, 1), (2, 8, 64, 1), (4, 8, 32, 1), (2, 8, 16, 2), (2, 16, 32, 2),
+                                 (4, 8, 8, 2)])
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_bitwise_vs_oracle(rt, boundary, cfg, graph):
+    steps = 9
+    pairs, fields, coefs, batch = make_batch(rt, SHAPES, seed=5, boundary=boundary)
+    batch.configure(*cfg)
+    print("config", batch.config_name(), "graph", graph, flush=True)
+    batch.run(steps, graph=graph)
+    check_against_oracle(pairs, fields, coefs, boundary, steps)
+    batch.destroy()
+    for g1, g2 in pairs:
+        g1.free()
+        g2.free()
+
+
+def test_graph_replays_with_eager_work_between(rt):
+    steps = 6
+    pairs, fields, coefs, batch = make_batch(rt, SHAPES[:6], seed=9, boundary=0)
+    batch.configure(2, 8, 16, 2)
+    for rep in range(1, 4):
+        batch.run(steps, graph=True)
+        # Eager work on the same stream between replays.
+        junk = rt.Grid((16, 16, 16))
+        junk.fill_uniform(rep)
+        junk.reduce64(rt.PS_SUM)
+        junk.free()
+        rt.synchronize()
+        check_against_oracle(pairs, fields, coefs, 0, steps * rep)
+    batch.destroy()
+
+
+def test_tune_restores_grids_and_picks_a_config(rt):
+    pairs, fields, coefs, batch = make_batch(rt, SHAPES, seed=3, boundary=1)
+    ms = batch.tune(4)
+    print("tuned:", batch.config_name(), "%.3f ms" % ms, flush=True)
+    assert ms > 0
+    for (g1, g2), f in zip(pairs, fields):
+        assert np.array_equal(g1.copyout(), f)
+    batch.run(8)
+    check_against_oracle(pairs, fields, coefs, 1, 8)
+    batch.destroy()
+
+
+def test_batch_rejects_wide_or_mismatched_grids(rt):
+    a, b = rt.Grid((300, 4, 4)), rt.Grid((300, 4, 4))
+    with pytest.raises(rt.PhysisError):
+        rt.Batch([(a, b)], [[0.1] * 6 + [0.4]])
+    c, d = rt.Grid((8, 4, 4)), rt.Grid((8, 4, 5))
+    with pytest.raises(rt.PhysisError):
+        rt.Batch([(c, d)], [[0.1] * 6 + [0.4]])
