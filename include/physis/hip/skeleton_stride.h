@@ -77,7 +77,7 @@ __device__ __forceinline__ void StrideStep(const typename Plan::Args &a, int nx,
   const bool halo_lo = wy == 0, halo_hi = wy == BY - 1;
   const int64_t hlo_row = (int64_t)FixIndex<W>(y0 - 1, ny) * nx;
   const int64_t hhi_row = (int64_t)FixIndex<W>(y0 + BY, ny) * nx;
-  const bool active = y < ny;
+  const bool active = y >= 0 && y < ny;
   auto zo = [&](int z) { return (int64_t)FixIndex<W>(z, nz) * plane; };
 
   float b[NP], c[NP], t[NP], p[NP] = {}, hlo[NP] = {}, hhi[NP] = {}, nlo[NP] = {}, nhi[NP] = {};
@@ -145,7 +145,7 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
   const bool halo_lo = wy == 0, halo_hi = wy == BY - 1;
   const int64_t hlo_row = (int64_t)FixIndex<W>(ya - 1, ny) * nx;
   const int64_t hhi_row = (int64_t)FixIndex<W>(ya + BY, ny) * nx;
-  const bool out_row = wy >= 1 && wy <= BY - 2 && y < ny;
+  const bool out_row = wy >= 1 && wy <= BY - 2 && y >= 0 && y < ny;
   auto zo = [&](int z) { return (int64_t)FixIndex<W>(z, nz) * plane; };
 
   float b0[NP], c0[NP], t0[NP], p0[NP] = {}, hlo[NP] = {}, hhi[NP] = {}, nlo[NP] = {}, nhi[NP] = {};

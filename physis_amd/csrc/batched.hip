@@ -127,7 +127,7 @@ KernelFn StepKernel(int np, int by) {
   using P = BatchPlan<kPer>;
 #define PSH_K(NP, BY) \
   if (np == NP && by == BY) return reinterpret_cast<KernelFn>(&BatchStep<P, NP, BY>);
-  PSH_K(1, 4) PSH_K(1, 8) PSH_K(2, 4) PSH_K(2, 8) PSH_K(4, 4) PSH_K(4, 8)
+  PSH_K(1, 4) PSH_K(1, 8) PSH_K(1, 16) PSH_K(2, 4) PSH_K(2, 8) PSH_K(2, 16) PSH_K(4, 4) PSH_K(4, 8) PSH_K(4, 16)
 #undef PSH_K
   return nullptr;
 }
@@ -153,7 +153,11 @@ struct DevArray {
       CheckHip(hipMalloc(&p, v.size() * sizeof(T)), "hipMalloc(batch table)");
       n = v.size();
     }
-    if (!v.empty()) CheckHip(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+    if (!v.empty()) {
+      // Synchronous: the host vector may go away as soon as this returns.
+      CheckHip(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+      CheckHip(hipStreamSynchronize(s), "upload sync");
+    }
   }
   ~DevArray() {
     if (p) (void)hipFree(p);
@@ -219,9 +223,11 @@ struct Batch {
     tables_ready = true;
   }
 
+  // A degree-2 run also needs the one-step kernel of the same shape for its
+  // unfused steps.
   bool Supports(const Config &c) const {
-    if (64 * c.np < max_nx) return false;
-    if (c.degree == 2) return TB2Kernel<false>(c.np, c.by) != nullptr;
+    if (64 * c.np < max_nx || c.zchunk <= 0) return false;
+    if (c.degree == 2 && TB2Kernel<false>(c.np, c.by) == nullptr) return false;
     return StepKernel<false>(c.np, c.by) != nullptr;
   }
 
@@ -264,24 +270,29 @@ struct Batch {
   // Queues `steps` steps of every grid with configuration c.
   hipError_t Enqueue(const Config &c, int steps, hipStream_t s) {
     BuildTables(s);
-    Plan &p = PlanFor(c, s);
-    if (p.nitems == 0 || steps <= 0) return hipSuccess;
-    dim3 grid(p.blocks), block(64, c.by);
+    // Unfused steps always run on the one-step work list (row tiles start at
+    // their first row); fused pairs on the two-step list (tiles start one row
+    // above their first output row).
+    const Config c1{c.np, c.by, c.zchunk, 1};
+    Plan &p1 = PlanFor(c1, s);
+    Plan &p2 = PlanFor(c, s);
+    if (p1.nitems == 0 || steps <= 0) return hipSuccess;
+    dim3 block(64, c.by);
     auto step = [&](const BatchPlan<false>::Args *t) {
       KernelFn k = periodic ? StepKernel<true>(c.np, c.by) : StepKernel<false>(c.np, c.by);
       const void *tv = t;
-      const Item *iv = p.items.p;
-      int n = p.nitems;
+      const Item *iv = p1.items.p;
+      int n = p1.nitems;
       void *args[] = {&tv, &iv, &n};
-      return hipLaunchKernel(k, grid, block, args, 0, s);
+      return hipLaunchKernel(k, dim3(p1.blocks), block, args, 0, s);
     };
     auto pair = [&](const BatchPlan<false>::Args *tin, const BatchPlan<false>::Args *tout) {
       KernelFn k = periodic ? TB2Kernel<true>(c.np, c.by) : TB2Kernel<false>(c.np, c.by);
       const void *ti = tin, *to = tout;
-      const Item *iv = p.items.p;
-      int n = p.nitems;
+      const Item *iv = p2.items.p;
+      int n = p2.nitems;
       void *args[] = {&ti, &to, &iv, &n};
-      return hipLaunchKernel(k, grid, block, args, 0, s);
+      return hipLaunchKernel(k, dim3(p2.blocks), block, args, 0, s);
     };
     hipError_t e = hipSuccess;
     int first = 0;
@@ -308,7 +319,9 @@ struct Batch {
     auto key = std::make_pair(c, steps);
     auto it = graphs.find(key);
     if (it == graphs.end()) {
+      // Everything that allocates or uploads happens before the capture.
       BuildTables(s);
+      PlanFor(Config{c.np, c.by, c.zchunk, 1}, s);
       PlanFor(c, s);
       CheckHip(hipStreamSynchronize(s), "sync before capture");
       hipGraph_t g = nullptr;
