@@ -121,6 +121,40 @@ def batched_bench(rt, n_grids=256, steps=200, seed=2024):
             "speedup": round(seq / bat, 2), "tuned_config": cfg, "tune_s": round(tune_s, 2)}
 
 
+def served_bench(rt, n_req=24, rate=40.0, seed=7, budget_gib=48, tick=50):
+    """Config 5: Poisson arrivals at a fixed rate of mixed requests (70%
+    7-point diffusion, 30% 19-point Jacobi), cubic edges uniform in [32, 384],
+    FP32 or BF16 storage per request, each run to residual <= 1e-5 or 2000
+    steps; grids resident on the device, admission by HBM budget.  The same
+    trace is served by the continuous-batching server and by a sequential
+    loop of plain single-grid runs."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t, reqs = 0.0, []
+    for i in range(n_req):
+        t += rng.exponential(1.0 / rate)
+        reqs.append(dict(kind=0 if rng.random() < 0.7 else 1, n=int(rng.integers(32, 385)),
+                         storage=int(rng.random() < 0.5), seed=1000 + i, arrival=t, max_steps=2000, tol=1e-5))
+    out = {"workload": "%d requests, Poisson %.0f/s, 70%% diffusion / 30%% Jacobi, edges U[32,384], "
+                       "FP32|BF16, tol 1e-5 or 2000 steps, tick %d, budget %d GiB, seed %d"
+                       % (n_req, rate, tick, budget_gib, seed)}
+    for mode, name in ((rt.SERVE_BATCHED, "served"), (rt.SERVE_SEQUENTIAL, "sequential")):
+        t0 = time.perf_counter()
+        res = rt.serve_trace(reqs, budget_gib << 30, tick, mode)
+        wall = time.perf_counter() - t0
+        lat = np.array([r["finished"] - q["arrival"] for r, q in zip(res, reqs)])
+        ok = sum(1 for r in res if r["status"] == 0)
+        out[name] = {"solves_per_s": round(ok / wall, 3), "wall_s": round(wall, 3),
+                     "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+                     "p99_latency_s": round(float(np.percentile(lat, 99)), 4), "completed": ok}
+        if name == "served":
+            served = res
+    same = all(a["steps"] == b["steps"] and a["checksum"] == b["checksum"] for a, b in zip(served, res))
+    out["identical_results"] = same
+    out["throughput_ratio"] = round(out["served"]["solves_per_s"] / max(out["sequential"]["solves_per_s"], 1e-9), 2)
+    return out
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "diffusion7_512_traffic.json")
     try:
@@ -140,6 +174,7 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="also time every kernel variant (stderr)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched many-grid leg")
+    ap.add_argument("--no-served", action="store_true", help="skip the served request-mix leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the stencil path on a single MI355X "
@@ -191,6 +226,10 @@ def main():
     if not args.no_batched:
         batched = batched_bench(rt)
         print("batched: %s" % json.dumps(batched), file=sys.stderr, flush=True)
+    served = None
+    if not args.no_served:
+        served = served_bench(rt)
+        print("served: %s" % json.dumps(served), file=sys.stderr, flush=True)
 
     wall, dev = results[0]
     glups = n ** 3 * args.steps / wall / 1e9
@@ -216,7 +255,8 @@ def main():
                    "periodic_glups": round(n ** 3 * args.steps / results[1][0] / 1e9, 2),
                    "hbm_roofline_pct": round(100 * achieved / HBM_PEAK_GBS, 2),
                    "stream_copy_gbs": round(copy_gbs, 1),
-                   "batched": batched},
+                   "batched": batched,
+                   "served": served},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,

@@ -63,6 +63,15 @@ __device__ __forceinline__ void Store4(typename S::stored_t *p, const float (&v)
   }
 }
 
+// A value as it would read back after a store and a load in storage S: the
+// on-chip intermediate of fused steps goes through the same rounding as an
+// intermediate written to HBM, so fused and unfused runs agree bitwise.
+template <class S>
+__device__ __forceinline__ float RoundTrip(float v) {
+  if constexpr (sizeof(typename S::stored_t) == 4) return v;
+  else return S::widen(S::narrow(v));
+}
+
 // The seven taps of one point; at() is resolved at compile time once the
 // body is inlined (its offsets are literals).
 struct StarCtx {

@@ -127,7 +127,7 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
         c.s = sv[j];
         c.b = b0[j];
         c.t = t0[j];
-        t1[j] = Plan::compute(c, a1, xs + j, yf, kf);
+        t1[j] = RoundTrip<S>(Plan::compute(c, a1, xs + j, yf, kf));
       }
       // u2 at plane k-1 on the output rows.
       if (out_row && k - 1 >= zb) {

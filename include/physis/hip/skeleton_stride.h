@@ -182,7 +182,7 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
         cx.s = R0[wy + 2][x + 4];
         cx.b = b0[q];
         cx.t = t0[q];
-        t1[q] = Plan::compute(cx, a1, x, yf, kf);
+        t1[q] = RoundTrip<S>(Plan::compute(cx, a1, x, yf, kf));
       }
     }
     if (out_row && k - 1 >= zb) {

@@ -58,6 +58,9 @@ extern void *PSHipGridDevicePtr(const __PSGrid *g);
  * lo + (hi - lo) * u_i, u_i = (splitmix64(seed + (i+1)*0x9E3779B97F4A7C15) >> 40) * 2^-24,
  * i the linear (x fastest) index.  Asynchronous. */
 extern int PSHipGridFillUniform(__PSGrid *g, uint64_t seed, float lo, float hi);
+/* Fills a float grid with `value` (profile = 0) or with the Himeno initial
+ * pressure (float)(z*z) / ((nz-1)*(nz-1)) (profile = 1).  Asynchronous. */
+extern int PSHipGridFillPattern(__PSGrid *g, int profile, float value);
 
 /* Streaming device copy of src into dst (same byte size, a multiple of 16)
  * with one plain kernel: the HBM bandwidth yardstick.  Asynchronous. */
@@ -110,6 +113,56 @@ extern const char *PSHipDiffusion7VariantName(int variant);
  * the FP64 sum of ss^2 of sweep i when gosa is not NULL. */
 extern int PSHipHimenoRun(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids,
                           float omega, int sweeps, double *gosa, int variant);
+
+/* ---- batched many-grid diffusion (no reference counterpart) -------------
+ * Grid pairs (a[i], b[i]) of any shape (nx <= 512), FP32 or BF16 storage
+ * (one kind per batch), coef = 7 floats per pair (ce, cw, cn, cs, ct, cb, cc).
+ * Each step is one launch over all grids; runs are captured into hipGraphs. */
+extern int PSHipBatchCreate(void **out, __PSGrid *const *a, __PSGrid *const *b, int n, const float *coef,
+                            int boundary);
+/* aux[i] (if aux and aux[i] are not NULL): caller-owned third buffer of the
+ * grid's byte size used by fused steps; otherwise the batch allocates one. */
+extern int PSHipBatchCreateEx(void **out, __PSGrid *const *a, __PSGrid *const *b, void *const *aux, int n,
+                              const float *coef, int boundary);
+extern int PSHipBatchDestroy(void *batch);
+/* steps steps of every grid (A->B, B->A, ...); use_graph: capture and replay. */
+extern int PSHipBatchRun(void *batch, int steps, int use_graph);
+/* The same on a caller stream (a hipStream_t), without a graph. */
+extern int PSHipBatchRunOn(void *batch, int steps, void *stream);
+/* Launch shape: points per lane np (row width 64*np), rows per workgroup by,
+ * planes per work item zchunk, temporal degree 1 or 2. */
+extern int PSHipBatchConfigure(void *batch, int np, int by, int zchunk, int degree);
+extern int PSHipBatchConfigName(void *batch, char *buf, int len);
+/* Times every candidate shape for `steps` steps on the batch (contents
+ * restored) and keeps the fastest. */
+extern int PSHipBatchTune(void *batch, int steps, float *best_ms);
+
+/* ---- stencil-solve server (no reference counterpart) ---------------------- */
+typedef struct {
+  int kind;         /* 0: 7-point diffusion; 1: 19-point Himeno-form Jacobi */
+  int n;            /* cubic edge, 3..512 */
+  int storage;      /* PS_HIP_STORAGE_NATIVE or PS_HIP_STORAGE_BF16 */
+  uint64_t seed;    /* initial field / coefficients */
+  double arrival;   /* seconds after the start of the trace */
+  int max_steps;    /* step cap */
+  double tol;       /* residual tolerance */
+} PSHipRequest;
+
+typedef struct {
+  int status;       /* PSHipStatus */
+  int steps;
+  double residual;
+  double admitted;  /* seconds after the trace start */
+  double finished;
+  double checksum;  /* FP64 sum of the final field */
+} PSHipResult;
+
+/* Serves the requests in real time.  mode 0: continuous batching under an
+ * HBM budget with batched diffusion and overlapped Jacobi streams; mode 1:
+ * one request at a time through the single-grid paths.  tick_steps (even):
+ * steps between residual checks. */
+extern int PSHipServeTrace(const PSHipRequest *reqs, int n, int64_t budget_bytes, int tick_steps, int mode,
+                           PSHipResult *results);
 
 #ifdef __cplusplus
 }

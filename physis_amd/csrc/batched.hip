@@ -38,18 +38,23 @@ struct Coef7 {
 
 // The diffusion body (reference examples/diffusion-benchmark/diffusion3d_physis.c:29-58;
 // periodic form test_7-pt-periodic.c:12-24), one value per point.
-template <bool kPeriodic>
+// Grid pointers are untyped so that one argument table serves every storage
+// kind.
+struct BatchArgs {
+  const void *in;
+  void *out;
+  int dim[3];
+  Coef7 k;
+};
+
+template <class S, bool kPeriodic>
 struct BatchPlan {
-  using Storage = psk::Native<float>;
+  using Storage = S;
+  using st = typename S::stored_t;
   static constexpr bool kWrap = kPeriodic;
-  struct Args {
-    const float *in;
-    float *out;
-    int dim[3];
-    Coef7 k;
-  };
-  __host__ __device__ static const float *stage_ptr(const Args &a) { return a.in; }
-  __host__ __device__ static float *out_ptr(const Args &a) { return a.out; }
+  using Args = BatchArgs;
+  __host__ __device__ static const st *stage_ptr(const Args &a) { return static_cast<const st *>(a.in); }
+  __host__ __device__ static st *out_ptr(const Args &a) { return static_cast<st *>(a.out); }
   template <class C>
   __device__ static float compute(const C &ctx, const Args &a, const int x, const int y, const int z) {
     const int nx = a.dim[0], ny = a.dim[1], nz = a.dim[2];
@@ -122,24 +127,38 @@ std::string ConfigName(const Config &c) {
 
 using KernelFn = const void *;
 
-template <bool kPer>
-KernelFn StepKernel(int np, int by) {
-  using P = BatchPlan<kPer>;
+template <class S, bool kPer>
+KernelFn StepKernelT(int np, int by) {
+  using P = BatchPlan<S, kPer>;
 #define PSH_K(NP, BY) \
   if (np == NP && by == BY) return reinterpret_cast<KernelFn>(&BatchStep<P, NP, BY>);
   PSH_K(1, 4) PSH_K(1, 8) PSH_K(1, 16) PSH_K(2, 4) PSH_K(2, 8) PSH_K(2, 16) PSH_K(4, 4) PSH_K(4, 8) PSH_K(4, 16)
+  PSH_K(8, 4) PSH_K(8, 8)
 #undef PSH_K
   return nullptr;
 }
 
-template <bool kPer>
-KernelFn TB2Kernel(int np, int by) {
-  using P = BatchPlan<kPer>;
+template <class S, bool kPer>
+KernelFn TB2KernelT(int np, int by) {
+  using P = BatchPlan<S, kPer>;
 #define PSH_K(NP, BY) \
   if (np == NP && by == BY) return reinterpret_cast<KernelFn>(&BatchTB2<P, NP, BY>);
-  PSH_K(1, 8) PSH_K(2, 8) PSH_K(4, 8) PSH_K(1, 16) PSH_K(2, 16) PSH_K(4, 16)
+  PSH_K(1, 8) PSH_K(2, 8) PSH_K(4, 8) PSH_K(8, 8) PSH_K(1, 16) PSH_K(2, 16) PSH_K(4, 16)
 #undef PSH_K
   return nullptr;
+}
+
+// Kernel for a storage kind (PSHipStorage) and boundary form.
+KernelFn StepKernel(int storage, bool per, int np, int by) {
+  if (storage == PS_HIP_STORAGE_BF16)
+    return per ? StepKernelT<psk::BF16, true>(np, by) : StepKernelT<psk::BF16, false>(np, by);
+  return per ? StepKernelT<psk::Native<float>, true>(np, by) : StepKernelT<psk::Native<float>, false>(np, by);
+}
+
+KernelFn TB2Kernel(int storage, bool per, int np, int by) {
+  if (storage == PS_HIP_STORAGE_BF16)
+    return per ? TB2KernelT<psk::BF16, true>(np, by) : TB2KernelT<psk::BF16, false>(np, by);
+  return per ? TB2KernelT<psk::Native<float>, true>(np, by) : TB2KernelT<psk::Native<float>, false>(np, by);
 }
 
 // A device array of T, reallocated on growth.
@@ -168,7 +187,9 @@ struct DevArray {
 struct Batch {
   std::vector<__PSGrid *> a, b;
   std::vector<Coef7> coef;
-  std::vector<float *> aux;  // per-grid third buffer (fused steps)
+  std::vector<void *> aux;   // per-grid third buffer (fused steps)
+  std::vector<bool> owns_aux;  // false: borrowed from the caller
+  int storage = PS_HIP_STORAGE_NATIVE;
   bool periodic = false;
   int max_nx = 0;
   size_t bytes = 0;
@@ -179,7 +200,7 @@ struct Batch {
     int blocks = 0;
   };
   std::map<Config, Plan> plans;
-  DevArray<BatchPlan<false>::Args> t_ab, t_ba, t_a_aux_in, t_aux_a_out, t_a_aux_out, t_aux_in, t_aux_b;
+  DevArray<BatchArgs> t_ab, t_ba, t_a_aux_in, t_aux_a_out, t_a_aux_out, t_aux_in, t_aux_b;
   bool tables_ready = false;
   struct Graph {
     hipGraphExec_t exec = nullptr;
@@ -190,21 +211,21 @@ struct Batch {
   ~Batch() {
     for (auto &g : graphs)
       if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
-    for (float *p : aux)
-      if (p) (void)hipFree(p);
+    for (size_t i = 0; i < aux.size(); ++i)
+      if (aux[i] && owns_aux[i]) (void)hipFree(aux[i]);
   }
 
   int Count() const { return (int)a.size(); }
 
   void BuildTables(hipStream_t s) {
     if (tables_ready) return;
-    using Args = BatchPlan<false>::Args;  // identical layout for both boundary forms
+    using Args = BatchArgs;
     std::vector<Args> ab, ba, a_in, aux_out_a, aux_out, aux_in, aux_b;
     for (int i = 0; i < Count(); ++i) {
       Args x;
       for (int d = 0; d < 3; ++d) x.dim[d] = a[i]->dim[d];
       x.k = coef[i];
-      float *A = static_cast<float *>(a[i]->p0), *B = static_cast<float *>(b[i]->p0), *X = aux[i];
+      void *A = a[i]->p0, *B = b[i]->p0, *X = aux[i];
       x.in = A; x.out = B; ab.push_back(x);          // map 1: A -> B
       x.in = B; x.out = A; ba.push_back(x);          // map 2: B -> A
       x.in = A; x.out = nullptr; a_in.push_back(x);  // fused input A
@@ -227,8 +248,8 @@ struct Batch {
   // unfused steps.
   bool Supports(const Config &c) const {
     if (64 * c.np < max_nx || c.zchunk <= 0) return false;
-    if (c.degree == 2 && TB2Kernel<false>(c.np, c.by) == nullptr) return false;
-    return StepKernel<false>(c.np, c.by) != nullptr;
+    if (c.degree == 2 && TB2Kernel(storage, false, c.np, c.by) == nullptr) return false;
+    return StepKernel(storage, false, c.np, c.by) != nullptr;
   }
 
   Plan &PlanFor(const Config &c, hipStream_t s) {
@@ -257,7 +278,7 @@ struct Batch {
     p.items.Assign(items, s);
     p.nitems = (int)items.size();
     // Persistent grid: as many workgroups as fit on the device at once.
-    KernelFn k = c.degree == 2 ? TB2Kernel<false>(c.np, c.by) : StepKernel<false>(c.np, c.by);
+    KernelFn k = c.degree == 2 ? TB2Kernel(storage, false, c.np, c.by) : StepKernel(storage, false, c.np, c.by);
     int per_cu = 1, cus = 1;
     CheckHip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64 * c.by, 0), "occupancy");
     hipDeviceProp_t prop;
@@ -278,16 +299,16 @@ struct Batch {
     Plan &p2 = PlanFor(c, s);
     if (p1.nitems == 0 || steps <= 0) return hipSuccess;
     dim3 block(64, c.by);
-    auto step = [&](const BatchPlan<false>::Args *t) {
-      KernelFn k = periodic ? StepKernel<true>(c.np, c.by) : StepKernel<false>(c.np, c.by);
+    auto step = [&](const BatchArgs *t) {
+      KernelFn k = StepKernel(storage, periodic, c.np, c.by);
       const void *tv = t;
       const Item *iv = p1.items.p;
       int n = p1.nitems;
       void *args[] = {&tv, &iv, &n};
       return hipLaunchKernel(k, dim3(p1.blocks), block, args, 0, s);
     };
-    auto pair = [&](const BatchPlan<false>::Args *tin, const BatchPlan<false>::Args *tout) {
-      KernelFn k = periodic ? TB2Kernel<true>(c.np, c.by) : TB2Kernel<false>(c.np, c.by);
+    auto pair = [&](const BatchArgs *tin, const BatchArgs *tout) {
+      KernelFn k = TB2Kernel(storage, periodic, c.np, c.by);
       const void *ti = tin, *to = tout;
       const Item *iv = p2.items.p;
       int n = p2.nitems;
@@ -344,7 +365,7 @@ struct Batch {
 std::vector<Config> Candidates(const Batch &b) {
   std::vector<Config> out;
   for (int deg : {1, 2})
-    for (int np : {1, 2, 4})
+    for (int np : {1, 2, 4, 8})
       for (int by : {4, 8, 16})
         for (int zc : {16, 32, 64}) {
           Config c{np, by, zc, deg};
@@ -357,7 +378,7 @@ std::vector<Config> Candidates(const Batch &b) {
 }
 
 Config DefaultConfig(const Batch &b) {
-  int np = b.max_nx <= 64 ? 1 : b.max_nx <= 128 ? 2 : 4;
+  int np = b.max_nx <= 64 ? 1 : b.max_nx <= 128 ? 2 : b.max_nx <= 256 ? 4 : 8;
   return Config{np, 8, 32, 2};
 }
 
@@ -368,18 +389,25 @@ using namespace physis_hip;
 
 extern "C" {
 
-int PSHipBatchCreate(void **out, __PSGrid *const *a, __PSGrid *const *b, int n, const float *coef,
-                     int boundary) {
+// Like PSHipBatchCreate; aux (may be NULL) supplies a third buffer per grid
+// (same byte size as the grid, owned by the caller) where aux[i] != NULL.
+int PSHipBatchCreateEx(void **out, __PSGrid *const *a, __PSGrid *const *b, void *const *aux, int n,
+                       const float *coef, int boundary) {
   if (!out || !a || !b || !coef || n <= 0) return PS_HIP_ERR_INVALID;
   if (boundary != PS_HIP_CLAMPED && boundary != PS_HIP_PERIODIC) return PS_HIP_ERR_INVALID;
   *out = nullptr;
   Batch *bt = new Batch();
   bt->periodic = boundary == PS_HIP_PERIODIC;
+  bt->storage = a[0] ? Rec(a[0])->storage : PS_HIP_STORAGE_NATIVE;
+  if (bt->storage == PS_HIP_STORAGE_FP16) {
+    delete bt;
+    return PS_HIP_ERR_UNSUPPORTED;
+  }
   for (int i = 0; i < n; ++i) {
     const __PSGrid *x = a[i], *y = b[i];
     if (!x || !y || x->num_dims != 3 || y->num_dims != 3 || x->elm_size != 4 || y->elm_size != 4 ||
-        Rec(x)->storage != PS_HIP_STORAGE_NATIVE || Rec(y)->storage != PS_HIP_STORAGE_NATIVE ||
-        x->dim[0] != y->dim[0] || x->dim[1] != y->dim[1] || x->dim[2] != y->dim[2] || x->dim[0] > 256) {
+        Rec(x)->storage != bt->storage || Rec(y)->storage != bt->storage || x->dim[0] != y->dim[0] ||
+        x->dim[1] != y->dim[1] || x->dim[2] != y->dim[2] || x->dim[0] > 512) {
       delete bt;
       return PS_HIP_ERR_INVALID;
     }
@@ -388,18 +416,33 @@ int PSHipBatchCreate(void **out, __PSGrid *const *a, __PSGrid *const *b, int n, 
     bt->coef.push_back(Coef7{coef[7 * i], coef[7 * i + 1], coef[7 * i + 2], coef[7 * i + 3], coef[7 * i + 4],
                              coef[7 * i + 5], coef[7 * i + 6]});
     bt->max_nx = std::max(bt->max_nx, x->dim[0]);
-    float *aux = nullptr;
-    if (hipMalloc(&aux, Rec(x)->bytes) != hipSuccess) {
+    void *x_aux = aux ? aux[i] : nullptr;
+    bt->owns_aux.push_back(x_aux == nullptr);
+    if (!x_aux && hipMalloc(&x_aux, Rec(x)->bytes) != hipSuccess) {
       (void)hipGetLastError();
+      bt->owns_aux.pop_back();
       delete bt;
       return PS_HIP_ERR_NOMEM;
     }
-    bt->aux.push_back(aux);
+    bt->aux.push_back(x_aux);
     bt->bytes += Rec(x)->bytes;
   }
   bt->tuned = DefaultConfig(*bt);
   *out = bt;
   return PS_HIP_OK;
+}
+
+int PSHipBatchCreate(void **out, __PSGrid *const *a, __PSGrid *const *b, int n, const float *coef,
+                     int boundary) {
+  return PSHipBatchCreateEx(out, a, b, nullptr, n, coef, boundary);
+}
+
+// Queues `steps` steps on a caller stream (a hipStream_t), without a graph.
+int PSHipBatchRunOn(void *h, int steps, void *stream) {
+  if (!h || steps < 0) return PS_HIP_ERR_INVALID;
+  Batch *b = static_cast<Batch *>(h);
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : Runtime::Get().stream();
+  return b->Enqueue(b->tuned, steps, s) == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
 }
 
 int PSHipBatchDestroy(void *h) {

@@ -98,10 +98,33 @@ extern "C" int PSHipStreamCopy(__PSGrid *dst, const __PSGrid *src) {
   return hipGetLastError() == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
 }
 
-extern "C" int PSHipGridFillUniform(__PSGrid *g, uint64_t seed, float lo, float hi) {
+namespace physis_hip {
+namespace {
+
+// p(x, y, z) = (float)(z*z) / ((nz-1)*(nz-1)) when kProfile, else `value`;
+// the profile is the Himeno initial pressure (reference
+// examples/himeno/himenobmtxpa_physis.c:313-329, Physis z = the original's i).
+template <class S>
+__global__ void __launch_bounds__(256)
+fill_pattern(typename S::stored_t *dst, int64_t n, int64_t plane, int nz, int profile, float value) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    float v = value;
+    if (profile) {
+      const int k = (int)(i / plane);
+      v = (float)(k * k) / ((nz - 1) * (nz - 1));
+    }
+    S::store_cached(dst + i, v);
+  }
+}
+
+}  // namespace
+
+// Seeded uniform fill on stream s (see PSHipGridFillUniform).
+int FillUniformOn(__PSGrid *g, uint64_t seed, float lo, float hi, hipStream_t s) {
   if (!g || !g->p0) return PS_HIP_ERR_INVALID;
   const GridRec *r = Rec(g);
-  hipStream_t s = Runtime::Get().stream();
   const int64_t n = g->num_elms;
   const unsigned nb = BlocksFor(n);
   if (r->storage == PS_HIP_STORAGE_BF16)
@@ -116,3 +139,35 @@ extern "C" int PSHipGridFillUniform(__PSGrid *g, uint64_t seed, float lo, float 
     return PS_HIP_ERR_UNSUPPORTED;
   return hipGetLastError() == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
 }
+
+// Constant fill (profile = 0) or the Himeno z profile (profile = 1) of a
+// float grid on stream s.
+int FillPatternOn(__PSGrid *g, int profile, float value, hipStream_t s) {
+  if (!g || !g->p0 || g->elm_size != 4) return PS_HIP_ERR_INVALID;
+  const GridRec *r = Rec(g);
+  const int64_t n = g->num_elms, plane = (int64_t)g->dim[0] * g->dim[1];
+  const int nz = g->num_dims == 3 ? g->dim[2] : 1;
+  if (profile && nz < 2) return PS_HIP_ERR_INVALID;
+  const unsigned nb = BlocksFor(n);
+  if (r->storage == PS_HIP_STORAGE_BF16)
+    hipLaunchKernelGGL(fill_pattern<psk::BF16>, dim3(nb), dim3(256), 0, s, (uint16_t *)g->p0, n, plane, nz, profile,
+                       value);
+  else if (r->storage == PS_HIP_STORAGE_FP16)
+    hipLaunchKernelGGL(fill_pattern<psk::FP16>, dim3(nb), dim3(256), 0, s, (uint16_t *)g->p0, n, plane, nz, profile,
+                       value);
+  else
+    hipLaunchKernelGGL(fill_pattern<psk::Native<float>>, dim3(nb), dim3(256), 0, s, (float *)g->p0, n, plane, nz,
+                       profile, value);
+  return hipGetLastError() == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
+}
+
+}  // namespace physis_hip
+
+extern "C" int PSHipGridFillUniform(__PSGrid *g, uint64_t seed, float lo, float hi) {
+  return FillUniformOn(g, seed, lo, hi, Runtime::Get().stream());
+}
+
+extern "C" int PSHipGridFillPattern(__PSGrid *g, int profile, float value) {
+  return FillPatternOn(g, profile, value, Runtime::Get().stream());
+}
+

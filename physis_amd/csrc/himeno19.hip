@@ -74,25 +74,29 @@ const Variant kVariants[] = {
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
-template <class S>
-int RunTyped(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
-             double *gosa, const Variant &v) {
-  using P = HimenoPlan<S>;
-  using st = typename S::stored_t;
-  Runtime &rt = Runtime::Get();
-  hipStream_t s = rt.stream();
+psk::Box InteriorBox(const __PSGrid *p0) {
   psk::Box box;
   for (int d = 0; d < 3; ++d) {
     box.lo[d] = 1;
     box.hi[d] = p0->dim[d] - 1;
   }
-  int nblocks = 0;
-  if (v.by == 4) nblocks = psk::ZMarchBlocks<64, 4>(box, v.zchunk);
-  else nblocks = psk::ZMarchBlocks<64, 8>(box, v.zchunk);
-  const size_t nres = sweeps > 0 ? (size_t)sweeps : 1;
-  double *scratch = static_cast<double *>(rt.Scratch((nres + (size_t)nblocks + 1) * sizeof(double)));
-  double *gosa_dev = scratch;
-  double *partials = scratch + nres;
+  return box;
+}
+
+int Blocks(const Variant &v, const psk::Box &box) {
+  return v.by == 4 ? psk::ZMarchBlocks<64, 4>(box, v.zchunk) : psk::ZMarchBlocks<64, 8>(box, v.zchunk);
+}
+
+// Queues `sweeps` sweeps p0 -> p1 -> p0 ... on stream s.  gosa_dev (may be
+// NULL) receives the FP64 sum of ss^2 of each sweep; partials needs
+// Blocks(v, box) doubles.
+template <class S>
+int EnqueueTyped(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
+                 double *gosa_dev, double *partials, const Variant &v, hipStream_t s) {
+  using P = HimenoPlan<S>;
+  using st = typename S::stored_t;
+  const psk::Box box = InteriorBox(p0);
+  const int nblocks = Blocks(v, box);
   typename P::Args a;
   for (int n = 0; n < NCOEF; ++n) a.c[n] = static_cast<const st *>(grids[n]->p0);
   for (int d = 0; d < 3; ++d) a.dim[d] = p0->dim[d];
@@ -104,11 +108,26 @@ int RunTyped(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, in
     hipError_t e = v.by == 4 ? psk::LaunchZMarch<P, 64, 4>(a, box, v.zchunk, true, s)
                              : psk::LaunchZMarch<P, 64, 8>(a, box, v.zchunk, true, s);
     if (e != hipSuccess) return PS_HIP_ERR_DEVICE;
-    if (gosa && nblocks > 0) {
+    if (gosa_dev && nblocks > 0) {
       hipLaunchKernelGGL(SumPartials, dim3(1), dim3(256), 0, s, partials, nblocks, gosa_dev + i);
       if (hipGetLastError() != hipSuccess) return PS_HIP_ERR_DEVICE;
     }
   }
+  return PS_HIP_OK;
+}
+
+template <class S>
+int RunTyped(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
+             double *gosa, const Variant &v) {
+  Runtime &rt = Runtime::Get();
+  hipStream_t s = rt.stream();
+  const int nblocks = Blocks(v, InteriorBox(p0));
+  const size_t nres = sweeps > 0 ? (size_t)sweeps : 1;
+  double *scratch = static_cast<double *>(rt.Scratch((nres + (size_t)nblocks + 1) * sizeof(double)));
+  double *gosa_dev = scratch;
+  double *partials = scratch + nres;
+  int st = EnqueueTyped<S>(p0, p1, grids, omega, sweeps, gosa ? gosa_dev : nullptr, partials, v, s);
+  if (st != PS_HIP_OK) return st;
   if (gosa && sweeps > 0) {
     if (nblocks == 0) {
       for (int i = 0; i < sweeps; ++i) gosa[i] = 0.0;
@@ -122,6 +141,23 @@ int RunTyped(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, in
 }
 
 }  // namespace
+
+int HimenoPartials(const __PSGrid *p0) { return Blocks(kVariants[0], InteriorBox(p0)); }
+
+int HimenoEnqueue(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
+                  double *gosa_dev, double *partials, hipStream_t s) {
+  const Variant &v = kVariants[0];
+  switch (Rec(p0)->storage) {
+    case PS_HIP_STORAGE_NATIVE:
+      return EnqueueTyped<psk::Native<float>>(p0, p1, grids, omega, sweeps, gosa_dev, partials, v, s);
+    case PS_HIP_STORAGE_BF16:
+      return EnqueueTyped<psk::BF16>(p0, p1, grids, omega, sweeps, gosa_dev, partials, v, s);
+    case PS_HIP_STORAGE_FP16:
+      return EnqueueTyped<psk::FP16>(p0, p1, grids, omega, sweeps, gosa_dev, partials, v, s);
+  }
+  return PS_HIP_ERR_INVALID;
+}
+
 }  // namespace physis_hip
 
 using namespace physis_hip;

@@ -88,4 +88,18 @@ void ConvertFromStorage(float *dst, const void *src, int64_t n, int storage,
 
 size_t StoredElmSize(int elm_size, int storage);
 
+// Stream-explicit building blocks used by the solve server (serve.hip).
+int FillUniformOn(__PSGrid *g, uint64_t seed, float lo, float hi, hipStream_t s);
+int FillPatternOn(__PSGrid *g, int profile, float value, hipStream_t s);
+// FP64 reduction of n elements; result at result_dev[0], partials in
+// result_dev[1 .. ReduceScratchDoubles()).
+int ReduceAsync(const void *p, int64_t n, int elm_size, int storage, enum PSReduceOp op, double *result_dev,
+                double *partials_dev, hipStream_t s);
+size_t ReduceScratchDoubles();
+// Himeno sweeps on stream s with caller scratch: gosa_dev[i] (may be NULL)
+// receives sweep i's FP64 sum of ss^2; partials needs HimenoPartials(p0).
+int HimenoEnqueue(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
+                  double *gosa_dev, double *partials, hipStream_t s);
+int HimenoPartials(const __PSGrid *p0);
+
 }  // namespace physis_hip

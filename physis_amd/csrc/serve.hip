@@ -1,0 +1,410 @@
+// serve.hip -- a continuous-batching server for stencil solves.
+//
+// Requests (7-point diffusion or 19-point Himeno-form Jacobi, cubic grids,
+// FP32 or BF16 storage) arrive over time.  The server keeps every admitted
+// request's grids resident on the device and advances all of them in
+// "ticks" of a fixed number of steps:
+//   - arrived requests wait in FIFO order and are admitted as soon as their
+//     grids fit the HBM budget (memory is returned when a request retires);
+//   - all active diffusion requests of one storage kind advance together in
+//     one batched launch per step (batched.hip) on the main stream, while the
+//     Jacobi requests sweep on a second stream, so the two kinds overlap;
+//   - after each tick every request's residual is computed on the device
+//     (diffusion: ||u_n - u_{n-1}|| / ||u_n||, FP64; Jacobi: rms of ss over
+//     the interior from the fused FP64 epilogue) and finished requests
+//     (residual <= tol, or max_steps) retire: an FP64 checksum of the final
+//     field is taken and the grids are freed.
+// Every request runs the same number of steps and the same arithmetic alone
+// or in any mix (batching does not change per-grid results, reductions have
+// a fixed order), so results are bitwise independent of the mix.
+// Mode 1 ("sequential") serves the same trace one request at a time through
+// the plain single-grid paths, as a loop of PSStencilRun calls would: the
+// baseline the batched mode is compared with.
+//
+// No reference counterpart (the reference has no serving layer).
+#include "runtime_hip.h"
+#include "physis/hip/storage.h"
+#include "physis/hip/wave_reduce.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <map>
+#include <thread>
+#include <vector>
+
+extern "C" int PSHipBatchCreateEx(void **out, __PSGrid *const *a, __PSGrid *const *b, void *const *aux, int n,
+                                  const float *coef, int boundary);
+extern "C" int PSHipBatchRunOn(void *h, int steps, void *stream);
+extern "C" int PSHipBatchDestroy(void *h);
+
+namespace physis_hip {
+namespace {
+
+constexpr int kNormThreads = 256;
+constexpr int kNormBlocks = 512;
+
+// Partial sums of (a-b)^2 and a^2 per workgroup (fixed assignment).
+template <class S>
+__global__ void __launch_bounds__(kNormThreads)
+DiffNormPass1(const typename S::stored_t *a, const typename S::stored_t *b, int64_t n, double *partials) {
+  __shared__ double red[kNormThreads / 64];
+  double d2 = 0.0, a2 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kNormThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kNormThreads) {
+    const double x = S::load(a + i), y = S::load(b + i);
+    d2 += (x - y) * (x - y);
+    a2 += x * x;
+  }
+  d2 = psk::block_reduce<kNormThreads>(d2, psk::OpSum(), red, threadIdx.x);
+  a2 = psk::block_reduce<kNormThreads>(a2, psk::OpSum(), red, threadIdx.x);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = d2;
+    partials[2 * blockIdx.x + 1] = a2;
+  }
+}
+
+__global__ void __launch_bounds__(kNormThreads) DiffNormPass2(const double *partials, int nb, double *out) {
+  __shared__ double red[kNormThreads / 64];
+  double d2 = 0.0, a2 = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kNormThreads) {
+    d2 += partials[2 * i];
+    a2 += partials[2 * i + 1];
+  }
+  d2 = psk::block_reduce<kNormThreads>(d2, psk::OpSum(), red, threadIdx.x);
+  a2 = psk::block_reduce<kNormThreads>(a2, psk::OpSum(), red, threadIdx.x);
+  if (threadIdx.x == 0) {
+    out[0] = d2;
+    out[1] = a2;
+  }
+}
+
+void DiffNorm(const __PSGrid *a, const __PSGrid *b, double *partials, double *out, hipStream_t s) {
+  const int64_t n = a->num_elms;
+  int nb = (int)std::min<int64_t>(kNormBlocks, (n + kNormThreads * 4 - 1) / (kNormThreads * 4));
+  nb = std::max(nb, 1);
+  if (Rec(a)->storage == PS_HIP_STORAGE_BF16)
+    hipLaunchKernelGGL(DiffNormPass1<psk::BF16>, dim3(nb), dim3(kNormThreads), 0, s,
+                       static_cast<const uint16_t *>(a->p0), static_cast<const uint16_t *>(b->p0), n, partials);
+  else
+    hipLaunchKernelGGL(DiffNormPass1<psk::Native<float>>, dim3(nb), dim3(kNormThreads), 0, s,
+                       static_cast<const float *>(a->p0), static_cast<const float *>(b->p0), n, partials);
+  hipLaunchKernelGGL(DiffNormPass2, dim3(1), dim3(kNormThreads), 0, s, partials, nb, out);
+}
+
+const float kDiffCoef[7] = {0.1f, 0.1f, 0.1f, 0.1f, 0.1f, 0.1f, 0.4f};
+constexpr float kOmega = 0.8f;
+
+double Seconds(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+}  // namespace physis_hip
+
+using namespace physis_hip;
+
+// PSHipRequest / PSHipResult are declared in physis_hip_ext.h.  Request
+// kinds: 0 = 7-point diffusion (clamped, cc=0.4, 0.1 neighbours, field
+// uniform [0,1) from seed); 1 = 19-point Jacobi of the Himeno form
+// (coefficients uniform [0.9,1.1], a3 in [0.15,0.18], bnd=1, wrk1=0,
+// p=(z/(n-1))^2, omega=0.8).
+
+namespace physis_hip {
+namespace {
+
+struct Live {
+  int idx = -1;
+  const PSHipRequest *rq = nullptr;
+  __PSGrid *A = nullptr, *B = nullptr;  // diffusion u_n, u_{n-1} / Jacobi p0, p1
+  void *aux = nullptr;                  // diffusion third buffer
+  __PSGrid *coef[12] = {};
+  double *dev = nullptr;                // per-request scratch (device)
+  int dev_doubles = 0;
+  size_t bytes = 0;
+  int steps = 0;
+  double residual = INFINITY;
+  double admitted = 0.0;
+};
+
+size_t RequestBytes(const PSHipRequest &r) {
+  const size_t pts = (size_t)r.n * r.n * r.n;
+  const size_t es = r.storage == PS_HIP_STORAGE_NATIVE ? 4 : 2;
+  return pts * es * (r.kind == 0 ? 3 : 14);
+}
+
+class Server {
+ public:
+  Server(const PSHipRequest *reqs, int n, int64_t budget, int tick, PSHipResult *out)
+      : reqs_(reqs), n_(n), budget_(budget), tick_(tick), out_(out) {
+    Runtime &rt = Runtime::Get();
+    main_ = rt.stream();
+    CheckHip(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreate");
+    CheckHip(hipHostMalloc(&host_, sizeof(double) * 4 * (size_t)std::max(n, 1), hipHostMallocDefault),
+             "hipHostMalloc");
+  }
+  ~Server() {
+    for (auto &b : batches_) (void)PSHipBatchDestroy(b.second.handle);
+    (void)hipStreamSynchronize(side_);
+    (void)hipStreamDestroy(side_);
+    (void)hipHostFree(host_);
+  }
+
+  int Run(bool sequential);
+
+ private:
+  int Admit(int idx, double now, hipStream_t s);
+  void Retire(Live &lv, double now);
+  void AdvanceTick(std::vector<Live *> &act, hipStream_t diff_stream, hipStream_t jac_stream, bool batched);
+  void Release(Live &lv);
+
+  struct BatchEntry {
+    void *handle = nullptr;
+    std::vector<int> ids;
+  };
+
+  const PSHipRequest *reqs_;
+  int n_;
+  int64_t budget_;
+  int tick_;
+  PSHipResult *out_;
+  hipStream_t main_, side_;
+  double *host_ = nullptr;
+  int64_t used_ = 0;
+  std::vector<Live> live_;
+  std::map<int, BatchEntry> batches_;  // key: storage kind
+  std::chrono::steady_clock::time_point t0_;
+};
+
+int Server::Admit(int idx, double now, hipStream_t s) {
+  const PSHipRequest &r = reqs_[idx];
+  Live lv;
+  lv.idx = idx;
+  lv.rq = &r;
+  lv.admitted = now;
+  const int dim[3] = {r.n, r.n, r.n};
+  auto grid = [&](__PSGrid **g) { return PSHipGridNew(g, 4, 3, dim, r.storage); };
+  int st = grid(&lv.A);
+  if (st == PS_HIP_OK) st = grid(&lv.B);
+  if (r.kind == 0) {
+    if (st == PS_HIP_OK && hipMalloc(&lv.aux, (size_t)r.n * r.n * r.n * (r.storage ? 2 : 4)) != hipSuccess) {
+      (void)hipGetLastError();
+      st = PS_HIP_ERR_NOMEM;
+    }
+    lv.dev_doubles = 2 + 2 * kNormBlocks + (int)ReduceScratchDoubles();
+  } else {
+    for (int i = 0; i < 12 && st == PS_HIP_OK; ++i) st = grid(&lv.coef[i]);
+    if (st == PS_HIP_OK) lv.dev_doubles = tick_ + HimenoPartials(lv.A) + (int)ReduceScratchDoubles() + 1;
+  }
+  if (st == PS_HIP_OK && hipMalloc(&lv.dev, sizeof(double) * lv.dev_doubles) != hipSuccess) {
+    (void)hipGetLastError();
+    st = PS_HIP_ERR_NOMEM;
+  }
+  if (st != PS_HIP_OK) {
+    Release(lv);
+    return st;
+  }
+  lv.bytes = RequestBytes(r);
+  used_ += (int64_t)lv.bytes;
+  // Initial state.
+  if (r.kind == 0) {
+    FillUniformOn(lv.A, r.seed, 0.f, 1.f, s);
+    FillPatternOn(lv.B, 0, 0.f, s);
+  } else {
+    FillPatternOn(lv.A, 1, 0.f, s);
+    FillPatternOn(lv.B, 1, 0.f, s);
+    for (int i = 0; i < 12; ++i) {
+      if (i == 3) FillUniformOn(lv.coef[i], r.seed + i, 0.15f, 0.18f, s);
+      else if (i == 10) FillPatternOn(lv.coef[i], 0, 1.f, s);
+      else if (i == 11) FillPatternOn(lv.coef[i], 0, 0.f, s);
+      else FillUniformOn(lv.coef[i], r.seed + i, 0.9f, 1.1f, s);
+    }
+  }
+  live_.push_back(lv);
+  return PS_HIP_OK;
+}
+
+void Server::Release(Live &lv) {
+  for (__PSGrid **g : {&lv.A, &lv.B}) {
+    if (*g) PSHipGridFree(*g);
+    *g = nullptr;
+  }
+  for (auto &c : lv.coef) {
+    if (c) PSHipGridFree(c);
+    c = nullptr;
+  }
+  if (lv.aux) (void)hipFree(lv.aux);
+  if (lv.dev) (void)hipFree(lv.dev);
+  lv.aux = nullptr;
+  lv.dev = nullptr;
+}
+
+void Server::Retire(Live &lv, double now) {
+  // Checksum of the final field (A holds u_n for diffusion; after an even
+  // number of sweeps p0 holds the Jacobi field).
+  double *res = lv.dev;
+  CheckHip((hipError_t)(ReduceAsync(lv.A->p0, lv.A->num_elms, 4, Rec(lv.A)->storage, PS_SUM, res, nullptr, main_) ==
+                                PS_HIP_OK
+                            ? hipSuccess
+                            : hipErrorUnknown),
+           "checksum");
+  double *slot = host_ + 2 * (size_t)n_;  // past the per-request residual slots
+  CheckHip(hipMemcpyAsync(slot, res, sizeof(double), hipMemcpyDeviceToHost, main_), "checksum copy");
+  CheckHip(hipStreamSynchronize(main_), "checksum sync");
+  PSHipResult &o = out_[lv.idx];
+  o.status = PS_HIP_OK;
+  o.steps = lv.steps;
+  o.residual = lv.residual;
+  o.admitted = lv.admitted;
+  o.finished = now;
+  o.checksum = *slot;
+  used_ -= (int64_t)lv.bytes;
+  Release(lv);
+}
+
+void Server::AdvanceTick(std::vector<Live *> &act, hipStream_t ds, hipStream_t js, bool batched) {
+  // Diffusion: one batch per storage kind (rebuilt when its membership changes).
+  std::map<int, std::vector<Live *>> groups;
+  for (Live *lv : act)
+    if (lv->rq->kind == 0) groups[lv->rq->storage].push_back(lv);
+  for (auto &g : groups) {
+    std::vector<Live *> &v = g.second;
+    if (batched) {
+      std::vector<int> ids;
+      for (Live *lv : v) ids.push_back(lv->idx);
+      BatchEntry &be = batches_[g.first];
+      if (be.ids != ids) {
+        if (be.handle) PSHipBatchDestroy(be.handle);
+        std::vector<__PSGrid *> a, b;
+        std::vector<void *> aux;
+        std::vector<float> coef;
+        for (Live *lv : v) {
+          a.push_back(lv->A);
+          b.push_back(lv->B);
+          aux.push_back(lv->aux);
+          coef.insert(coef.end(), kDiffCoef, kDiffCoef + 7);
+        }
+        if (PSHipBatchCreateEx(&be.handle, a.data(), b.data(), aux.data(), (int)v.size(), coef.data(),
+                               PS_HIP_CLAMPED) != PS_HIP_OK)
+          Fatal("serve", "batch creation failed");
+        be.ids = ids;
+      }
+      if (PSHipBatchRunOn(be.handle, tick_, ds) != PS_HIP_OK) Fatal("serve", "batched run failed");
+    } else {
+      for (Live *lv : v) {
+        __PSDomain dom = PSDomain3DNew(0, lv->rq->n, 0, lv->rq->n, 0, lv->rq->n);
+        if (PSHipDiffusion7Run(lv->A, lv->B, &dom, kDiffCoef, PS_HIP_CLAMPED, tick_, 0) != PS_HIP_OK)
+          Fatal("serve", "diffusion run failed");
+      }
+    }
+    for (Live *lv : v) DiffNorm(lv->A, lv->B, lv->dev + 2, lv->dev, ds);
+    for (Live *lv : v)
+      CheckHip(hipMemcpyAsync(host_ + 2 * lv->idx, lv->dev, 2 * sizeof(double), hipMemcpyDeviceToHost, ds),
+               "norm copy");
+  }
+  // Jacobi requests sweep on their own stream.
+  for (Live *lv : act) {
+    if (lv->rq->kind != 1) continue;
+    double *gosa = lv->dev;
+    double *partials = lv->dev + tick_;
+    if (HimenoEnqueue(lv->A, lv->B, lv->coef, kOmega, tick_, gosa, partials, js) != PS_HIP_OK)
+      Fatal("serve", "jacobi run failed");
+    CheckHip(hipMemcpyAsync(host_ + 2 * lv->idx, gosa + tick_ - 1, sizeof(double), hipMemcpyDeviceToHost, js),
+             "gosa copy");
+  }
+  CheckHip(hipStreamSynchronize(ds), "tick sync");
+  CheckHip(hipStreamSynchronize(js), "tick sync");
+  for (Live *lv : act) {
+    lv->steps += tick_;
+    const double *h = host_ + 2 * lv->idx;
+    if (lv->rq->kind == 0) {
+      lv->residual = h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0;
+    } else {
+      const double interior = std::pow((double)(lv->rq->n - 2), 3.0);
+      lv->residual = std::sqrt(h[0] / interior);
+    }
+  }
+}
+
+int Server::Run(bool sequential) {
+  t0_ = std::chrono::steady_clock::now();
+  for (int i = 0; i < n_; ++i) out_[i] = PSHipResult{PS_HIP_ERR_INVALID, 0, 0.0, 0.0, 0.0, 0.0};
+  std::vector<int> order(n_);
+  for (int i = 0; i < n_; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return reqs_[a].arrival < reqs_[b].arrival; });
+  size_t next = 0;
+  std::deque<int> waiting;
+  int done = 0;
+  live_.reserve(n_);
+  while (done < n_) {
+    const double now = Seconds(t0_);
+    while (next < order.size() && reqs_[order[next]].arrival <= now) waiting.push_back(order[next++]);
+    // Admission by HBM budget, FIFO (sequential mode: one at a time).
+    while (!waiting.empty()) {
+      const int idx = waiting.front();
+      if (sequential && !live_.empty()) break;
+      const int64_t need = (int64_t)RequestBytes(reqs_[idx]);
+      if (need > budget_) {
+        out_[idx].status = PS_HIP_ERR_NOMEM;  // can never fit
+        waiting.pop_front();
+        ++done;
+        continue;
+      }
+      if (used_ + need > budget_) break;
+      const int st = Admit(idx, Seconds(t0_), main_);
+      if (st == PS_HIP_ERR_NOMEM && !live_.empty()) break;  // wait for memory to free up
+      waiting.pop_front();
+      if (st != PS_HIP_OK) {
+        out_[idx].status = st;
+        ++done;
+      }
+    }
+    if (live_.empty()) {
+      if (next >= order.size() && waiting.empty()) break;
+      std::this_thread::sleep_for(std::chrono::microseconds(200));
+      continue;
+    }
+    CheckHip(hipStreamSynchronize(main_), "admission sync");
+    std::vector<Live *> act;
+    for (Live &lv : live_) act.push_back(&lv);
+    AdvanceTick(act, main_, sequential ? main_ : side_, !sequential);
+    // Retire finished requests.
+    const double t = Seconds(t0_);
+    std::vector<Live> keep;
+    for (Live &lv : live_) {
+      if (lv.residual <= lv.rq->tol || lv.steps >= lv.rq->max_steps) {
+        Retire(lv, t);
+        ++done;
+      } else {
+        keep.push_back(lv);
+      }
+    }
+    live_.swap(keep);
+  }
+  return PS_HIP_OK;
+}
+
+}  // namespace
+}  // namespace physis_hip
+
+extern "C" {
+
+// Serves `n` requests in real time (arrival offsets from now).  mode 0:
+// continuous batching; mode 1: one request at a time through the plain
+// single-grid paths.  budget_bytes bounds the device memory of admitted
+// requests; tick_steps (even) is the number of steps between residual checks.
+int PSHipServeTrace(const PSHipRequest *reqs, int n, int64_t budget_bytes, int tick_steps, int mode,
+                    PSHipResult *results) {
+  if (!reqs || !results || n <= 0 || budget_bytes <= 0 || tick_steps <= 0 || (tick_steps & 1)) return PS_HIP_ERR_INVALID;
+  for (int i = 0; i < n; ++i) {
+    const PSHipRequest &r = reqs[i];
+    if ((r.kind != 0 && r.kind != 1) || r.n < 3 || r.n > 512 ||
+        (r.storage != PS_HIP_STORAGE_NATIVE && r.storage != PS_HIP_STORAGE_BF16) || r.max_steps <= 0)
+      return PS_HIP_ERR_INVALID;
+  }
+  Runtime::Get().EnsureDevice();
+  Server srv(reqs, n, budget_bytes, tick_steps, results);
+  return srv.Run(mode == 1);
+}
+
+}  // extern "C"

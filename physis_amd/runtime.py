@@ -281,6 +281,37 @@ class Batch:
             self.handle = None
 
 
+class Request(ctypes.Structure):
+    """PSHipRequest: kind 0 = 7-point diffusion, 1 = 19-point Himeno-form Jacobi."""
+    _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_int), ("storage", ctypes.c_int), ("seed", ctypes.c_uint64),
+                ("arrival", ctypes.c_double), ("max_steps", ctypes.c_int), ("tol", ctypes.c_double)]
+
+
+class Result(ctypes.Structure):
+    """PSHipResult."""
+    _fields_ = [("status", ctypes.c_int), ("steps", ctypes.c_int), ("residual", ctypes.c_double),
+                ("admitted", ctypes.c_double), ("finished", ctypes.c_double), ("checksum", ctypes.c_double)]
+
+
+SERVE_BATCHED, SERVE_SEQUENTIAL = 0, 1
+
+
+def serve_trace(requests, budget_bytes, tick_steps=50, mode=SERVE_BATCHED):
+    """Serves a list of request dicts (kind, n, storage, seed, arrival,
+    max_steps, tol) in real time; returns one result dict per request."""
+    L = lib()
+    L.PSHipServeTrace.argtypes = [ctypes.POINTER(Request), ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(Result)]
+    n = len(requests)
+    reqs = (Request * n)(*[Request(r["kind"], r["n"], r.get("storage", STORAGE_NATIVE), r.get("seed", 1),
+                                   r.get("arrival", 0.0), r.get("max_steps", 2000), r.get("tol", 1e-5))
+                           for r in requests])
+    res = (Result * n)()
+    check(L.PSHipServeTrace(reqs, n, budget_bytes, tick_steps, mode, res), "PSHipServeTrace")
+    return [{"status": r.status, "steps": r.steps, "residual": r.residual, "admitted": r.admitted,
+             "finished": r.finished, "checksum": r.checksum} for r in res]
+
+
 class Timer:
     """HIP events on the runtime stream."""
 

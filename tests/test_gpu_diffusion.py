@@ -135,6 +135,20 @@ def test_16bit_storage_vs_fp64(rt, storage, tol):
     assert err < tol
 
 
+@pytest.mark.parametrize("storage", [1, 2])
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_16bit_fused_equals_unfused(rt, storage, boundary):
+    """With 16-bit storage the fused kernel rounds its on-chip intermediate
+    exactly as a store/load would, so it matches separate steps bitwise."""
+    field = O.uniform_field((64, 48, 40), seed=21)
+    names = rt.diffusion7_variants()
+    one = [v for v, n in names.items() if n == "star4_by8_z128"][0]
+    fused = [v for v, n in names.items() if n.startswith("tb2_w1")][0]
+    a1, a2 = run_hip(rt, field, K_ASYM, boundary, 10, variant=one, storage=storage)
+    b1, b2 = run_hip(rt, field, K_ASYM, boundary, 10, variant=fused, storage=storage)
+    assert np.array_equal(a1, b1) and np.array_equal(a2, b2)
+
+
 def test_invalid_arguments_are_rejected(rt):
     g1 = rt.Grid((8, 8, 8))
     g2 = rt.Grid((8, 8, 4))

@@ -1,0 +1,98 @@
+"""The continuous-batching solve server on the GPU.
+
+Each request's result (step count and the FP64 checksum of its final field)
+must be bitwise the same served alone, in the mix, under a tight HBM budget,
+and in the one-at-a-time sequential mode; FP32 requests also match the CPU
+oracle run for the same number of steps."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+MIX = [
+    dict(kind=0, n=24, storage=0, seed=11, arrival=0.000, max_steps=120, tol=1e-5),
+    dict(kind=1, n=19, storage=0, seed=12, arrival=0.002, max_steps=120, tol=1e-5),
+    dict(kind=0, n=37, storage=1, seed=13, arrival=0.004, max_steps=80, tol=1e-5),
+    dict(kind=0, n=50, storage=0, seed=14, arrival=0.006, max_steps=120, tol=1e-5),
+    dict(kind=1, n=33, storage=1, seed=15, arrival=0.008, max_steps=60, tol=1e-5),
+    dict(kind=0, n=8, storage=0, seed=16, arrival=0.010, max_steps=2000, tol=1e-3),
+    dict(kind=0, n=61, storage=0, seed=17, arrival=0.012, max_steps=40, tol=1e-5),
+    dict(kind=1, n=26, storage=0, seed=18, arrival=0.014, max_steps=100, tol=1e-5),
+]
+TICK = 20
+BIG_BUDGET = 8 << 30
+
+
+def key(r):
+    return (r["status"], r["steps"], r["checksum"])
+
+
+@pytest.fixture(scope="module")
+def mixed(rt):
+    return rt.serve_trace(MIX, BIG_BUDGET, TICK, rt.SERVE_BATCHED)
+
+
+def test_all_requests_complete(rt, mixed):
+    for r, q in zip(mixed, MIX):
+        assert r["status"] == 0
+        assert r["steps"] % TICK == 0 and r["steps"] >= TICK
+        assert r["steps"] <= q["max_steps"] + TICK - 1
+        assert r["finished"] >= r["admitted"] >= 0.0
+        if r["steps"] < q["max_steps"]:
+            assert r["residual"] <= q["tol"]
+    # The small diffusion request converges to its loose tolerance early.
+    assert mixed[5]["steps"] < 2000
+
+
+def test_same_alone_and_in_the_mix(rt, mixed):
+    for q, r in zip(MIX, mixed):
+        alone = rt.serve_trace([dict(q, arrival=0.0)], BIG_BUDGET, TICK, rt.SERVE_BATCHED)[0]
+        assert key(alone) == key(r), q
+
+
+def test_sequential_mode_gives_identical_results(rt, mixed):
+    seq = rt.serve_trace(MIX, BIG_BUDGET, TICK, rt.SERVE_SEQUENTIAL)
+    for q, a, b in zip(MIX, mixed, seq):
+        assert key(a) == key(b), q
+
+
+def test_tight_budget_admits_one_at_a_time_with_same_results(rt, mixed):
+    biggest = max(q["n"] ** 3 * (4 if q["storage"] == 0 else 2) * (3 if q["kind"] == 0 else 14) for q in MIX)
+    res = rt.serve_trace(MIX, biggest, TICK, rt.SERVE_BATCHED)
+    for q, a, b in zip(MIX, mixed, res):
+        assert key(a) == key(b), q
+
+
+def test_request_larger_than_budget_is_rejected(rt):
+    res = rt.serve_trace([dict(kind=0, n=64, storage=0, seed=1, arrival=0.0, max_steps=20, tol=0.0)],
+                         64 ** 3 * 4 * 3 - 1, TICK, rt.SERVE_BATCHED)
+    assert res[0]["status"] == 2
+
+
+def test_fp32_results_match_oracle(rt, mixed):
+    for q, r in zip(MIX, mixed):
+        if q["storage"] != 0:
+            continue
+        n, steps = q["n"], r["steps"]
+        if q["kind"] == 0:
+            field = O.uniform_field((n, n, n), seed=q["seed"])
+            g1, _ = O.diffusion7(field, [0.1] * 6 + [0.4], periodic=False, steps=steps)
+            want = float(np.sum(g1.astype(np.float64)))
+        else:
+            coefs = []
+            for i in range(12):
+                if i == 3:
+                    coefs.append(O.uniform_field((n, n, n), seed=q["seed"] + i, lo=0.15, hi=0.18))
+                elif i == 10:
+                    coefs.append(np.ones((n, n, n), np.float32))
+                elif i == 11:
+                    coefs.append(np.zeros((n, n, n), np.float32))
+                else:
+                    coefs.append(O.uniform_field((n, n, n), seed=q["seed"] + i, lo=0.9, hi=1.1))
+            z = np.arange(n, dtype=np.float32)
+            p = np.broadcast_to((z * z / np.float32((n - 1) * (n - 1)))[:, None, None], (n, n, n)).astype(np.float32)
+            p0, _, _ = O.himeno(p, coefs, 0.8, steps)
+            want = float(np.sum(p0.astype(np.float64)))
+        assert abs(r["checksum"] - want) <= 1e-12 * max(1.0, abs(want)), (q, r["checksum"], want)
