@@ -155,6 +155,52 @@ def served_bench(rt, n_req=24, rate=40.0, seed=7, budget_gib=48, tick=50):
     return out
 
 
+def himeno_bench(rt, sweeps=400, seed=300):
+    """Config 3: 19-point Jacobi (Himeno form) on 257x129x129, coefficients
+    a0..a2, b0..b2, c0..c2 uniform [0.9, 1.1], a3 uniform [0.15, 0.18], bnd 1,
+    wrk1 0, p = (z/(nz-1))^2, omega 0.8, 400 sweeps with the FP64 sum of ss^2
+    per sweep; FP32 and BF16 storage.  Bytes per sweep follow the reference's
+    GetThroughput (himenobmtxpa_physis.c:418-432)."""
+    shape = (257, 129, 129)
+    nx, ny, nz = shape
+    interior = (nx - 2) * (ny - 2) * (nz - 2)
+    whole = nx * ny * nz
+    out = {"workload": "Himeno 19-pt Jacobi %dx%dx%d, %d sweeps, omega 0.8, FP64 residual per sweep" % (
+        nx, ny, nz, sweeps)}
+    for storage, name in ((rt.STORAGE_NATIVE, "fp32"), (rt.STORAGE_BF16, "bf16")):
+        grids = []
+        for i in range(12):
+            g = rt.Grid(shape, storage=storage)
+            if i == 3:
+                g.fill_uniform(seed + i, 0.15, 0.18)
+            elif i == 10:
+                rt.lib().PSHipGridFillPattern(g.handle, 0, 1.0)
+            elif i == 11:
+                rt.lib().PSHipGridFillPattern(g.handle, 0, 0.0)
+            else:
+                g.fill_uniform(seed + i, 0.9, 1.1)
+            grids.append(g)
+        p0, p1 = rt.Grid(shape, storage=storage), rt.Grid(shape, storage=storage)
+        rt.lib().PSHipGridFillPattern(p0.handle, 1, 0.0)
+        rt.lib().PSHipGridFillPattern(p1.handle, 1, 0.0)
+        rt.himeno_run(p0, p1, grids, 0.8, 4)  # warm
+        timer = rt.Timer()
+        rt.synchronize()
+        t0 = time.perf_counter()
+        timer.begin()
+        rt.himeno_run(p0, p1, grids, 0.8, sweeps)
+        timer.end()
+        wall = time.perf_counter() - t0
+        dev = timer.elapsed_ms() / 1e3
+        es = 4 if storage == rt.STORAGE_NATIVE else 2
+        bytes_sweep = es * (12 * interior + whole + interior)
+        out[name] = {"ms_per_sweep": round(dev / sweeps * 1e3, 4), "glups": round(interior * sweeps / dev / 1e9, 1),
+                     "gbs": round(bytes_sweep * sweeps / dev / 1e9, 1), "wall_s": round(wall, 3)}
+        for g in grids + [p0, p1]:
+            g.free()
+    return out
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "diffusion7_512_traffic.json")
     try:
@@ -226,6 +272,8 @@ def main():
     if not args.no_batched:
         batched = batched_bench(rt)
         print("batched: %s" % json.dumps(batched), file=sys.stderr, flush=True)
+    himeno = himeno_bench(rt)
+    print("himeno: %s" % json.dumps(himeno), file=sys.stderr, flush=True)
     served = None
     if not args.no_served:
         served = served_bench(rt)
@@ -256,6 +304,7 @@ def main():
                    "hbm_roofline_pct": round(100 * achieved / HBM_PEAK_GBS, 2),
                    "stream_copy_gbs": round(copy_gbs, 1),
                    "batched": batched,
+                   "himeno": himeno,
                    "served": served},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},

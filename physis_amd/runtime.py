@@ -83,6 +83,7 @@ def lib():
             L.PSHipHimenoRun.argtypes = [_GP, _GP, ctypes.POINTER(_GP), ctypes.c_float, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.PSHipStreamCopy.argtypes = [_GP, _GP]
+        L.PSHipGridFillPattern.argtypes = [_GP, ctypes.c_int, ctypes.c_float]
         if hasattr(L, "PSHipGridFillUniform"):
             L.PSHipGridFillUniform.argtypes = [_GP, ctypes.c_uint64, ctypes.c_float, ctypes.c_float]
         _lib = L

@@ -83,7 +83,7 @@ def test_tune_restores_grids_and_picks_a_config(rt):
 
 
 def test_batch_rejects_wide_or_mismatched_grids(rt):
-    a, b = rt.Grid((300, 4, 4)), rt.Grid((300, 4, 4))
+    a, b = rt.Grid((600, 4, 4)), rt.Grid((600, 4, 4))
     with pytest.raises(rt.PhysisError):
         rt.Batch([(a, b)], [[0.1] * 6 + [0.4]])
     c, d = rt.Grid((8, 4, 4)), rt.Grid((8, 4, 5))

@@ -74,6 +74,27 @@ def test_himeno_config3_shape_400_sweeps_vs_fp64(rt):
     np.testing.assert_allclose(gosa, wgosa, rtol=1e-4)
 
 
+def test_himeno_convergent_form_400_sweeps_vs_fp64(rt):
+    """With the config-3 coefficient ranges (b-terms ~1) the relaxation
+    diverges (|p| ~ 3e3 after 40 sweeps, ~1e63 after 400 in FP64), so 400
+    sweeps are checked on the original Himeno coefficients (a=c=1, b=0,
+    a3=1/6), which converge."""
+    shape = (65, 33, 33)
+    nx, ny, nz = shape
+    vals = [1.0, 1.0, 1.0, 1.0 / 6.0, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, 1.0, 0.0]
+    coefs = [np.full((nz, ny, nx), np.float32(v), np.float32) for v in vals]
+    z = np.arange(nz, dtype=np.float32)
+    p = np.broadcast_to((z * z / np.float32((nz - 1) * (nz - 1)))[:, None, None], (nz, ny, nx)).astype(np.float32)
+    got0, _, gosa = run_hip(rt, p, coefs, 0.8, 400)
+    want0, _, wgosa = O.himeno(p.astype(np.float64), [c.astype(np.float64) for c in coefs], 0.8, 400,
+                               dtype=np.float64)
+    rel = np.max(np.abs(got0 - want0)) / np.max(np.abs(want0))
+    print("FP32 storage, 400 sweeps: max rel error vs FP64", rel, flush=True)
+    assert rel < 1e-5
+    assert gosa[-1] < gosa[0]
+    np.testing.assert_allclose(gosa, wgosa, rtol=1e-3)
+
+
 @pytest.mark.parametrize("storage,tol", [(1, 3e-2), (2, 5e-3)])
 def test_himeno_16bit_storage_vs_fp64(rt, storage, tol):
     shape = (65, 33, 33)
