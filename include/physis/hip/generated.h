@@ -81,11 +81,12 @@ inline int FloatStorage(const __PSGrid *const *grids, int n) {
 }
 
 // Launch shape for the one-step LDS-ring skeleton by map rank.
+// parity: -1 for every point, 0/1 for the red/black points of a red-black map.
 template <class P, int RANK>
-inline hipError_t LaunchRing(const typename P::Args &a, const Box &box, hipStream_t s) {
-  if (RANK == 1) return LaunchZMarch<P, 64, 1>(a, box, 0, true, s);
-  if (RANK == 2) return LaunchZMarch<P, 64, 4>(a, box, 0, true, s);
-  return LaunchZMarch<P, 64, 4>(a, box, 64, true, s);
+inline hipError_t LaunchRing(const typename P::Args &a, const Box &box, hipStream_t s, int parity = -1) {
+  if (RANK == 1) return LaunchZMarch<P, 64, 1>(a, box, 0, true, s, parity);
+  if (RANK == 2) return LaunchZMarch<P, 64, 4>(a, box, 0, true, s, parity);
+  return LaunchZMarch<P, 64, 4>(a, box, 64, true, s, parity);
 }
 
 // One step of a star plan: the vectorised skeleton when rows are whole

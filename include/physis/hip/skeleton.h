@@ -69,6 +69,7 @@ struct Geometry {
   int tiles_y;
   int zchunk;  // output planes per workgroup
   int remap;   // XCD-aware tile numbering
+  int parity;  // -1: every point; 0 (red) / 1 (black): points with (x+y+z+parity) even
 };
 
 constexpr int kNumXcd = 8;
@@ -123,7 +124,7 @@ ZMarch(typename Plan::Args a, Box box, Geometry geo) {
   const int ze = min(zb + geo.zchunk, box.hi[2]);
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
   const int x = x0 + tx, y = y0 + ty;
-  const bool active = x < box.hi[0] && y < box.hi[1];
+  const bool in_box = x < box.hi[0] && y < box.hi[1];
 
   SV buf[NSA][PER];
   int goff[PER];
@@ -177,6 +178,9 @@ ZMarch(typename Plan::Args a, Box box, Geometry geo) {
   c.center = (ty + RY) * TW + tx + RX;
   double acc = 0.0;
   for (int z = zb; z < ze; ++z) {
+    // Red-black maps (reference translator/reference_translator.cc:462-477)
+    // update only the points of one colour.
+    const bool active = in_box && (geo.parity < 0 || ((x + y + z + geo.parity) & 1) == 0);
     if constexpr (NS > 0) {
       store_plane(z + RZ);
       __syncthreads();
@@ -210,7 +214,7 @@ inline int ZMarchBlocks(const Box &box, int zchunk) {
 // Host-side launch of a plan over a box with tile BX x BY.
 template <class Plan, int BX, int BY>
 inline hipError_t LaunchZMarch(const typename Plan::Args &a, const Box &box, int zchunk,
-                               bool remap, hipStream_t stream) {
+                               bool remap, hipStream_t stream, int parity = -1) {
   const int ex = box.hi[0] - box.lo[0], ey = box.hi[1] - box.lo[1], ez = box.hi[2] - box.lo[2];
   if (ex <= 0 || ey <= 0 || ez <= 0) return hipSuccess;  // empty domain: nothing to run
   Geometry geo;
@@ -219,6 +223,7 @@ inline hipError_t LaunchZMarch(const typename Plan::Args &a, const Box &box, int
   if (zchunk <= 0 || zchunk > ez) zchunk = ez;
   geo.zchunk = zchunk;
   geo.remap = remap ? 1 : 0;
+  geo.parity = parity;
   const int nchunks = (ez + zchunk - 1) / zchunk;
   dim3 grid(geo.tiles_x * geo.tiles_y, nchunks, 1);
   dim3 block(BX, BY, 1);

@@ -390,7 +390,9 @@ class Translator {
   std::string EmitPlan(const Kernel &k);
   std::string EmitMapAndLaunch(const Kernel &k);
   std::string RewriteHost(size_t b, size_t e);
-  std::string EmitRun(const std::vector<std::pair<std::string, std::string>> &maps, int id);
+  // modes[i]: 0 plain map, 1 red-black (red then black pass), 2 red only, 3 black only.
+  std::string EmitRun(const std::vector<std::pair<std::string, std::string>> &maps, const std::vector<int> &modes,
+                      int id);
 
   std::vector<Tok> t_;
   Config cfg_;
@@ -720,16 +722,16 @@ std::string Translator::EmitMapAndLaunch(const Kernel &k) {
   }
   o << "  return a;\n}\n\n";
   // One map application.
+  // parity: -1 all points; 0/1 the red/black points of a red-black map.
   o << "template <class SF>\nstatic void __PSLaunchT_" << f.name << "(const struct " << S
-    << " *s, hipStream_t st) {\n";
+    << " *s, hipStream_t st, int parity) {\n";
   o << "  typename " << P << "<SF>::Args a = __PSArgs_" << f.name << "<SF>(s);\n";
   o << "  psk::Box box = psk::BoxFromDomain(s->dom, " << k.rank << ");\n";
   if (k.star)
-    o << "  psk::CheckLaunch(psk::LaunchStarStep<" << P << "<SF>>(a, box, a." << k.staged << ".dim[0], st), \""
-      << f.name << "\");\n";
-  else
-    o << "  psk::CheckLaunch(psk::LaunchRing<" << P << "<SF>, " << k.rank << ">(a, box, st), \"" << f.name
-      << "\");\n";
+    o << "  if (parity < 0) {\n    psk::CheckLaunch(psk::LaunchStarStep<" << P << "<SF>>(a, box, a." << k.staged
+      << ".dim[0], st), \"" << f.name << "\");\n    return;\n  }\n";
+  o << "  psk::CheckLaunch(psk::LaunchRing<" << P << "<SF>, " << k.rank << ">(a, box, st, parity), \"" << f.name
+    << "\");\n";
   o << "}\n\n";
   // Storage dispatch.
   std::vector<std::string> floats;
@@ -744,12 +746,12 @@ std::string Translator::EmitMapAndLaunch(const Kernel &k) {
     o << "};\n  return psk::FloatStorage(g, " << floats.size() << ");\n";
   }
   o << "}\n\n";
-  o << "static void __PSLaunch_" << f.name << "(const struct " << S << " *s) {\n";
+  o << "static void __PSLaunch_" << f.name << "(const struct " << S << " *s, int parity) {\n";
   o << "  hipStream_t st = (hipStream_t)__PSGetStream();\n";
   o << "  switch (__PSStorage_" << f.name << "(s)) {\n";
-  o << "    case PS_HIP_STORAGE_BF16: __PSLaunchT_" << f.name << "<psk::BF16>(s, st); break;\n";
-  o << "    case PS_HIP_STORAGE_FP16: __PSLaunchT_" << f.name << "<psk::FP16>(s, st); break;\n";
-  o << "    default: __PSLaunchT_" << f.name << "<psk::Native<float>>(s, st); break;\n";
+  o << "    case PS_HIP_STORAGE_BF16: __PSLaunchT_" << f.name << "<psk::BF16>(s, st, parity); break;\n";
+  o << "    case PS_HIP_STORAGE_FP16: __PSLaunchT_" << f.name << "<psk::FP16>(s, st, parity); break;\n";
+  o << "    default: __PSLaunchT_" << f.name << "<psk::Native<float>>(s, st, parity); break;\n";
   o << "  }\n";
   if (cfg_.error_check) o << "  __PSCheckDeviceError(\"" << f.name << "\");\n";
   o << "}\n\n";
@@ -790,7 +792,8 @@ std::string Translator::EmitMapAndLaunch(const Kernel &k) {
   return o.str();
 }
 
-std::string Translator::EmitRun(const std::vector<std::pair<std::string, std::string>> &maps, int id) {
+std::string Translator::EmitRun(const std::vector<std::pair<std::string, std::string>> &maps,
+                                const std::vector<int> &modes, int id) {
   // maps: (kernel name, argument text of the map call)
   std::ostringstream o;
   o << "static float __PSStencilRun_" << id << "(int iter";
@@ -802,14 +805,24 @@ std::string Translator::EmitRun(const std::vector<std::pair<std::string, std::st
     o << "\");\n  __PSStopwatch __st;\n  __PSStopwatchStart(&__st);\n";
   }
   bool fusable = cfg_.fuse && maps.size() == 2 && maps[0].first == maps[1].first &&
-                 kernels_.at(maps[0].first).star;
+                 kernels_.at(maps[0].first).star && modes[0] == 0 && modes[1] == 0;
   if (fusable) {
     const std::string &kn = maps[0].first;
     o << "  if (iter > 1 && __PSFusable_" << kn << "(&s0, &s1)) {\n";
     o << "    __PSRunPingPong_" << kn << "(&s0, &s1, iter);\n  } else {\n";
   }
   o << "  for (int i = 0; i < iter; ++i) {\n";
-  for (size_t i = 0; i < maps.size(); ++i) o << "    __PSLaunch_" << maps[i].first << "(&s" << i << ");\n";
+  // A red-black map runs its red then its black pass (reference
+  // reference_translator.cc:628-640); MapRed / MapBlack run one colour.
+  for (size_t i = 0; i < maps.size(); ++i) {
+    const std::string call = "    __PSLaunch_" + maps[i].first + "(&s" + std::to_string(i);
+    switch (modes[i]) {
+      case 1: o << call << ", 0);\n" << call << ", 1);\n"; break;
+      case 2: o << call << ", 0);\n"; break;
+      case 3: o << call << ", 1);\n"; break;
+      default: o << call << ", -1);\n"; break;
+    }
+  }
   o << "  }\n";
   if (fusable) o << "  }\n";
   if (cfg_.trace) {
@@ -872,17 +885,19 @@ std::string Translator::RewriteHost(size_t b, size_t e) {
       i = close;
       continue;
     }
-    if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" &&
-        (tk.s == "PSStencilMapRedBlack" || tk.s == "PSStencilMapRed" || tk.s == "PSStencilMapBlack"))
-      Die(tk.s + " is not supported by this target");
     if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" && tk.s == "PSStencilRun") {
       size_t close = MatchClose(t_, i + 1);
       auto args = SplitArgs(t_, i + 2, close);
       std::vector<std::pair<std::string, std::string>> maps;
+      std::vector<int> modes;
       std::string iter = "1";
+      static const std::map<std::string, int> kMapModes = {
+          {"PSStencilMap", 0}, {"PSStencilMapRedBlack", 1}, {"PSStencilMapRed", 2}, {"PSStencilMapBlack", 3}};
       for (size_t a = 0; a < args.size(); ++a) {
         size_t ab = args[a].first, ae = args[a].second;
-        if (t_[ab].s == "PSStencilMap" && ab + 1 < ae && t_[ab + 1].s == "(") {
+        auto mode = kMapModes.find(t_[ab].s);
+        if (mode != kMapModes.end() && ab + 1 < ae && t_[ab + 1].s == "(") {
+          modes.push_back(mode->second);
           size_t mc = MatchClose(t_, ab + 1);
           auto margs = SplitArgs(t_, ab + 2, mc);
           std::string kn = Text(t_, margs[0].first, margs[0].second);
@@ -899,7 +914,7 @@ std::string Translator::RewriteHost(size_t b, size_t e) {
         }
       }
       int id = run_id_++;
-      runs_.push_back(EmitRun(maps, id));
+      runs_.push_back(EmitRun(maps, modes, id));
       std::string call = "__PSStencilRun_" + std::to_string(id) + "(" + iter;
       for (auto &m : maps) call += ", " + m.second;
       call += ")";
@@ -929,7 +944,6 @@ std::string Translator::Run(const std::vector<std::string> &includes) {
     if (t_[i].kind == Tok::kIdent && t_[i + 1].s == "(" &&
         (t_[i].s == "PSStencilMap" || t_[i].s == "PSStencilMapRedBlack" || t_[i].s == "PSStencilMapRed" ||
          t_[i].s == "PSStencilMapBlack")) {
-      if (t_[i].s != "PSStencilMap") Die(t_[i].s + " is not supported by this target");
       const std::string &kn = t_[i + 2].s;
       auto it = byname.find(kn);
       if (it == byname.end()) Die("kernel '" + kn + "' is not defined in this file");
