@@ -125,6 +125,52 @@ extern void __PSReduceGridDouble(void *buf, enum PSReduceOp op, __PSGrid *g);
  * (a hipStream_t). */
 extern void *__PSGetStream(void);
 
+/* ---- user-defined point types ----------------------------------------------
+ * A grid of a C struct point type (DeclareGrid3D(Name, struct T)) lives in
+ * HBM as a structure of arrays: each scalar of T -- every element of an
+ * array member counts as one -- is a plane of num_elms values, x fastest.
+ * Planes are plane_stride elements apart (num_elms rounded up to 64, so each
+ * plane starts 256-byte aligned) and ordered by member, then by the member's
+ * row-major array index.  Kernels then load one member of 64 neighbouring
+ * points with one coalesced access.
+ *
+ * physisc-hip describes T with a PSHipUtype and emits the devNewFunc /
+ * devFreeFunc / devCopyinFunc / devCopyoutFunc that __PSGridNew,
+ * __PSGridFree, __PSGridCopyin and __PSGridCopyout take, as thin wrappers
+ * of the calls below.  They replace the per-type functions the reference's
+ * CUDA builder generates (translator/cuda_runtime_builder.cc:319-368 device
+ * type, :369-473 new, :474-718 free, :719-871 copyin/copyout), which
+ * transpose on the host; here the AoS <-> SoA transpose runs on the device.
+ * The plain PSGridCopyin/PSGridCopyout/PSGridFree also accept such grids. */
+typedef struct {
+  int elm_size;            /* sizeof(T) on the host */
+  int num_members;
+  const int *offset;       /* byte offset of member m in T */
+  const int *scalar_size;  /* bytes of one scalar of member m: 1, 2, 4 or 8 */
+  const int *count;        /* scalars in member m (product of its extents) */
+} PSHipUtype;
+
+/* The device representation a devNewFunc returns (g->dev of such a grid). */
+typedef struct {
+  int dim[3];
+  void *p0;                /* first plane */
+  int64_t num_elms;
+  int64_t plane_stride;    /* elements from one plane to the next */
+  size_t bytes;
+  const PSHipUtype *type;
+  unsigned magic;
+} __PSGridUtype_dev;
+
+/* Allocates the zero-filled planes of a grid of `type` (aborts on error). */
+extern void *__PSHipUtypeDevNew(const PSHipUtype *type, int num_dims,
+                                PSVectorInt dim);
+extern void __PSHipUtypeDevFree(void *dev);
+/* Blocking host AoS array -> device planes, and back. */
+extern void __PSHipUtypeCopyin(void *dev, const void *src);
+extern void __PSHipUtypeCopyout(void *dev, void *dst);
+/* Device address of the first plane of member m of a user-type grid. */
+extern void *__PSHipUtypeMember(const __PSGrid *g, int member);
+
 /* ---- linear offsets (x fastest) ------------------------------------------ */
 static inline PSIndex __PSGridGetOffset1D(__PSGrid *g, PSIndex i1) {
   (void)g;

@@ -165,6 +165,9 @@ int GridNewImpl(__PSGrid **out, int elm_size, int num_dims, const int *dim,
     // User-defined point type: the generated helper builds the device form.
     r->user_type = true;
     g->dev = static_cast<__PSGrid_dev *>(func(num_dims, g->dim));
+    // p0 stays NULL: calls that address a grid as one array of primitive
+    // elements (fills, reductions, device copies, built-in programs) reject
+    // it; the planes are reached through __PSHipUtypeMember.
     g->p0 = nullptr;
     *out = g;
     return PS_HIP_OK;
@@ -261,7 +264,10 @@ int PSHipGridFree(__PSGrid *g) {
   if (!g) return PS_HIP_ERR_INVALID;
   GridRec *r = Rec(g);
   Runtime &rt = Runtime::Get();
-  if (g->dev && !r->user_type && g->p0) {
+  if (r->user_type) {
+    if (g->dev && !IsUtypeDev(g->dev)) return PS_HIP_ERR_INVALID;  // needs its own devFreeFunc
+    if (g->dev && UtypeFree(g->dev) != PS_HIP_OK) return PS_HIP_ERR_DEVICE;
+  } else if (g->dev && g->p0) {
     // Kernels queued on the stream may still read the buffer.
     if (hipStreamSynchronize(rt.stream()) != hipSuccess) return PS_HIP_ERR_DEVICE;
     if (hipFree(g->p0) != hipSuccess) return PS_HIP_ERR_DEVICE;
@@ -287,8 +293,10 @@ void __PSGridFree(__PSGrid *g, __PSGrid_devFreeFunc func) {
 void PSGridFree(void *g) { __PSGridFree(static_cast<__PSGrid *>(g), nullptr); }
 
 int PSHipGridCopyin(__PSGrid *g, const void *src) {
-  if (!g || !src || !g->p0) return PS_HIP_ERR_INVALID;
+  if (!g || !src) return PS_HIP_ERR_INVALID;
   GridRec *r = Rec(g);
+  if (r->user_type) return UtypeCopyin(g->dev, src);
+  if (!g->p0) return PS_HIP_ERR_INVALID;
   Runtime &rt = Runtime::Get();
   hipStream_t s = rt.stream();
   if (r->storage == PS_HIP_STORAGE_NATIVE) {
@@ -312,8 +320,10 @@ int PSHipGridCopyin(__PSGrid *g, const void *src) {
 }
 
 int PSHipGridCopyout(__PSGrid *g, void *dst) {
-  if (!g || !dst || !g->p0) return PS_HIP_ERR_INVALID;
+  if (!g || !dst) return PS_HIP_ERR_INVALID;
   GridRec *r = Rec(g);
+  if (r->user_type) return UtypeCopyout(g->dev, dst);
+  if (!g->p0) return PS_HIP_ERR_INVALID;
   Runtime &rt = Runtime::Get();
   hipStream_t s = rt.stream();
   if (r->storage == PS_HIP_STORAGE_NATIVE) {
@@ -385,6 +395,7 @@ void __PSGridSet(__PSGrid *g, void *buf, ...) {
   }
   va_end(vl);
   GridRec *r = Rec(g);
+  if (r->user_type) Fatal("__PSGridSet", "not supported for user-defined point types");
   hipStream_t s = Runtime::Get().stream();
   if (r->storage == PS_HIP_STORAGE_NATIVE) {
     PSH_CHECK(hipMemcpyAsync(static_cast<char *>(g->p0) + offset * g->elm_size, buf,

@@ -102,4 +102,11 @@ int HimenoEnqueue(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omeg
                   double *gosa_dev, double *partials, hipStream_t s);
 int HimenoPartials(const __PSGrid *p0);
 
+// User-defined point types (utype_hip.hip): `dev` is the __PSGridUtype_dev a
+// generated devNewFunc returned through __PSHipUtypeDevNew.
+bool IsUtypeDev(const void *dev);
+int UtypeCopyin(void *dev, const void *src);
+int UtypeCopyout(void *dev, void *dst);
+int UtypeFree(void *dev);
+
 }  // namespace physis_hip

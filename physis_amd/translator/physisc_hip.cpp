@@ -30,8 +30,17 @@
 // one value becomes a star plan (vectorised skeleton, and two maps of a
 // ping-pong PSStencilRun fuse into one pass per two steps).
 //
-// Not supported (rejected with a message): user-defined point types,
-// red-black maps, kernel reductions, grids captured from globals.
+//   DeclareGrid<r>D(Name, struct T) -> the declaration plus T's layout
+//                                      description and SoA plane helpers;
+//                                      PSGridGet(g, ..).m[i], PSGridEmit(g, v)
+//                                      and PSGridEmitUtype(g.m[i], v) address
+//                                      one plane per scalar of T
+//                                      (reference cuda_runtime_builder.cc:319-1102)
+//   PSStencilMapRedBlack/Red/Black   -> the map run over the points of one or
+//                                      both colours (x+y+z+c even)
+//
+// Not supported (rejected with a message): kernel reductions, grids
+// captured from globals, nested structs inside point types.
 //
 // Usage: physisc-hip [--config FILE] [-o OUT] [-I DIR]... [-D NAME[=V]]... INPUT.c
 // Config keys (key = value lines): TRACE_KERNEL, KERNEL_ERROR_CHECK (bools),
@@ -187,21 +196,153 @@ std::string Text(const std::vector<Tok> &t, size_t b, size_t e) {
 // ------------------------------------------------------------ grid types --
 struct GridType {
   int rank = 0;
-  std::string elem;  // float double int long
+  std::string elem;   // float double int long, or the C type of a point struct
+  std::string utype;  // name of a user-defined point type ("" for primitives)
 };
 
+// A user-defined point type: DeclareGrid<r>D(Name, struct T) with T a struct
+// of scalars and arrays of scalars (reference docs/programming.md:118-137).
+struct UMember {
+  std::string name;
+  std::string ctype;      // scalar type text
+  std::vector<int> dims;  // array extents (empty for a scalar member)
+  int count = 1;          // product of dims
+};
+
+struct UType {
+  std::string name;   // Name of PSGrid<r>D<Name>
+  std::string ctype;  // "struct T" or a typedef name
+  int rank = 0;
+  std::vector<UMember> members;
+  const UMember *Find(const std::string &m) const {
+    for (const UMember &u : members)
+      if (u.name == m) return &u;
+    return nullptr;
+  }
+};
+
+// Grid type name (PSGrid3DPoint) -> point type, filled before any parsing
+// that needs grid types.
+std::map<std::string, UType> g_utypes;
+
 bool ParseGridType(const std::string &id, GridType *gt) {
-  // PSGrid<r>D<Float|Double|Int|Long>
-  if (id.size() < 10 || id.compare(0, 6, "PSGrid") != 0 || !isdigit((unsigned char)id[6]) || id[7] != 'D')
+  // PSGrid<r>D<Float|Double|Int|Long>, or a declared PSGrid<r>D<Name>
+  if (id.size() < 9 || id.compare(0, 6, "PSGrid") != 0 || !isdigit((unsigned char)id[6]) || id[7] != 'D')
     return false;
   std::string rest = id.substr(8);
   static const std::map<std::string, std::string> kElem = {
       {"Float", "float"}, {"Double", "double"}, {"Int", "int"}, {"Long", "long"}};
   auto it = kElem.find(rest);
-  if (it == kElem.end()) return false;
+  if (it == kElem.end()) {
+    auto u = g_utypes.find(id);
+    if (u == g_utypes.end()) return false;
+    gt->rank = u->second.rank;
+    gt->elem = u->second.ctype;
+    gt->utype = u->second.name;
+    return true;
+  }
   gt->rank = id[6] - '0';
   gt->elem = it->second;
+  gt->utype.clear();
   return gt->rank >= 1 && gt->rank <= 3;
+}
+
+// Index of the '{' that matches the '}' at `close`.
+size_t MatchOpen(const std::vector<Tok> &t, size_t close) {
+  int depth = 0;
+  for (size_t i = close + 1; i-- > 0;) {
+    if (t[i].kind != Tok::kPunct) continue;
+    if (t[i].s == "}") ++depth;
+    else if (t[i].s == "{" && --depth == 0) return i;
+  }
+  Die("unbalanced '}'");
+}
+
+// Members of the struct body between braces lb and rb: scalar members and
+// arrays of scalars with constant extents (macros are already expanded).
+std::vector<UMember> ParseStructBody(const std::vector<Tok> &t, size_t lb, size_t rb, const std::string &what) {
+  static const std::set<std::string> kTypeWords = {"float", "double", "int",      "long",  "short",
+                                                   "char",  "unsigned", "signed", "const"};
+  std::vector<UMember> out;
+  size_t i = lb + 1;
+  while (i < rb) {
+    size_t semi = i;
+    while (semi < rb && t[semi].s != ";") {
+      if (t[semi].s == "{") Die(what + ": nested aggregates are not supported as point members");
+      ++semi;
+    }
+    if (semi == rb) Die(what + ": expected ';' after a member");
+    std::string ctype;
+    size_t q = i;
+    while (q < semi && t[q].kind == Tok::kIdent && kTypeWords.count(t[q].s)) {
+      if (t[q].s != "const") ctype += (ctype.empty() ? "" : " ") + t[q].s;
+      ++q;
+    }
+    if (ctype.empty()) Die(what + ": point members must have scalar C types");
+    for (auto &d : SplitArgs(t, q, semi)) {
+      size_t a = d.first;
+      if (a >= d.second || t[a].kind != Tok::kIdent) Die(what + ": unsupported member declarator");
+      UMember m;
+      m.name = t[a].s;
+      m.ctype = ctype;
+      size_t b = a + 1;
+      while (b < d.second && t[b].s == "[") {
+        size_t c = MatchClose(t, b);
+        char *end = nullptr;
+        long v = c == b + 2 && t[b + 1].kind == Tok::kNum ? strtol(t[b + 1].s.c_str(), &end, 0) : 0;
+        if (v <= 0 || !end || *end != '\0') Die(what + ": array extents of members must be integer constants");
+        m.dims.push_back((int)v);
+        m.count *= (int)v;
+        b = c + 1;
+      }
+      if (b != d.second) Die(what + ": unsupported member declarator");
+      out.push_back(m);
+    }
+    i = semi + 1;
+  }
+  if (out.empty()) Die(what + ": point type without members");
+  return out;
+}
+
+// Finds every DeclareGrid<r>D(Name, type) and the definition of its struct.
+void CollectUtypes(const std::vector<Tok> &t) {
+  for (size_t i = 0; i + 1 < t.size(); ++i) {
+    const std::string &id = t[i].s;
+    if (t[i].kind != Tok::kIdent || t[i + 1].s != "(" || id.size() != 13 || id.compare(0, 11, "DeclareGrid") != 0 ||
+        id[12] != 'D' || id[11] < '1' || id[11] > '3')
+      continue;
+    size_t close = MatchClose(t, i + 1);
+    auto args = SplitArgs(t, i + 2, close);
+    if (args.size() != 2 || args[0].second != args[0].first + 1) Die(id + ": expected (Name, type)");
+    UType u;
+    u.name = t[args[0].first].s;
+    u.rank = id[11] - '0';
+    u.ctype = Text(t, args[1].first, args[1].second);
+    static const std::set<std::string> kPrimitive = {"float", "double", "int", "long"};
+    if (kPrimitive.count(u.ctype)) continue;  // the built-in grid types
+    size_t lb = 0, rb = 0;
+    const size_t tb = args[1].first, te = args[1].second;
+    if (te - tb == 2 && t[tb].s == "struct") {
+      const std::string &tag = t[tb + 1].s;
+      for (size_t j = 0; j + 2 < i; ++j)
+        if (t[j].s == "struct" && t[j + 1].s == tag && t[j + 2].s == "{") {
+          lb = j + 2;
+          rb = MatchClose(t, lb);
+          break;
+        }
+    } else if (te - tb == 1) {
+      const std::string &name = t[tb].s;  // typedef struct [tag] { ... } name;
+      for (size_t j = 1; j + 1 < i; ++j)
+        if (t[j].s == name && t[j - 1].s == "}" && t[j + 1].s == ";") {
+          rb = j - 1;
+          lb = MatchOpen(t, rb);
+          break;
+        }
+    }
+    if (!rb) Die(id + "(" + u.name + ", " + u.ctype + "): the point struct must be defined before it");
+    u.members = ParseStructBody(t, lb, rb, "point type " + u.ctype);
+    g_utypes["PSGrid" + std::string(1, id[11]) + "D" + u.name] = u;
+  }
 }
 
 bool ParseGridNew(const std::string &id, GridType *gt) {
@@ -211,6 +352,72 @@ bool ParseGridNew(const std::string &id, GridType *gt) {
 
 std::string StorageOf(const GridType &g) {
   return g.elem == "float" ? "SF" : "psk::Native<" + g.elem + ">";
+}
+
+// Linear index of a member element: ((s0) * d1 + (s1)) * d2 + (s2) ...
+std::string MemberIndex(const UMember &m, const std::vector<std::string> &subs) {
+  std::string lin = "(" + subs[0] + ")";
+  for (size_t d = 1; d < subs.size(); ++d)
+    lin = "(" + lin + " * " + std::to_string(m.dims[d]) + " + (" + subs[d] + "))";
+  return lin;
+}
+
+// Support code for one point type, emitted right after its DeclareGrid:
+// the layout description, the devNew/devFree/devCopyin/devCopyout functions
+// __PSGridNew & co. take, the kernel-side reference (one pointer per
+// member's first plane) and whole-point load/store helpers.
+std::string EmitUtypeSupport(const UType &u) {
+  std::ostringstream o;
+  const std::string &N = u.name, &T = u.ctype;
+  const size_t nm = u.members.size();
+  o << "\n// Point type " << T << " (grid type PSGrid" << u.rank << "D" << N << "): " << nm
+    << " members, one SoA plane per scalar.\n";
+  o << "static const int __PSUOff_" << N << "[] = {";
+  for (size_t m = 0; m < nm; ++m) o << (m ? ", " : "") << "(int)offsetof(" << T << ", " << u.members[m].name << ")";
+  o << "};\nstatic const int __PSUSize_" << N << "[] = {";
+  for (size_t m = 0; m < nm; ++m) o << (m ? ", " : "") << "(int)sizeof(" << u.members[m].ctype << ")";
+  o << "};\nstatic const int __PSUCount_" << N << "[] = {";
+  for (size_t m = 0; m < nm; ++m) o << (m ? ", " : "") << u.members[m].count;
+  o << "};\n";
+  o << "static const PSHipUtype __PSUtype_" << N << " = {(int)sizeof(" << T << "), " << nm << ", __PSUOff_" << N
+    << ", __PSUSize_" << N << ", __PSUCount_" << N << "};\n";
+  o << "static void *__PSGridDevNew_" << N << "(int nd, PSVectorInt dim) { return __PSHipUtypeDevNew(&__PSUtype_"
+    << N << ", nd, dim); }\n";
+  o << "static void __PSGridDevFree_" << N << "(void *d) { __PSHipUtypeDevFree(d); }\n";
+  o << "static void __PSGridDevCopyin_" << N << "(void *d, const void *src) { __PSHipUtypeCopyin(d, src); }\n";
+  o << "static void __PSGridDevCopyout_" << N << "(void *d, void *dst) { __PSHipUtypeCopyout(d, dst); }\n";
+  o << "static __PSGrid *__PSGridNewUtype_" << N << "(int d0, int d1, int d2) {\n"
+    << "  PSVectorInt d = {d0, d1, d2};\n  (void)__PSGridDevFree_" << N << ";\n  (void)__PSGridDevCopyin_" << N
+    << ";\n  (void)__PSGridDevCopyout_" << N << ";\n"
+    << "  return __PSGridNew((int)sizeof(" << T << "), " << u.rank << ", d, __PSGridDevNew_" << N << ");\n}\n";
+  o << "struct __PSURef_" << N << " {\n  int dim[3];\n  int64_t ps;  // elements between planes\n";
+  for (const UMember &m : u.members) o << "  " << m.ctype << " *" << m.name << ";\n";
+  o << "};\n";
+  o << "static inline struct __PSURef_" << N << " __PSUMakeRef_" << N << "(const __PSGrid *g) {\n"
+    << "  struct __PSURef_" << N << " r;\n  const __PSGridUtype_dev *d = (const __PSGridUtype_dev *)g->dev;\n"
+    << "  for (int i = 0; i < 3; ++i) r.dim[i] = d->dim[i];\n  r.ps = d->plane_stride;\n";
+  for (size_t m = 0; m < nm; ++m)
+    o << "  r." << u.members[m].name << " = (" << u.members[m].ctype << " *)__PSHipUtypeMember(g, " << m << ");\n";
+  o << "  return r;\n}\n";
+  o << "__device__ __forceinline__ " << T << " __PSUGet_" << N << "(const struct __PSURef_" << N
+    << " &g, int64_t off) {\n  " << T << " v;\n";
+  for (const UMember &m : u.members) {
+    if (m.dims.empty()) o << "  v." << m.name << " = g." << m.name << "[off];\n";
+    else
+      o << "  for (int c = 0; c < " << m.count << "; ++c) ((" << m.ctype << " *)v." << m.name << ")[c] = g."
+        << m.name << "[c * g.ps + off];\n";
+  }
+  o << "  return v;\n}\n";
+  o << "__device__ __forceinline__ void __PSUPut_" << N << "(const struct __PSURef_" << N << " &g, int64_t off, const "
+    << T << " &v) {\n";
+  for (const UMember &m : u.members) {
+    if (m.dims.empty()) o << "  g." << m.name << "[off] = v." << m.name << ";\n";
+    else
+      o << "  for (int c = 0; c < " << m.count << "; ++c) g." << m.name << "[c * g.ps + off] = ((const " << m.ctype
+        << " *)v." << m.name << ")[c];\n";
+  }
+  o << "}\n";
+  return o.str();
 }
 
 // ------------------------------------------------------------- functions --
@@ -487,7 +694,15 @@ void Translator::AnalyzeKernel(Kernel &k) {
       if (!k.gtype.count(g)) Die("kernel " + f.name + ": PSGridEmit on unknown grid '" + g + "'");
       k.emitted.insert(g);
       ++k.emit_count;
-    } else if (name == "PSGridEmitUtype" || name == "PSGridSet") {
+    } else if (name == "PSGridEmitUtype") {
+      // PSGridEmitUtype(g.member[i].., v): one member of a point-type grid.
+      std::string g = t_[i + 2].s;
+      auto it = k.gtype.find(g);
+      if (it == k.gtype.end() || it->second.utype.empty())
+        Die("kernel " + f.name + ": PSGridEmitUtype on '" + g + "', which is not a grid of a point type");
+      k.emitted.insert(g);
+      ++k.emit_count;
+    } else if (name == "PSGridSet") {
       Die("kernel " + f.name + ": " + name + " is not supported by this target");
     }
   }
@@ -497,7 +712,7 @@ void Translator::AnalyzeKernel(Kernel &k) {
   for (const Get &g : k.gets) {
     if (!g.relative) continue;
     const GridType &gt = k.gtype[g.grid];
-    if (gt.rank != k.rank || k.emitted.count(g.grid)) continue;
+    if (gt.rank != k.rank || k.emitted.count(g.grid) || !gt.utype.empty()) continue;
     if (g.off[0] || g.off[1] || g.off[2]) count[g.grid]++;
   }
   int best = 0;
@@ -574,8 +789,48 @@ std::string Translator::RewriteBody(const Kernel &k, size_t b, size_t e, bool co
     if (!out.empty() && IsIdentChar(out.back()) && !s.empty() && IsIdentChar(s[0])) out += ' ';
     out += s;
   };
+  // Offset of the map's current point in grid g.
+  auto center = [&](const std::string &g, const GridType &gt) {
+    std::string at = "psk::Off" + std::to_string(gt.rank) + "(" + g;
+    for (int d = 0; d < gt.rank; ++d) at += ", " + k.idx[std::min(d, k.rank - 1)];
+    return at + ")";
+  };
+  // Member access `.name[s0][s1]..` of a point-type grid at tokens [q, e):
+  // returns the element's address expression and advances q past it.
+  auto member = [&](const std::string &g, const UType &u, size_t &q, const std::string &off) {
+    if (q + 1 >= e || t_[q].s != "." || t_[q + 1].kind != Tok::kIdent)
+      Die("kernel " + k.f->name + ": expected a member of point type " + u.ctype + " after '" + g + "'");
+    const UMember *m = u.Find(t_[q + 1].s);
+    if (!m) Die("kernel " + k.f->name + ": " + u.ctype + " has no member '" + t_[q + 1].s + "'");
+    q += 2;
+    std::vector<std::string> subs;
+    while (q < e && t_[q].s == "[") {
+      size_t c = MatchClose(t_, q);
+      subs.push_back(RewriteBody(k, q + 1, c, false));
+      q = c + 1;
+    }
+    if (subs.size() != m->dims.size())
+      Die("kernel " + k.f->name + ": member '" + m->name + "' must be indexed down to a scalar");
+    std::string idx = subs.empty() ? off : "(int64_t)" + MemberIndex(*m, subs) + " * " + g + ".ps + " + off;
+    return std::make_pair(m, g + "." + m->name + "[" + idx + "]");
+  };
   for (size_t i = b; i < e; ++i) {
     const Tok &tk = t_[i];
+    if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" && tk.s == "PSGridEmitUtype") {
+      size_t close = MatchClose(t_, i + 1);
+      auto args = SplitArgs(t_, i + 2, close);
+      if (args.size() != 2) Die("kernel " + k.f->name + ": PSGridEmitUtype takes a member and a value");
+      std::string g = t_[args[0].first].s;
+      const GridType &gt = k.gtype.at(g);
+      const UType &u = g_utypes.at("PSGrid" + std::to_string(gt.rank) + "D" + gt.utype);
+      size_t q = args[0].first + 1;
+      auto mem = member(g, u, q, center(g, gt));
+      if (q != args[0].second) Die("kernel " + k.f->name + ": unexpected tokens in the PSGridEmitUtype target");
+      add("(" + mem.second + " = (" + mem.first->ctype + ")(" +
+          RewriteBody(k, args[1].first, args[1].second, false) + "))");
+      i = close;
+      continue;
+    }
     if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" &&
         (tk.s == "PSGridGet" || tk.s == "PSGridGetPeriodic" || tk.s == "PSGridEmit" || tk.s == "PSGridDim")) {
       size_t close = MatchClose(t_, i + 1);
@@ -585,12 +840,31 @@ std::string Translator::RewriteBody(const Kernel &k, size_t b, size_t e, bool co
       std::vector<std::string> a;
       for (size_t q = 1; q < args.size(); ++q) a.push_back(RewriteBody(k, args[q].first, args[q].second, false));
       std::ostringstream r;
+      if (!gt.utype.empty() && tk.s != "PSGridDim") {
+        const UType &u = g_utypes.at("PSGrid" + std::to_string(gt.rank) + "D" + gt.utype);
+        if (tk.s == "PSGridEmit") {
+          r << "__PSUPut_" << u.name << "(" << g << ", " << center(g, gt) << ", " << a[0] << ")";
+        } else {
+          std::string off = "psk::Off" + std::string(tk.s == "PSGridGetPeriodic" ? "P" : "") +
+                            std::to_string(gt.rank) + "(" + g;
+          for (auto &s : a) off += ", " + s;
+          off += ")";
+          size_t q = close + 1;
+          if (q < e && t_[q].s == ".") {
+            r << "(" << member(g, u, q, off).second << ")";
+            close = q - 1;
+          } else {
+            r << "__PSUGet_" << u.name << "(" << g << ", " << off << ")";
+          }
+        }
+        add(r.str());
+        i = close;
+        continue;
+      }
       if (tk.s == "PSGridDim") {
         r << "(" << g << ".dim[" << a[0] << "])";
       } else if (tk.s == "PSGridEmit") {
-        std::string at = "psk::Off" + std::to_string(gt.rank) + "(" + g;
-        for (int d = 0; d < gt.rank; ++d) at += ", " + k.idx[std::min(d, k.rank - 1)];
-        at += ")";
+        const std::string at = center(g, gt);
         if (compute_form) r << "return (" << gt.elem << ")(" << a[0] << ")";
         else r << g << ".store(" << at << ", (" << gt.elem << ")(" << a[0] << "))";
       } else {
@@ -645,7 +919,8 @@ std::string Translator::EmitPlan(const Kernel &k) {
   o << "  static constexpr bool kReduce = false;\n";
   o << "  struct Args {\n";
   for (const Param &p : k.all) {
-    if (p.is_grid) o << "    psk::GridRef<" << StorageOf(p.gt) << "> " << p.name << ";\n";
+    if (p.is_grid && !p.gt.utype.empty()) o << "    struct __PSURef_" << p.gt.utype << " " << p.name << ";\n";
+    else if (p.is_grid) o << "    psk::GridRef<" << StorageOf(p.gt) << "> " << p.name << ";\n";
     else o << "    " << p.type << " " << p.name << ";\n";
   }
   o << "  };\n";
@@ -716,7 +991,9 @@ std::string Translator::EmitMapAndLaunch(const Kernel &k) {
   o << "template <class SF>\nstatic typename " << P << "<SF>::Args __PSArgs_" << f.name << "(const struct " << S
     << " *s) {\n  typename " << P << "<SF>::Args a;\n";
   for (const Param &p : k.all) {
-    if (p.is_grid) o << "  a." << p.name << " = psk::MakeRef<" << (p.gt.elem == "float" ? "SF" : StorageOf(p.gt))
+    if (p.is_grid && !p.gt.utype.empty())
+      o << "  a." << p.name << " = __PSUMakeRef_" << p.gt.utype << "(s->" << p.name << ");\n";
+    else if (p.is_grid) o << "  a." << p.name << " = psk::MakeRef<" << (p.gt.elem == "float" ? "SF" : StorageOf(p.gt))
                      << ">(s->" << p.name << ");\n";
     else o << "  a." << p.name << " = s->" << p.name << ";\n";
   }
@@ -856,6 +1133,30 @@ std::string Translator::RewriteHost(size_t b, size_t e) {
       continue;
     }
     GridType gt;
+    if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" && tk.s.size() == 13 &&
+        tk.s.compare(0, 11, "DeclareGrid") == 0) {
+      // The declaration stays (physis_hip.h turns it into a typedef); a
+      // point type's support code follows it.
+      size_t close = MatchClose(t_, i + 1);
+      size_t end = close + 1 < e && t_[close + 1].s == ";" ? close + 1 : close;
+      for (size_t q = i; q <= end; ++q) add(t_[q].s);
+      out += "\n";
+      auto u = g_utypes.find("PSGrid" + std::string(1, tk.s[11]) + "D" + t_[i + 2].s);
+      if (u != g_utypes.end()) out += EmitUtypeSupport(u->second);
+      i = end;
+      continue;
+    }
+    if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" && ParseGridNew(tk.s, &gt) && !gt.utype.empty()) {
+      size_t close = MatchClose(t_, i + 1);
+      auto args = SplitArgs(t_, i + 2, close);
+      if ((int)args.size() != gt.rank) Die(tk.s + ": expected " + std::to_string(gt.rank) + " extents");
+      std::string r = "__PSGridNewUtype_" + gt.utype + "(";
+      for (int d = 0; d < 3; ++d)
+        r += (d ? ", " : "") + (d < gt.rank ? "(" + RewriteHost(args[d].first, args[d].second) + ")" : std::string("1"));
+      add(r + ")");
+      i = close;
+      continue;
+    }
     if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" && ParseGridNew(tk.s, &gt)) {
       size_t close = MatchClose(t_, i + 1);
       auto args = SplitArgs(t_, i + 2, close);
@@ -935,6 +1236,7 @@ std::string Translator::RewriteHost(size_t b, size_t e) {
 }
 
 std::string Translator::Run(const std::vector<std::string> &includes) {
+  CollectUtypes(t_);
   funcs_ = FindFunctions(t_);
   CollectGridVars();
   std::map<std::string, const Func *> byname;

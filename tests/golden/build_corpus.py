@@ -5,8 +5,8 @@ translated with physisc-hip and compiled with hipcc into tests/_corpus_bin/
 (ignored by git; the executables travel with the tree to the GPU machine,
 where tests/test_gpu_corpus.py runs them and compares their stdout with the
 SHA-256 of the reference's own manual.ref outputs in manual_ref.json).
-Programs this target does not support (user-defined point types, the
-multi-file module test) are listed in SKIPPED with the reason.
+Programs this target does not support (the multi-file module test) are
+listed in SKIPPED with the reason.
 """
 import json
 import os
@@ -19,7 +19,6 @@ CASES = "/root/reference/tests/system_tests/test_cases"
 OUT = os.path.join(ROOT, "tests", "_corpus_bin")
 
 SKIPPED = {
-    "user-defined": "user-defined point types (PSGrid3DPoint...) are not lowered by physisc-hip",
     "module": "multi-file module test (no main in one unit)",
 }
 

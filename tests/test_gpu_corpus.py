@@ -5,7 +5,8 @@ tests/golden/build_corpus.py into tests/_corpus_bin/.  As in the reference's
 run_system_tests.sh (run_system_tests.sh.cmake:808-856), a program passes if
 its stdout equals the output of the reference's hand-written ref oracle
 (compared through the SHA-256 in tests/golden/manual_ref.json); the
-self-checking programs (test_01, 02, 03, 08-14, multi-kernels) must exit 0.
+self-checking programs (test_01, 02, 03, 08-14, multi-kernels and the
+user-defined point type checks) must exit 0.
 """
 import hashlib
 import json
@@ -26,12 +27,16 @@ COMPARED = ["test_7-pt", "test_7-pt-periodic", "test_7-pt-neumann-cond", "test_7
             "test_27-pt", "test_27-pt-periodic", "test_3-pt-periodic", "test_3-pt-1d", "test_9-pt-2d",
             "test_asymmetric", "test_asymmetric-periodic", "test_stencil-hole", "test_mixed-dim",
             "test_mixed-dim2", "test_mixed-dim3", "test_redblack", "test_redblack-periodic",
-            "test_redblack-separated"]
+            "test_redblack-separated", "test_user-defined-type1", "test_user-defined-type3",
+            "test_user-defined-type5", "test_user-defined-type-multi-dim-member",
+            "test_user-defined-type-multi-members"]
 # Programs without a manual.ref of their own, compared with the oracle of an
 # equivalent program: MapRed followed by MapBlack is one MapRedBlack.
 GOLDEN_OF = {"test_redblack-separated": "test_redblack"}
 SELF_CHECKING = ["test_01", "test_02", "test_03", "test_08", "test_09", "test_10", "test_11", "test_12",
-                 "test_13", "test_14", "test_multi-kernels"]
+                 "test_13", "test_14", "test_multi-kernels", "test_user-defined-type2", "test_user-defined-type4",
+                 "test_user-defined-type-copyin-copyout", "test_user-defined-type-kernel-copy",
+                 "test_user-defined-type-transpose"]
 
 
 def exe(name):

@@ -117,3 +117,69 @@ def test_invalid_grid_shapes(rt):
         rt.Grid((4, 4, 4), dtype=np.float64, storage=rt.STORAGE_BF16)
     with pytest.raises(rt.PhysisError):
         rt.Grid((2048, 2048, 1024))  # 2^32 elements: beyond 32-bit PSIndex
+
+
+class _Utype(ctypes.Structure):
+    _fields_ = [("elm_size", ctypes.c_int), ("num_members", ctypes.c_int),
+                ("offset", ctypes.POINTER(ctypes.c_int)), ("scalar_size", ctypes.POINTER(ctypes.c_int)),
+                ("count", ctypes.POINTER(ctypes.c_int))]
+
+
+def test_user_type_grid_roundtrip_with_padding(rt):
+    """A point struct {double a; float b[3]; char c;} (24 bytes, 3 padding
+    bytes) through the devNewFunc form of __PSGridNew and the plain
+    PSGridCopyin/Copyout/PSGridFree, and through the __PSHipUtype* calls a
+    generated devCopyin/devCopyout/devFree make.  Members come back exactly;
+    padding comes back as zeros.  Chunked staging is exercised by a grid
+    larger than one 32 MiB chunk."""
+    L = rt.lib()
+    dt = np.dtype({"names": ["a", "b", "c"], "formats": ["<f8", ("<f4", 3), "i1"], "offsets": [0, 8, 20],
+                   "itemsize": 24})
+    off = (ctypes.c_int * 3)(0, 8, 20)
+    size = (ctypes.c_int * 3)(8, 4, 1)
+    count = (ctypes.c_int * 3)(1, 3, 1)
+    desc = _Utype(24, 3, off, size, count)
+    L.__PSHipUtypeDevNew.restype = ctypes.c_void_p
+    L.__PSHipUtypeDevNew.argtypes = [ctypes.POINTER(_Utype), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    for name in ("__PSHipUtypeCopyin", "__PSHipUtypeCopyout"):
+        getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.__PSHipUtypeDevFree.argtypes = [ctypes.c_void_p]
+    new_func_t = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int))
+
+    @new_func_t
+    def dev_new(nd, dim):
+        return L.__PSHipUtypeDevNew(ctypes.byref(desc), nd, dim)
+
+    L.__PSGridNew.restype = ctypes.c_void_p
+    L.__PSGridNew.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), new_func_t]
+    for name in ("PSGridCopyin", "PSGridCopyout"):
+        getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.PSGridFree.argtypes = [ctypes.c_void_p]
+    rng = np.random.default_rng(5)
+    for shape in [(7, 5, 3), (160, 128, 80)]:  # the second is 39 MiB of structs
+        n = int(np.prod(shape))
+        src = np.zeros(n, dtype=dt)
+        src["a"] = rng.random(n)
+        src["b"] = rng.random((n, 3)).astype(np.float32)
+        src["c"] = rng.integers(-128, 127, n, dtype=np.int8)
+        raw = src.view(np.uint8).reshape(n, 24)
+        raw[:, 21:] = 0xAB  # padding on the host side is not carried
+        dims = (ctypes.c_int * 3)(*shape)
+        print("user-type grid", shape, flush=True)
+        g = L.__PSGridNew(24, 3, dims, dev_new)
+        assert g
+        out = np.zeros(n, dtype=dt)
+        L.PSGridCopyout(g, out.ctypes.data)  # zero filled at creation
+        assert not out.view(np.uint8).any()
+        L.PSGridCopyin(g, src.ctypes.data)
+        L.PSGridCopyout(g, out.ctypes.data)
+        assert np.array_equal(out["a"], src["a"]) and np.array_equal(out["b"], src["b"])
+        assert np.array_equal(out["c"], src["c"])
+        assert not out.view(np.uint8).reshape(n, 24)[:, 21:].any()
+        L.PSGridFree(g)
+        dev = L.__PSHipUtypeDevNew(ctypes.byref(desc), 3, dims)
+        L.__PSHipUtypeCopyin(dev, src.ctypes.data)
+        out2 = np.zeros(n, dtype=dt)
+        L.__PSHipUtypeCopyout(dev, out2.ctypes.data)
+        assert np.array_equal(out2.view(np.uint8), out.view(np.uint8))
+        L.__PSHipUtypeDevFree(dev)
