@@ -37,20 +37,31 @@ def translate(src, out_hip, config=None, defines=()):
     return out_hip
 
 
-def compile_hip(hip, exe):
+def compile_hip(hip, exe, extra_c=()):
+    """hipcc the generated file and link it with libphysis_rt_hip; extra_c
+    are plain C files of the program (a module's host side), compiled with
+    the system C compiler and linked in, as the reference's test driver does
+    for its module tests (run_system_tests.sh.cmake:605-631)."""
     out_dir = os.path.dirname(os.path.abspath(exe))
     rel = os.path.relpath(LIB_DIR, out_dir)
     rpath = "$ORIGIN/" + rel if os.path.abspath(out_dir).startswith(HERE) else LIB_DIR
-    cmd = [HIPCC] + HIP_FLAGS + ["-I", INCLUDE, "-x", "hip", hip, "-L", LIB_DIR, "-lphysis_rt_hip",
-                                 "-Wl,-rpath," + rpath, "-o", exe]
+    objs = []
+    for c in extra_c:
+        obj = exe + "." + os.path.splitext(os.path.basename(c))[0] + ".o"
+        subprocess.run(["gcc", "-O2", "-c", c, "-o", obj], check=True)
+        objs.append(obj)
+    cmd = [HIPCC] + HIP_FLAGS + ["-I", INCLUDE, "-x", "hip", hip, "-x", "none"] + objs + \
+          ["-L", LIB_DIR, "-lphysis_rt_hip", "-Wl,-rpath," + rpath, "-o", exe]
     subprocess.run(cmd, check=True)
+    for obj in objs:
+        os.remove(obj)
     return exe
 
 
-def build(src, exe, config=None, defines=(), keep_hip=None):
-    hip = keep_hip or (os.path.splitext(exe)[0] + ".hip")
+def build(src, exe, config=None, defines=(), keep_hip=None, extra_c=()):
+    hip = keep_hip or exe + ".hip"
     translate(src, hip, config, defines)
-    return compile_hip(hip, exe)
+    return compile_hip(hip, exe, extra_c)
 
 
 def main(argv=None):
@@ -60,12 +71,14 @@ def main(argv=None):
     ap.add_argument("--config")
     ap.add_argument("-D", dest="defines", action="append", default=[])
     ap.add_argument("--emit-only", action="store_true")
+    ap.add_argument("--with-c", dest="extra_c", action="append", default=[],
+                    help="plain C source of the program to compile and link in")
     args = ap.parse_args(argv)
     stem = os.path.splitext(os.path.basename(args.source))[0]
     if args.emit_only:
         translate(args.source, args.output or stem + ".hip", args.config, args.defines)
         return 0
-    build(args.source, args.output or stem, args.config, args.defines)
+    build(args.source, args.output or stem, args.config, args.defines, extra_c=args.extra_c)
     return 0
 
 

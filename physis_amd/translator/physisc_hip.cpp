@@ -444,6 +444,14 @@ std::vector<Func> FindFunctions(const std::vector<Tok> &t) {
       stmt_begin = i + 1;
       continue;
     }
+    if (k.kind == Tok::kPunct && k.s == "{" && i >= 2 && t[i - 1].kind == Tok::kStr && t[i - 2].s == "extern") {
+      stmt_begin = i + 1;  // extern "C" { ... }: its definitions are top level
+      continue;
+    }
+    if (k.kind == Tok::kPunct && k.s == "}") {  // end of an extern "C" block
+      stmt_begin = i + 1;
+      continue;
+    }
     if (k.kind == Tok::kPunct && k.s == "{") {
       // A top-level brace: function body if preceded by ") " (with optional
       // attributes), else a struct/union/enum/initializer body.
@@ -1339,7 +1347,9 @@ std::string Preprocess(const std::string &path, const std::vector<std::string> &
     std::ofstream f(in_path);
     f << stripped;
   }
-  std::string cmd = "gcc -E -P -nostdinc -DPHYSIS_USER -I" + tmp;
+  // The generated file is compiled as HIP C++, so the program's own
+  // `#ifdef __cplusplus` branches (extern "C" blocks of a module) are taken.
+  std::string cmd = "gcc -E -P -nostdinc -DPHYSIS_USER -D__cplusplus=201703L -I" + tmp;
   for (const std::string &d : defs) cmd += " '-D" + d + "'";
   cmd += " " + in_path + " 2>&1";
   FILE *p = popen(cmd.c_str(), "r");

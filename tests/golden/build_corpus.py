@@ -5,8 +5,9 @@ translated with physisc-hip and compiled with hipcc into tests/_corpus_bin/
 (ignored by git; the executables travel with the tree to the GPU machine,
 where tests/test_gpu_corpus.py runs them and compares their stdout with the
 SHA-256 of the reference's own manual.ref outputs in manual_ref.json).
-Programs this target does not support (the multi-file module test) are
-listed in SKIPPED with the reason.
+Programs this target does not support would be listed in SKIPPED with the
+reason; there are none.  A module test (test_*.module.c, a DSL unit without
+main) is linked with its plain C driver test_*.module_base.c.
 """
 import json
 import os
@@ -18,9 +19,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 CASES = "/root/reference/tests/system_tests/test_cases"
 OUT = os.path.join(ROOT, "tests", "_corpus_bin")
 
-SKIPPED = {
-    "module": "multi-file module test (no main in one unit)",
-}
+SKIPPED = {}
 
 
 def main():
@@ -32,14 +31,17 @@ def main():
     os.makedirs(OUT, exist_ok=True)
     built, failed = [], []
     for name in sorted(os.listdir(CASES)):
-        if not name.startswith("test_") or not name.endswith(".c") or ".manual." in name:
+        if not name.startswith("test_") or not name.endswith(".c") or ".manual." in name or \
+                name.endswith(".module_base.c"):
             continue
         stem = name[:-2]
         if any(k in stem for k in SKIPPED):
             continue
         exe = os.path.join(OUT, stem)
+        # A module test is a DSL unit without main plus a plain C driver.
+        extra = [os.path.join(CASES, stem[:-len(".module")] + ".module_base.c")] if stem.endswith(".module") else []
         try:
-            phc.build(os.path.join(CASES, name), exe)
+            phc.build(os.path.join(CASES, name), exe, extra_c=extra)
             os.remove(exe + ".hip")
             built.append(stem)
         except subprocess.CalledProcessError:

@@ -29,10 +29,11 @@ COMPARED = ["test_7-pt", "test_7-pt-periodic", "test_7-pt-neumann-cond", "test_7
             "test_mixed-dim2", "test_mixed-dim3", "test_redblack", "test_redblack-periodic",
             "test_redblack-separated", "test_user-defined-type1", "test_user-defined-type3",
             "test_user-defined-type5", "test_user-defined-type-multi-dim-member",
-            "test_user-defined-type-multi-members"]
+            "test_user-defined-type-multi-members", "test_7-pt.module"]
 # Programs without a manual.ref of their own, compared with the oracle of an
 # equivalent program: MapRed followed by MapBlack is one MapRedBlack.
-GOLDEN_OF = {"test_redblack-separated": "test_redblack"}
+# The module test (a DSL unit driven from plain C) runs test_7-pt's program.
+GOLDEN_OF = {"test_redblack-separated": "test_redblack", "test_7-pt.module": "test_7-pt"}
 SELF_CHECKING = ["test_01", "test_02", "test_03", "test_08", "test_09", "test_10", "test_11", "test_12",
                  "test_13", "test_14", "test_multi-kernels", "test_user-defined-type2", "test_user-defined-type4",
                  "test_user-defined-type-copyin-copyout", "test_user-defined-type-kernel-copy",
