@@ -13,6 +13,7 @@ import json
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
@@ -22,30 +23,37 @@ OUT = os.path.join(ROOT, "tests", "_corpus_bin")
 SKIPPED = {}
 
 
-def main():
+def main(jobs=8):
     if not os.path.isdir(CASES):
         print("reference corpus not present; nothing to build", file=sys.stderr)
         return 0
     sys.path.insert(0, ROOT)
     from physis_amd import phc
     os.makedirs(OUT, exist_ok=True)
-    built, failed = [], []
+    stems = []
     for name in sorted(os.listdir(CASES)):
         if not name.startswith("test_") or not name.endswith(".c") or ".manual." in name or \
                 name.endswith(".module_base.c"):
             continue
         stem = name[:-2]
-        if any(k in stem for k in SKIPPED):
-            continue
+        if not any(k in stem for k in SKIPPED):
+            stems.append(stem)
+
+    def one(stem):
         exe = os.path.join(OUT, stem)
         # A module test is a DSL unit without main plus a plain C driver.
         extra = [os.path.join(CASES, stem[:-len(".module")] + ".module_base.c")] if stem.endswith(".module") else []
         try:
-            phc.build(os.path.join(CASES, name), exe, extra_c=extra)
+            phc.build(os.path.join(CASES, stem + ".c"), exe, extra_c=extra)
             os.remove(exe + ".hip")
-            built.append(stem)
+            return stem, True
         except subprocess.CalledProcessError:
-            failed.append(stem)
+            return stem, False
+
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        results = list(pool.map(one, stems))
+    built = [s for s, ok in results if ok]
+    failed = [s for s, ok in results if not ok]
     with open(os.path.join(OUT, "manifest.json"), "w") as f:
         json.dump({"built": built, "failed": failed}, f, indent=1)
     print("built %d, failed %d %s" % (len(built), len(failed), failed), file=sys.stderr)
