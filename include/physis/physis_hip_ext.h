@@ -113,6 +113,19 @@ extern const char *PSHipDiffusion7VariantName(int variant);
  * the FP64 sum of ss^2 of sweep i when gosa is not NULL. */
 extern int PSHipHimenoRun(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids,
                           float omega, int sweeps, double *gosa, int variant);
+/* The same sweeps with `degree` (1..4) of them fused per HBM pass (temporal
+ * blocking; no reference counterpart): each coefficient element is loaded
+ * once per pass instead of once per sweep.  Tiles of 64 x by points with
+ * rows_per_wave tile rows per wave: (by, rows_per_wave) = (8, 1) or (8, 2)
+ * for degree <= 3, (16, 1) or (16, 2) for any degree; zchunk = output
+ * planes per workgroup (0: all); prefetch = march steps between a plane's
+ * load and its first use (0: the shape's default; 1..3 for some shapes,
+ * PS_HIP_ERR_UNSUPPORTED otherwise).  p0,
+ * p1 and gosa end exactly as after PSHipHimenoRun; degree > 1 uses a third
+ * runtime-owned buffer of the grid's size; grids must be below 4 GiB. */
+extern int PSHipHimenoRunTB(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids,
+                            float omega, int sweeps, double *gosa, int degree,
+                            int by, int rows_per_wave, int zchunk, int prefetch);
 
 /* ---- batched many-grid diffusion (no reference counterpart) -------------
  * Grid pairs (a[i], b[i]) of any shape (nx <= 512), FP32 or BF16 storage

@@ -103,14 +103,15 @@ def diffusion7(field, coef, periodic, steps, dom=None, dtype=np.float32, threads
     return g1, g2
 
 
-def himeno(p_init, coefs, omega, sweeps, dtype=np.float32):
+def himeno(p_init, coefs, omega, sweeps, dtype=np.float32, p1_init=None):
     """Runs Himeno sweeps; coefs = 12 arrays (a0,a1,a2,a3,b0,b1,b2,c0,c1,c2,bnd,wrk1).
+    p0 starts as p_init, p1 as p1_init (default: p_init too).
 
     Returns (p0, p1, gosa[sweeps])."""
     nz, ny, nx = p_init.shape
     dim = np.asarray([nx, ny, nz], np.int32)
     p0 = np.ascontiguousarray(p_init, dtype=dtype).copy()
-    p1 = p0.copy()
+    p1 = p0.copy() if p1_init is None else np.ascontiguousarray(p1_init, dtype=dtype).copy()
     cs = [np.ascontiguousarray(c, dtype=dtype) for c in coefs]
     ptrs = (ctypes.c_void_p * 12)(*[c.ctypes.data for c in cs])
     gosa = np.zeros(max(sweeps, 1), np.float64)

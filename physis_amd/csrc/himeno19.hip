@@ -11,6 +11,10 @@
 #include "physis/hip/skeleton.h"
 #include "physis/hip/storage.h"
 
+#include <algorithm>
+#include <type_traits>
+#include <utility>
+
 namespace physis_hip {
 namespace {
 
@@ -54,6 +58,233 @@ struct HimenoPlan {
   }
 };
 
+// ---------------------------------------------------------------------------
+// T sweeps per HBM pass (temporal blocking).
+//
+// 12 of the 14 grids a sweep streams are coefficients read at the point
+// itself, so fusing T sweeps into one z-march cuts HBM traffic by close to T:
+// each coefficient plane is loaded once into a per-thread register queue and
+// used by all T sweeps as the march passes it.  One workgroup is 64 lanes (x)
+// by BY waves (y), one point per thread.  Level 0 is the input field u_k;
+// level t = 1..T is u_{k+t}.  Level t of plane z is computed at march step
+// s = z + t - 1 from the level t-1 planes z-1, z, z+1 (held in an LDS ring of
+// four planes per level, one barrier per level and step), so every level
+// trails the one below it by one plane.  The xy region a level is needed on
+// shrinks by one point per side and level: a tile writes the inner
+// (64 - 2(T-1)) x (BY - 2(T-1)) points of level T, and neighbouring tiles
+// overlap (their shared coefficient lines meet in L2).
+//
+// Only interior points [1, n-1) are relaxed (the domain of the reference
+// program, himenobmtxpa_physis.c:372-374).  Sweeps never write the boundary
+// shell, so the shell of u_j is the shell of the buffer the j-th sweep writes
+// into: buffer A for even j, B for odd j (p0 and p1 of the program).  Every
+// level takes its shell values from there, which lets the pass read from and
+// write into a third buffer.  Each value is computed by the same FP32
+// expression as the one-sweep kernel, so results are bitwise equal to T
+// separate sweeps; the FP64 sum of ss^2 of every level is accumulated over
+// the tile's own points only (each interior point counted once per sweep).
+struct TBArgs {
+  const void *in;        // u_k (interior read; shell taken from shell[k0 & 1])
+  void *out;             // u_{k+T}, interior points only
+  const void *shell[2];  // buffers holding the shell of even / odd iterates
+  const void *c[NCOEF];
+  int dim[3];
+  int k0;
+  int zchunk;
+  int tiles_x;
+  float omega;
+  double *partials;  // [T][blocks]
+};
+
+// Calls f(std::integral_constant<int, J>{}) for J = 0 .. N-1 in order.
+template <class F, int... J>
+__device__ __forceinline__ void StaticFor(F &&f, std::integer_sequence<int, J...>) {
+  (f(std::integral_constant<int, J>{}), ...);
+}
+
+// A workgroup is NW waves of 64 lanes; wave w holds the R rows w, w + NW,
+// ... of a BY = NW * R row tile (R independent points per thread keep more
+// loads and LDS reads in flight between barriers).  Prefetch distance D:
+// coefficient and level-0 planes are loaded D march steps before their
+// first use.  The register queues hold Q = T + D planes and the march is
+// unrolled Q times, so a plane's registers never move (a register copy of a
+// load in flight would wait for it).
+template <class S, int T, int NW, int R, int D>
+__global__ void __launch_bounds__(64 * NW)
+HimenoTB(TBArgs a) {
+  using st = typename S::stored_t;
+  constexpr int BY = NW * R;
+  constexpr int TW = 66, TH = BY + 2, PLANE = TW * TH, NT = 64 * NW;
+  constexpr int PER = (PLANE + NT - 1) / NT;
+  constexpr int OX = 64 - 2 * (T - 1), OY = BY - 2 * (T - 1);
+  constexpr int Q = T + D;
+  static_assert(OY >= 1, "tile too short for this temporal degree");
+  __shared__ float ring[T][4][PLANE];
+
+  const int nx = a.dim[0], ny = a.dim[1], nz = a.dim[2];
+  const int b = psk::XcdTile(blockIdx.x, gridDim.x);
+  const int tile_x = b % a.tiles_x, tile_y = b / a.tiles_x;
+  const int ox = 1 + tile_x * OX, oy = 1 + tile_y * OY;  // first owned point
+  const int lane = threadIdx.x, w = threadIdx.y, tid = w * 64 + lane;
+  const int x = ox - (T - 1) + lane;
+  const int zb = 1 + (int)blockIdx.y * a.zchunk;
+  const int ze = min(zb + a.zchunk, nz - 1);
+  // Byte offsets are 32-bit (the host keeps grids below 4 GiB on this
+  // path), so every load is a uniform base address plus one VGPR offset.
+  const uint32_t plane = (uint32_t)(nx * ny) * (uint32_t)sizeof(st);
+  const char *in = static_cast<const char *>(a.in);
+  char *out = static_cast<char *>(a.out);
+  const char *shell0 = static_cast<const char *>(a.shell[a.k0 & 1]);
+  auto ld = [](const char *base, uint32_t off) { return S::load(reinterpret_cast<const st *>(base + off)); };
+
+  // Per row r of this thread: tile row ty = w + r*NW, grid row y.
+  bool col_in[R], col_int[R], owned[R];
+  uint32_t col[R];
+  int center[R], ty[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    ty[r] = w + r * NW;
+    const int y = oy - (T - 1) + ty[r];
+    col_in[r] = x >= 0 && x < nx && y >= 0 && y < ny;
+    col_int[r] = x >= 1 && x < nx - 1 && y >= 1 && y < ny - 1;
+    col[r] = col_in[r] ? (uint32_t)(y * nx + x) * (uint32_t)sizeof(st) : 0u;
+    owned[r] = lane >= T - 1 && lane < 64 - (T - 1) && ty[r] >= T - 1 && ty[r] < BY - (T - 1);
+    center[r] = (ty[r] + 1) * TW + lane + 1;
+  }
+
+  // Level-0 plane staging: element e of the (66 x BY+2) plane.
+  uint32_t goff[PER];
+  int gkind[PER];  // 0: outside the grid, 1: shell column, 2: interior column
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int e = tid + j * NT;
+    const int gx = ox - T + e % TW, gy = oy - T + e / TW;
+    const bool inb = e < PLANE && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+    goff[j] = inb ? (uint32_t)(gy * nx + gx) * (uint32_t)sizeof(st) : 0u;
+    gkind[j] = !inb ? 0 : (gx >= 1 && gx < nx - 1 && gy >= 1 && gy < ny - 1) ? 2 : 1;
+  }
+  // Queues: plane z lives in slot (z - s0) mod Q.
+  float pre[Q][PER];
+  float cq[Q][R][NCOEF];
+  auto load0 = [&](float (&dst)[PER], int z) {
+    const bool zin = z >= 0 && z < nz, zint = z >= 1 && z < nz - 1;
+    const uint32_t zo = (uint32_t)(zin ? z : 0) * plane;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      float v = 0.f;
+      if (zin && gkind[j] != 0) v = ld(gkind[j] == 2 && zint ? in : shell0, zo + goff[j]);
+      dst[j] = v;
+    }
+  };
+  auto store0 = [&](const float (&src)[PER], int z) {
+    float *rg = ring[0][(z + 8) & 3];
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+      if (PLANE % NT == 0 || tid + j * NT < PLANE) rg[tid + j * NT] = src[j];
+  };
+  auto loadc = [&](float (&dst)[R][NCOEF], int z) {
+    const bool zok = z >= 1 && z < nz - 1;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const bool ok = col_int[r] && zok;
+      const uint32_t o = col[r] + (uint32_t)(ok ? z : 0) * plane;
+#pragma unroll
+      for (int n = 0; n < NCOEF; ++n) dst[r][n] = ok ? ld(static_cast<const char *>(a.c[n]), o) : 0.f;
+    }
+  };
+
+  const int s0 = zb - (T - 1);        // first march step (level 1 at plane s0)
+  const int send = ze + T - 1;        // one past the last step
+  {
+    float tmp[PER];
+    load0(tmp, s0 - 1);
+    store0(tmp, s0 - 1);
+    load0(tmp, s0);
+    store0(tmp, s0);
+  }
+  StaticFor([&](auto dc) {
+    constexpr int d = decltype(dc)::value;
+    load0(pre[(d + 1) % Q], s0 + 1 + d);  // planes s0+1 .. s0+D
+    loadc(cq[d % Q], s0 + d);             // planes s0 .. s0+D-1
+  }, std::make_integer_sequence<int, D>{});
+
+  double acc[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) acc[t] = 0.0;
+
+  // One march step s (J = (s - s0) mod Q).
+  auto step = [&](auto jc, const int s) {
+    constexpr int J = decltype(jc)::value;
+    store0(pre[(J + 1) % Q], s + 1);
+    __syncthreads();
+    if (s + 1 + D < ze + T) load0(pre[(J + 1 + D) % Q], s + 1 + D);
+    loadc(cq[(J + D) % Q], s + D);
+    StaticFor([&](auto tc) {
+      constexpr int t = decltype(tc)::value + 1;
+      const int z = s - t + 1;
+      // Needed on this level: lanes/rows [t-1, 64-(t-1)) x [t-1, BY-(t-1)),
+      // planes [zb - (T-t), ze + (T-t)).
+      const bool zneed = z >= zb - (T - t) && z < ze + (T - t) && z >= 0 && z < nz;
+      const bool zint = z >= 1 && z < nz - 1;
+      const float *r0 = ring[t - 1][(z + 7) & 3], *r1 = ring[t - 1][(z + 8) & 3], *r2 = ring[t - 1][(z + 9) & 3];
+      float v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const bool need = zneed && col_in[r] && lane >= t - 1 && lane < 64 - (t - 1) && ty[r] >= t - 1 &&
+                          ty[r] < BY - (t - 1);
+        v[r] = 0.f;
+        if (need) {
+          if (col_int[r] && zint) {
+            const float *g = cq[(J - t + 1 + Q) % Q][r];
+            const int c = center[r];
+            auto P = [&](int dx, int dy, int dz) {
+              const float *rg = dz < 0 ? r0 : dz > 0 ? r2 : r1;
+              return rg[c + dy * TW + dx];
+            };
+            float s0v, ss;
+            s0v = g[A0] * P(0, 0, 1) + g[A1] * P(0, 1, 0) + g[A2] * P(1, 0, 0) +
+                  g[B0] * (P(0, 1, 1) - P(0, -1, 1) - P(0, 1, -1) + P(0, -1, -1)) +
+                  g[B1] * (P(1, 1, 0) - P(1, -1, 0) - P(-1, 1, 0) + P(-1, -1, 0)) +
+                  g[B2] * (P(1, 0, 1) - P(1, 0, -1) - P(-1, 0, 1) + P(-1, 0, -1)) + g[C0] * P(0, 0, -1) +
+                  g[C1] * P(0, -1, 0) + g[C2] * P(-1, 0, 0) + g[WRK1];
+            ss = (s0v * g[A3] - P(0, 0, 0)) * g[BND];
+            v[r] = P(0, 0, 0) + a.omega * ss;
+            if (owned[r] && z >= zb && z < ze) acc[t - 1] += (double)ss * (double)ss;
+            if (t == T && owned[r]) S::store(reinterpret_cast<st *>(out + ((uint32_t)z * plane + col[r])), v[r]);
+          } else if (t < T) {
+            v[r] = ld(static_cast<const char *>(a.shell[(a.k0 + t) & 1]), (uint32_t)z * plane + col[r]);
+          }
+        }
+      }
+      if constexpr (t < T) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          // Intermediate levels hold what a single sweep would have stored.
+          if constexpr (sizeof(st) == 2) v[r] = S::widen(S::narrow(v[r]));
+          ring[t][(z + 8) & 3][center[r]] = v[r];
+        }
+        __syncthreads();
+      }
+    }, std::make_integer_sequence<int, T>{});
+  };
+
+  for (int s = s0; s < send; s += Q) {
+    StaticFor([&](auto jc) {
+      constexpr int J = decltype(jc)::value;
+      if (s + J < send) step(jc, s + J);
+    }, std::make_integer_sequence<int, Q>{});
+  }
+
+  __shared__ double red[NT / 64];
+  const int blocks = gridDim.x * gridDim.y;
+  const int bid = blockIdx.x + blockIdx.y * gridDim.x;
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    const double r = psk::block_reduce<NT>(acc[t], psk::OpSum(), red, tid);
+    if (tid == 0) a.partials[(int64_t)t * blocks + bid] = r;
+  }
+}
+
 // Sums the per-workgroup partials of one sweep in index order.
 __global__ void __launch_bounds__(256) SumPartials(const double *partials, int n, double *out) {
   __shared__ double red[4];
@@ -61,6 +292,16 @@ __global__ void __launch_bounds__(256) SumPartials(const double *partials, int n
   for (int i = threadIdx.x; i < n; i += 256) acc += partials[i];
   const double r = psk::block_reduce<256>(acc, psk::OpSum(), red, threadIdx.x);
   if (threadIdx.x == 0) *out = r;
+}
+
+// One workgroup per level: out[t] = sum of partials[t*n .. t*n + n).
+__global__ void __launch_bounds__(256) SumPartialsLevels(const double *partials, int n, double *out) {
+  __shared__ double red[4];
+  const double *p = partials + (int64_t)blockIdx.x * n;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += p[i];
+  const double r = psk::block_reduce<256>(acc, psk::OpSum(), red, threadIdx.x);
+  if (threadIdx.x == 0) out[blockIdx.x] = r;
 }
 
 struct Variant {
@@ -140,6 +381,136 @@ int RunTyped(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, in
   return PS_HIP_OK;
 }
 
+// ---- temporal-blocking launches -------------------------------------------
+
+// Launch shape of the fused kernel: T sweeps per pass, tiles of 64 x by
+// points, rpw rows per wave (by / rpw waves per workgroup), zchunk output
+// planes per workgroup (0: all), prefetch distance pf in march steps (0:
+// the shape's default).
+struct TBConfig {
+  int T, by, rpw, zchunk, pf = 0;
+};
+
+struct TBGrid {
+  int tiles_x = 0, tiles_y = 0, chunks = 0;
+  int Blocks() const { return tiles_x * tiles_y * chunks; }
+};
+
+TBGrid TBGridFor(const int *dim, const TBConfig &c) {
+  TBGrid g;
+  const int ix = dim[0] - 2, iy = dim[1] - 2, iz = dim[2] - 2;
+  if (ix <= 0 || iy <= 0 || iz <= 0) return g;
+  const int ox = 64 - 2 * (c.T - 1), oy = c.by - 2 * (c.T - 1);
+  const int zc = c.zchunk <= 0 || c.zchunk > iz ? iz : c.zchunk;
+  g.tiles_x = (ix + ox - 1) / ox;
+  g.tiles_y = (iy + oy - 1) / oy;
+  g.chunks = (iz + zc - 1) / zc;
+  return g;
+}
+
+// Compiled shapes (T, waves, rows per wave); the one-sweep shapes of a
+// tile (T = 1) exist for every tile the fused shapes use.
+template <class S>
+const void *TBKernel(int T, int nw, int r, int pf) {
+  if (pf == 0) pf = T == 1 ? 2 : 1;
+#define PSH_TB(TT, NW, RR, DD) \
+  if (T == TT && nw == NW && r == RR && pf == DD) return reinterpret_cast<const void *>(&HimenoTB<S, TT, NW, RR, DD>);
+  PSH_TB(1, 8, 1, 2) PSH_TB(1, 8, 1, 3) PSH_TB(1, 8, 1, 4) PSH_TB(1, 4, 2, 2) PSH_TB(1, 8, 2, 2) PSH_TB(1, 16, 1, 2)
+  PSH_TB(2, 8, 1, 1) PSH_TB(2, 8, 1, 2) PSH_TB(2, 8, 1, 3) PSH_TB(2, 4, 2, 1) PSH_TB(2, 8, 2, 1) PSH_TB(2, 16, 1, 1)
+  PSH_TB(3, 8, 1, 1) PSH_TB(3, 4, 2, 1) PSH_TB(3, 8, 2, 1) PSH_TB(3, 16, 1, 1) PSH_TB(3, 16, 1, 2)
+  PSH_TB(4, 8, 2, 1) PSH_TB(4, 16, 1, 1)
+#undef PSH_TB
+  return nullptr;
+}
+
+const void *TBKernelFor(int storage, int T, int by, int rpw, int pf) {
+  if (rpw <= 0 || by % rpw != 0) return nullptr;
+  const int nw = by / rpw;
+  switch (storage) {
+    case PS_HIP_STORAGE_NATIVE: return TBKernel<psk::Native<float>>(T, nw, rpw, pf);
+    case PS_HIP_STORAGE_BF16: return TBKernel<psk::BF16>(T, nw, rpw, pf);
+    case PS_HIP_STORAGE_FP16: return TBKernel<psk::FP16>(T, nw, rpw, pf);
+  }
+  return nullptr;
+}
+
+// The fused shape and the one-sweep shape of the same tile (for the last
+// sweeps of a run) must both be compiled.
+bool TBSupported(const TBConfig &c) {
+  return c.zchunk >= 0 && c.T >= 1 && TBKernelFor(PS_HIP_STORAGE_NATIVE, c.T, c.by, c.rpw, c.pf) != nullptr &&
+         TBKernelFor(PS_HIP_STORAGE_NATIVE, 1, c.by, c.rpw, 0) != nullptr;
+}
+
+// Scratch doubles a run with configuration c needs for its partials.
+int TBPartials(const __PSGrid *p0, const TBConfig &c) {
+  const TBConfig one{1, c.by, c.rpw, c.zchunk};
+  return std::max(c.T * TBGridFor(p0->dim, c).Blocks(), TBGridFor(p0->dim, one).Blocks());
+}
+
+// One pass of T sweeps: u_{k0+T} = F^T(in) into out (interior only).
+int TBPass(int storage, const TBConfig &c, int T, const void *in, void *out, __PSGrid *A, __PSGrid *B,
+           __PSGrid *const *grids, float omega, int k0, double *gosa_dev, double *partials, hipStream_t s) {
+  const TBConfig cc{T, c.by, c.rpw, c.zchunk};
+  const TBGrid g = TBGridFor(A->dim, cc);
+  if (g.Blocks() == 0) return PS_HIP_OK;
+  const void *k = TBKernelFor(storage, T, c.by, c.rpw, T == c.T ? c.pf : 0);
+  if (!k) return PS_HIP_ERR_UNSUPPORTED;
+  TBArgs a;
+  a.in = in;
+  a.out = out;
+  a.shell[0] = A->p0;
+  a.shell[1] = B->p0;
+  for (int n = 0; n < NCOEF; ++n) a.c[n] = grids[n]->p0;
+  for (int d = 0; d < 3; ++d) a.dim[d] = A->dim[d];
+  a.k0 = k0;
+  const int iz = A->dim[2] - 2;
+  a.zchunk = c.zchunk <= 0 || c.zchunk > iz ? iz : c.zchunk;
+  a.tiles_x = g.tiles_x;
+  a.omega = omega;
+  a.partials = partials;
+  void *args[] = {&a};
+  if (hipLaunchKernel(k, dim3(g.tiles_x * g.tiles_y, g.chunks), dim3(64, c.by / c.rpw), args, 0, s) != hipSuccess)
+    return PS_HIP_ERR_DEVICE;
+  if (gosa_dev) {
+    hipLaunchKernelGGL(SumPartialsLevels, dim3(T), dim3(256), 0, s, partials, g.Blocks(), gosa_dev + k0);
+    if (hipGetLastError() != hipSuccess) return PS_HIP_ERR_DEVICE;
+  }
+  return PS_HIP_OK;
+}
+
+// `sweeps` sweeps starting from u_0 in A, leaving the grids as the
+// one-sweep sequence leaves them (u_n in A for even n, else in B; u_{n-1} in
+// the other).  Passes of c.T sweeps run while at least c.T + 2 sweeps remain
+// (input and output alternate between the third buffer X and the buffer of
+// the output's parity); the rest run one sweep per pass into the buffer of
+// their parity, so the last two always land in A and B.  gosa_dev[i] (may
+// be NULL) receives sweep i's FP64 sum of ss^2.
+int TBRun(const TBConfig &c, __PSGrid *A, __PSGrid *B, void *X, __PSGrid *const *grids, float omega, int sweeps,
+          double *gosa_dev, double *partials, hipStream_t s) {
+  const int storage = Rec(A)->storage;
+  if (Rec(A)->bytes > 0xffffffffull) return PS_HIP_ERR_UNSUPPORTED;  // 32-bit byte offsets
+  auto P = [&](int j) -> void * { return (j & 1) ? B->p0 : A->p0; };
+  void *cur = A->p0;
+  int k = 0;
+  if (c.T > 1) {
+    if (!X) return PS_HIP_ERR_INVALID;
+    while (sweeps - k >= c.T + 2) {
+      void *dst = cur != X ? X : P(k + c.T);
+      const int st = TBPass(storage, c, c.T, cur, dst, A, B, grids, omega, k, gosa_dev, partials, s);
+      if (st != PS_HIP_OK) return st;
+      cur = dst;
+      k += c.T;
+    }
+  }
+  for (; k < sweeps; ++k) {
+    void *dst = P(k + 1);
+    const int st = TBPass(storage, c, 1, cur, dst, A, B, grids, omega, k, gosa_dev, partials, s);
+    if (st != PS_HIP_OK) return st;
+    cur = dst;
+  }
+  return PS_HIP_OK;
+}
+
 }  // namespace
 
 int HimenoPartials(const __PSGrid *p0) { return Blocks(kVariants[0], InteriorBox(p0)); }
@@ -158,9 +529,76 @@ int HimenoEnqueue(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omeg
   return PS_HIP_ERR_INVALID;
 }
 
+int HimenoTBPartials(const __PSGrid *p0, int degree, int by, int rpw, int zchunk) {
+  return TBPartials(p0, TBConfig{degree, by, rpw, zchunk});
+}
+
+int HimenoEnqueueTB(__PSGrid *p0, __PSGrid *p1, void *aux, __PSGrid *const *grids, float omega, int sweeps,
+                    int degree, int by, int rpw, int zchunk, double *gosa_dev, double *partials, hipStream_t s) {
+  const TBConfig c{degree, by, rpw, zchunk};
+  if (!TBSupported(c)) return PS_HIP_ERR_UNSUPPORTED;
+  return TBRun(c, p0, p1, aux, grids, omega, sweeps, gosa_dev, partials, s);
+}
+
 }  // namespace physis_hip
 
 using namespace physis_hip;
+
+namespace {
+
+bool SameGridSet(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids) {
+  if (!p0 || !p1 || !grids) return false;
+  const int storage = Rec(p0)->storage;
+  auto same = [&](const __PSGrid *g) {
+    if (!g || g->num_dims != 3 || g->elm_size != 4 || !g->p0 || Rec(g)->storage != storage) return false;
+    for (int d = 0; d < 3; ++d)
+      if (g->dim[d] != p0->dim[d]) return false;
+    return true;
+  };
+  if (!same(p0) || !same(p1)) return false;
+  for (int n = 0; n < NCOEF; ++n)
+    if (!same(grids[n])) return false;
+  return true;
+}
+
+}  // namespace
+
+// Himeno sweeps with `degree` sweeps fused per HBM pass (1..4), tiles of
+// 64 x `by` points (8 or 16), `rows_per_wave` tile rows per wave (1 or 2)
+// and `zchunk` output planes per workgroup (0: all).  Same contract as PSHipHimenoRun: after the
+// run p0 and p1 hold exactly what `sweeps` single sweeps leave in them, and
+// gosa (may be NULL) receives the FP64 sum of ss^2 of every sweep.  Degree
+// > 1 uses the runtime's third grid buffer.
+extern "C" int PSHipHimenoRunTB(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
+                                double *gosa, int degree, int by, int rows_per_wave, int zchunk,
+                                int prefetch) {
+  if (!SameGridSet(p0, p1, grids) || sweeps < 0) return PS_HIP_ERR_INVALID;
+  const TBConfig c{degree, by, rows_per_wave, zchunk, prefetch};
+  if (!TBSupported(c)) return PS_HIP_ERR_UNSUPPORTED;
+  Runtime &rt = Runtime::Get();
+  hipStream_t s = rt.stream();
+  void *aux = nullptr;
+  if (degree > 1) {
+    aux = rt.Aux(Rec(p0)->bytes);
+    if (!aux) return PS_HIP_ERR_NOMEM;
+  }
+  const size_t nres = sweeps > 0 ? (size_t)sweeps : 1;
+  const size_t npart = (size_t)std::max(TBPartials(p0, c), 1);
+  double *scratch = static_cast<double *>(rt.Scratch((nres + npart) * sizeof(double)));
+  double *gosa_dev = scratch, *partials = scratch + nres;
+  if (gosa && sweeps > 0 && TBGridFor(p0->dim, c).Blocks() == 0) {
+    for (int i = 0; i < sweeps; ++i) gosa[i] = 0.0;
+    gosa = nullptr;
+  }
+  int st = TBRun(c, p0, p1, aux, grids, omega, sweeps, gosa ? gosa_dev : nullptr, partials, s);
+  if (st != PS_HIP_OK) return st;
+  if (gosa && sweeps > 0) {
+    if (hipMemcpyAsync(gosa, gosa_dev, sweeps * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+      return PS_HIP_ERR_DEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return PS_HIP_ERR_DEVICE;
+  }
+  return PS_HIP_OK;
+}
 
 extern "C" int PSHipHimenoRun(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega,
                               int sweeps, double *gosa, int variant) {

@@ -101,6 +101,13 @@ size_t ReduceScratchDoubles();
 int HimenoEnqueue(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
                   double *gosa_dev, double *partials, hipStream_t s);
 int HimenoPartials(const __PSGrid *p0);
+// The same with `degree` sweeps fused per pass (tiles of 64 x by points,
+// rpw rows per wave, zchunk planes per workgroup); aux is a third buffer of
+// the grid's size (unused for degree 1); partials needs HimenoTBPartials(...)
+// doubles.
+int HimenoEnqueueTB(__PSGrid *p0, __PSGrid *p1, void *aux, __PSGrid *const *grids, float omega, int sweeps,
+                    int degree, int by, int rpw, int zchunk, double *gosa_dev, double *partials, hipStream_t s);
+int HimenoTBPartials(const __PSGrid *p0, int degree, int by, int rpw, int zchunk);
 
 // User-defined point types (utype_hip.hip): `dev` is the __PSGridUtype_dev a
 // generated devNewFunc returned through __PSHipUtypeDevNew.

@@ -8,7 +8,12 @@
 //     grids fit the HBM budget (memory is returned when a request retires);
 //   - all active diffusion requests of one storage kind advance together in
 //     one batched launch per step (batched.hip) on the main stream, while the
-//     Jacobi requests sweep on a second stream, so the two kinds overlap;
+//     Jacobi requests sweep on a pool of side streams (request i on stream
+//     i mod 4), so the two kinds, and Jacobi requests among themselves,
+//     overlap.  Jacobi requests run several sweeps per HBM pass (temporal
+//     blocking, himeno19.hip) with a launch shape an in-process tuner picks
+//     per (storage, size class) by timing the candidates on the first
+//     admitted request of that class;
 //   - after each tick every request's residual is computed on the device
 //     (diffusion: ||u_n - u_{n-1}|| / ||u_n||, FP64; Jacobi: rms of ss over
 //     the interior from the fused FP64 epilogue) and finished requests
@@ -18,8 +23,10 @@
 // or in any mix (batching does not change per-grid results, reductions have
 // a fixed order), so results are bitwise independent of the mix.
 // Mode 1 ("sequential") serves the same trace one request at a time through
-// the plain single-grid paths, as a loop of PSStencilRun calls would: the
-// baseline the batched mode is compared with.
+// the plain single-grid paths (one kernel per sweep or per fused diffusion
+// step pair), as a loop of PSStencilRun calls would: the baseline the
+// batched mode is compared with.  Fused sweeps are bitwise equal to single
+// sweeps, so both modes compute the same fields.
 //
 // No reference counterpart (the reference has no serving layer).
 #include "runtime_hip.h"
@@ -94,6 +101,16 @@ void DiffNorm(const __PSGrid *a, const __PSGrid *b, double *partials, double *ou
 
 const float kDiffCoef[7] = {0.1f, 0.1f, 0.1f, 0.1f, 0.1f, 0.1f, 0.4f};
 constexpr float kOmega = 0.8f;
+constexpr int kSideStreams = 4;
+
+// A launch shape of the fused Jacobi kernel: degree, tile rows, rows per
+// wave, z chunk.
+struct JacobiShape {
+  int degree, by, rpw, zchunk;
+};
+// Candidates the tuner times (all compiled shapes of himeno19.hip).
+const JacobiShape kJacobiCandidates[] = {{1, 8, 1, 16}, {2, 8, 1, 32}, {2, 16, 1, 32}, {3, 16, 1, 32}};
+constexpr int kTuneSweeps = 8;
 
 double Seconds(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -117,7 +134,8 @@ struct Live {
   int idx = -1;
   const PSHipRequest *rq = nullptr;
   __PSGrid *A = nullptr, *B = nullptr;  // diffusion u_n, u_{n-1} / Jacobi p0, p1
-  void *aux = nullptr;                  // diffusion third buffer
+  void *aux = nullptr;                  // third buffer (fused steps / sweeps)
+  JacobiShape shape{1, 8, 1, 16};       // Jacobi launch shape (batched mode)
   __PSGrid *coef[12] = {};
   double *dev = nullptr;                // per-request scratch (device)
   int dev_doubles = 0;
@@ -130,7 +148,7 @@ struct Live {
 size_t RequestBytes(const PSHipRequest &r) {
   const size_t pts = (size_t)r.n * r.n * r.n;
   const size_t es = r.storage == PS_HIP_STORAGE_NATIVE ? 4 : 2;
-  return pts * es * (r.kind == 0 ? 3 : 14);
+  return pts * es * (r.kind == 0 ? 3 : 15);
 }
 
 class Server {
@@ -139,24 +157,29 @@ class Server {
       : reqs_(reqs), n_(n), budget_(budget), tick_(tick), out_(out) {
     Runtime &rt = Runtime::Get();
     main_ = rt.stream();
-    CheckHip(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto &st : side_) CheckHip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+    CheckHip(hipEventCreateWithFlags(&ready_, hipEventDisableTiming), "hipEventCreate");
     CheckHip(hipHostMalloc(&host_, sizeof(double) * 4 * (size_t)std::max(n, 1), hipHostMallocDefault),
              "hipHostMalloc");
   }
   ~Server() {
     for (auto &b : batches_) (void)PSHipBatchDestroy(b.second.handle);
-    (void)hipStreamSynchronize(side_);
-    (void)hipStreamDestroy(side_);
+    for (auto &st : side_) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
+    (void)hipEventDestroy(ready_);
     (void)hipHostFree(host_);
   }
 
   int Run(bool sequential);
 
  private:
-  int Admit(int idx, double now, hipStream_t s);
+  int Admit(int idx, double now, hipStream_t s, bool batched);
   void Retire(Live &lv, double now);
-  void AdvanceTick(std::vector<Live *> &act, hipStream_t diff_stream, hipStream_t jac_stream, bool batched);
+  void AdvanceTick(std::vector<Live *> &act, bool batched);
   void Release(Live &lv);
+  JacobiShape TuneJacobi(Live &lv, hipStream_t s);
 
   struct BatchEntry {
     void *handle = nullptr;
@@ -168,7 +191,8 @@ class Server {
   int64_t budget_;
   int tick_;
   PSHipResult *out_;
-  hipStream_t main_, side_;
+  hipStream_t main_, side_[kSideStreams];
+  hipEvent_t ready_ = nullptr;  // orders side-stream work after the main stream
   double *host_ = nullptr;
   int64_t used_ = 0;
   std::vector<Live> live_;
@@ -176,7 +200,46 @@ class Server {
   std::chrono::steady_clock::time_point t0_;
 };
 
-int Server::Admit(int idx, double now, hipStream_t s) {
+// Tuned Jacobi launch shapes per (storage, size class of 64 points): one
+// table for the process, so later servers reuse it.
+std::map<std::pair<int, int>, JacobiShape> &JacobiTuned() {
+  static std::map<std::pair<int, int>, JacobiShape> t;
+  return t;
+}
+
+// Times every candidate shape for kTuneSweeps sweeps on the request's own
+// (not yet initialised) grids and returns the fastest.
+JacobiShape Server::TuneJacobi(Live &lv, hipStream_t s) {
+  const std::pair<int, int> key(lv.rq->storage, (lv.rq->n + 63) / 64);
+  auto it = JacobiTuned().find(key);
+  if (it != JacobiTuned().end()) return it->second;
+  hipEvent_t e0, e1;
+  CheckHip(hipEventCreate(&e0), "event");
+  CheckHip(hipEventCreate(&e1), "event");
+  JacobiShape best = kJacobiCandidates[0];
+  float best_ms = 1e30f;
+  for (const JacobiShape &c : kJacobiCandidates) {
+    double *gosa = lv.dev, *partials = lv.dev + tick_;
+    CheckHip(hipEventRecord(e0, s), "record");
+    if (HimenoEnqueueTB(lv.A, lv.B, lv.aux, lv.coef, kOmega, kTuneSweeps, c.degree, c.by, c.rpw, c.zchunk, gosa,
+                        partials, s) != PS_HIP_OK)
+      continue;
+    CheckHip(hipEventRecord(e1, s), "record");
+    CheckHip(hipEventSynchronize(e1), "sync");
+    float ms = 0.f;
+    CheckHip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  JacobiTuned()[key] = best;
+  return best;
+}
+
+int Server::Admit(int idx, double now, hipStream_t s, bool batched) {
   const PSHipRequest &r = reqs_[idx];
   Live lv;
   lv.idx = idx;
@@ -186,15 +249,20 @@ int Server::Admit(int idx, double now, hipStream_t s) {
   auto grid = [&](__PSGrid **g) { return PSHipGridNew(g, 4, 3, dim, r.storage); };
   int st = grid(&lv.A);
   if (st == PS_HIP_OK) st = grid(&lv.B);
+  if (st == PS_HIP_OK && hipMalloc(&lv.aux, (size_t)r.n * r.n * r.n * (r.storage ? 2 : 4)) != hipSuccess) {
+    (void)hipGetLastError();
+    st = PS_HIP_ERR_NOMEM;
+  }
   if (r.kind == 0) {
-    if (st == PS_HIP_OK && hipMalloc(&lv.aux, (size_t)r.n * r.n * r.n * (r.storage ? 2 : 4)) != hipSuccess) {
-      (void)hipGetLastError();
-      st = PS_HIP_ERR_NOMEM;
-    }
     lv.dev_doubles = 2 + 2 * kNormBlocks + (int)ReduceScratchDoubles();
   } else {
     for (int i = 0; i < 12 && st == PS_HIP_OK; ++i) st = grid(&lv.coef[i]);
-    if (st == PS_HIP_OK) lv.dev_doubles = tick_ + HimenoPartials(lv.A) + (int)ReduceScratchDoubles() + 1;
+    if (st == PS_HIP_OK) {
+      int parts = HimenoPartials(lv.A);
+      for (const JacobiShape &c : kJacobiCandidates)
+        parts = std::max(parts, HimenoTBPartials(lv.A, c.degree, c.by, c.rpw, c.zchunk));
+      lv.dev_doubles = std::max(tick_, kTuneSweeps) + parts + (int)ReduceScratchDoubles() + 1;
+    }
   }
   if (st == PS_HIP_OK && hipMalloc(&lv.dev, sizeof(double) * lv.dev_doubles) != hipSuccess) {
     (void)hipGetLastError();
@@ -206,6 +274,7 @@ int Server::Admit(int idx, double now, hipStream_t s) {
   }
   lv.bytes = RequestBytes(r);
   used_ += (int64_t)lv.bytes;
+  if (r.kind == 1 && batched) lv.shape = TuneJacobi(lv, s);
   // Initial state.
   if (r.kind == 0) {
     FillUniformOn(lv.A, r.seed, 0.f, 1.f, s);
@@ -266,7 +335,8 @@ void Server::Retire(Live &lv, double now) {
   Release(lv);
 }
 
-void Server::AdvanceTick(std::vector<Live *> &act, hipStream_t ds, hipStream_t js, bool batched) {
+void Server::AdvanceTick(std::vector<Live *> &act, bool batched) {
+  hipStream_t ds = main_;
   // Diffusion: one batch per storage kind (rebuilt when its membership changes).
   std::map<int, std::vector<Live *>> groups;
   for (Live *lv : act)
@@ -306,18 +376,34 @@ void Server::AdvanceTick(std::vector<Live *> &act, hipStream_t ds, hipStream_t j
       CheckHip(hipMemcpyAsync(host_ + 2 * lv->idx, lv->dev, 2 * sizeof(double), hipMemcpyDeviceToHost, ds),
                "norm copy");
   }
-  // Jacobi requests sweep on their own stream.
+  // Jacobi requests: fused sweeps on the side streams (batched mode) or
+  // single sweeps one after another on the main stream (sequential mode).
+  std::vector<hipStream_t> used;
   for (Live *lv : act) {
     if (lv->rq->kind != 1) continue;
     double *gosa = lv->dev;
-    double *partials = lv->dev + tick_;
-    if (HimenoEnqueue(lv->A, lv->B, lv->coef, kOmega, tick_, gosa, partials, js) != PS_HIP_OK)
+    double *partials = lv->dev + std::max(tick_, kTuneSweeps);
+    hipStream_t js = batched ? side_[lv->idx % kSideStreams] : main_;
+    if (batched) {
+      // The side stream starts after the main stream's pending work (the
+      // request's initial fills).
+      if (std::find(used.begin(), used.end(), js) == used.end()) {
+        CheckHip(hipEventRecord(ready_, main_), "record");
+        CheckHip(hipStreamWaitEvent(js, ready_, 0), "wait");
+        used.push_back(js);
+      }
+      const JacobiShape &c = lv->shape;
+      if (HimenoEnqueueTB(lv->A, lv->B, lv->aux, lv->coef, kOmega, tick_, c.degree, c.by, c.rpw, c.zchunk, gosa,
+                          partials, js) != PS_HIP_OK)
+        Fatal("serve", "jacobi run failed");
+    } else if (HimenoEnqueue(lv->A, lv->B, lv->coef, kOmega, tick_, gosa, partials, js) != PS_HIP_OK) {
       Fatal("serve", "jacobi run failed");
+    }
     CheckHip(hipMemcpyAsync(host_ + 2 * lv->idx, gosa + tick_ - 1, sizeof(double), hipMemcpyDeviceToHost, js),
              "gosa copy");
   }
   CheckHip(hipStreamSynchronize(ds), "tick sync");
-  CheckHip(hipStreamSynchronize(js), "tick sync");
+  for (hipStream_t js : used) CheckHip(hipStreamSynchronize(js), "tick sync");
   for (Live *lv : act) {
     lv->steps += tick_;
     const double *h = host_ + 2 * lv->idx;
@@ -355,7 +441,7 @@ int Server::Run(bool sequential) {
         continue;
       }
       if (used_ + need > budget_) break;
-      const int st = Admit(idx, Seconds(t0_), main_);
+      const int st = Admit(idx, Seconds(t0_), main_, !sequential);
       if (st == PS_HIP_ERR_NOMEM && !live_.empty()) break;  // wait for memory to free up
       waiting.pop_front();
       if (st != PS_HIP_OK) {
@@ -368,10 +454,9 @@ int Server::Run(bool sequential) {
       std::this_thread::sleep_for(std::chrono::microseconds(200));
       continue;
     }
-    CheckHip(hipStreamSynchronize(main_), "admission sync");
     std::vector<Live *> act;
     for (Live &lv : live_) act.push_back(&lv);
-    AdvanceTick(act, main_, sequential ? main_ : side_, !sequential);
+    AdvanceTick(act, !sequential);
     // Retire finished requests.
     const double t = Seconds(t0_);
     std::vector<Live> keep;

@@ -233,6 +233,22 @@ def himeno_run(p0, p1, coef_grids, omega, sweeps, with_gosa=True, variant=0):
     return np.array(gosa[:sweeps], np.float64)
 
 
+def himeno_run_tb(p0, p1, coef_grids, omega, sweeps, degree, by=16, rows_per_wave=1, zchunk=0, prefetch=0,
+                  with_gosa=True):
+    """Himeno sweeps with `degree` sweeps fused per HBM pass (tiles of 64 x by
+    points, rows_per_wave rows per wave, zchunk output planes per workgroup,
+    0 = all); same results and final grid contents as himeno_run.  Returns
+    the FP64 sum of ss^2 per sweep."""
+    L = lib()
+    L.PSHipHimenoRunTB.argtypes = [_GP, _GP, ctypes.POINTER(_GP), ctypes.c_float, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_double)] + [ctypes.c_int] * 5
+    arr = (_GP * 12)(*[g.handle for g in coef_grids])
+    gosa = (ctypes.c_double * max(sweeps, 1))()
+    check(L.PSHipHimenoRunTB(p0.handle, p1.handle, arr, omega, sweeps, gosa if with_gosa else None,
+                             degree, by, rows_per_wave, zchunk, prefetch), "PSHipHimenoRunTB")
+    return np.array(gosa[:sweeps], np.float64)
+
+
 class Batch:
     """Many independent 7-point diffusion grid pairs advanced together: one
     launch per step (or per two fused steps) for all grids, captured into a

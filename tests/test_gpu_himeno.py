@@ -105,3 +105,65 @@ def test_himeno_16bit_storage_vs_fp64(rt, storage, tol):
     err = np.max(np.abs(got0 - want0))
     print("storage", storage, "max abs error vs FP64", err, flush=True)
     assert err < tol
+
+
+# Fused-sweep (temporal blocking) configurations: (degree, tile rows, rows
+# per wave, z chunk).
+TB_CONFIGS = [(1, 8, 1, 0), (1, 16, 2, 5), (2, 8, 1, 0), (2, 8, 2, 7), (2, 16, 2, 3), (2, 16, 1, 0), (3, 8, 1, 4),
+              (3, 8, 2, 0), (3, 16, 1, 0), (3, 16, 2, 6), (4, 16, 1, 0), (4, 16, 2, 2)]
+
+
+@pytest.mark.parametrize("shape", [(65, 33, 33), (19, 7, 5), (3, 3, 3), (129, 17, 9), (70, 40, 23), (4, 30, 12)])
+def test_himeno_fused_sweeps_bitwise_vs_oracle(rt, shape):
+    """T sweeps per pass leave p0, p1 and every sweep's residual exactly as
+    T single sweeps do.  p1 starts different from p0 (its boundary shell
+    too), so a pass that took the shell of the wrong iterate would show."""
+    p, coefs = himeno_inputs(shape, seed=400)
+    nz, ny, nx = p.shape
+    p1_init = O.uniform_field(shape, seed=77, lo=-1.0, hi=1.0)
+    for sweeps in (1, 2, 5, 6, 11):
+        want0, want1, wgosa = O.himeno(p, coefs, 0.8, sweeps, p1_init=p1_init)
+        for degree, by, rpw, zchunk in TB_CONFIGS:
+            print("shape", shape, "sweeps", sweeps, "config", (degree, by, rpw, zchunk), flush=True)
+            g0, g1 = rt.Grid(shape), rt.Grid(shape)
+            g0.copyin(p)
+            g1.copyin(p1_init)
+            grids = []
+            for c in coefs:
+                g = rt.Grid(shape)
+                g.copyin(c)
+                grids.append(g)
+            gosa = rt.himeno_run_tb(g0, g1, grids, 0.8, sweeps, degree, by, rpw, zchunk)
+            got0, got1 = g0.copyout(), g1.copyout()
+            for g in grids + [g0, g1]:
+                g.free()
+            assert np.array_equal(got0, want0), "p0 differs"
+            assert np.array_equal(got1, want1), "p1 differs"
+            np.testing.assert_allclose(gosa, wgosa, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("storage", [1, 2])
+def test_himeno_fused_sweeps_16bit_equal_single_sweeps(rt, storage):
+    """With 16-bit storage every level is rounded to storage, so fused and
+    single sweeps agree bitwise too."""
+    shape = (97, 45, 21)
+    p, coefs = himeno_inputs(shape, seed=500)
+    results = []
+    for degree, by, rpw, zchunk in [(1, 16, 1, 0), (2, 16, 2, 0), (3, 8, 1, 5), (4, 16, 1, 0), (4, 16, 2, 9)]:
+        print("storage", storage, "config", (degree, by, rpw, zchunk), flush=True)
+        g0, g1 = rt.Grid(shape, storage=storage), rt.Grid(shape, storage=storage)
+        g0.copyin(p)
+        g1.copyin(p)
+        grids = []
+        for c in coefs:
+            g = rt.Grid(shape, storage=storage)
+            g.copyin(c)
+            grids.append(g)
+        gosa = rt.himeno_run_tb(g0, g1, grids, 0.8, 14, degree, by, rpw, zchunk)
+        results.append((g0.copyout(), g1.copyout(), gosa))
+        for g in grids + [g0, g1]:
+            g.free()
+    ref = run_hip(rt, p, coefs, 0.8, 14, storage=storage)
+    for got in results:
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+        np.testing.assert_allclose(got[2], ref[2], rtol=1e-12, atol=1e-300)

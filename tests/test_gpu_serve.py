@@ -59,7 +59,7 @@ def test_sequential_mode_gives_identical_results(rt, mixed):
 
 
 def test_tight_budget_admits_one_at_a_time_with_same_results(rt, mixed):
-    biggest = max(q["n"] ** 3 * (4 if q["storage"] == 0 else 2) * (3 if q["kind"] == 0 else 14) for q in MIX)
+    biggest = max(q["n"] ** 3 * (4 if q["storage"] == 0 else 2) * (3 if q["kind"] == 0 else 15) for q in MIX)
     res = rt.serve_trace(MIX, biggest, TICK, rt.SERVE_BATCHED)
     for q, a, b in zip(MIX, mixed, res):
         assert key(a) == key(b), q
