@@ -32,7 +32,11 @@
 //   static constexpr bool kWrap;      periodic halo
 //   static constexpr bool kReduce;    point() returns a double that is summed
 //                                     (FP64, fixed order) into
-//                                     a.partials[workgroup]
+//                                     a.partials[workgroup]; workgroup 0 of
+//                                     the launch first calls
+//                                     Plan::fold_previous<NT>(a, lds, tid),
+//                                     which may sum the partials of the
+//                                     previous launch (complete by then)
 //   struct Args;                      by-value kernel arguments
 //   static void stage_dims(const Args&, int&, int&, int&);   staged grid extents
 //   static SV stage_load(const Args&, int s, int64_t off);    element of staged grid s
@@ -171,6 +175,11 @@ ZMarch(typename Plan::Args a, Box box, Geometry geo) {
       store_plane(zb + k);
     }
     load_plane(zb + RZ);
+  }
+
+  if constexpr (Plan::kReduce) {
+    __shared__ double fold[NT / 64 > 0 ? NT / 64 : 1];
+    if (blockIdx.x == 0 && blockIdx.y == 0) Plan::template fold_previous<NT>(a, fold, tid);
   }
 
   Ctx<SV, NSLOT, PLANE, TW, RZ> c;

@@ -85,4 +85,14 @@ __device__ __forceinline__ T block_reduce(T v, Op op, T *lds, int tid) {
   return r;
 }
 
+// Sum of p[0 .. n) over a workgroup of kThreads threads: thread i adds
+// p[i], p[i + kThreads], ... in order, then the workgroup tree-reduces.  The
+// grouping is fixed, so the result is bitwise reproducible.
+template <int kThreads>
+__device__ __forceinline__ double block_sum(const double *p, int n, double *lds, int tid) {
+  double acc = 0.0;
+  for (int i = tid; i < n; i += kThreads) acc += p[i];
+  return block_reduce<kThreads>(acc, OpSum(), lds, tid);
+}
+
 }  // namespace psk

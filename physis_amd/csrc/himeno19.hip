@@ -34,7 +34,18 @@ struct HimenoPlan {
     int dim[3];
     float omega;
     double *partials;
+    // The previous sweep's partials, summed by workgroup 0 into *prev_out
+    // (none when prev_out is NULL).
+    const double *prev_partials;
+    int prev_blocks;
+    double *prev_out;
   };
+  template <int NT>
+  __device__ static void fold_previous(const Args &a, double *lds, int tid) {
+    if (!a.prev_out) return;
+    const double r = psk::block_sum<NT>(a.prev_partials, a.prev_blocks, lds, tid);
+    if (tid == 0) *a.prev_out = r;
+  }
   __device__ static void stage_dims(const Args &a, int &nx, int &ny, int &nz) {
     nx = a.dim[0];
     ny = a.dim[1];
@@ -94,6 +105,11 @@ struct TBArgs {
   int tiles_x;
   float omega;
   double *partials;  // [T][blocks]
+  // The previous launch's partials ([prev_levels][prev_blocks]), summed by
+  // workgroup 0 into prev_out[0 .. prev_levels) (none when prev_out is NULL).
+  const double *prev_partials;
+  int prev_levels, prev_blocks;
+  double *prev_out;
 };
 
 // Calls f(std::integral_constant<int, J>{}) for J = 0 .. N-1 in order.
@@ -212,6 +228,14 @@ HimenoTB(TBArgs a) {
 #pragma unroll
   for (int t = 0; t < T; ++t) acc[t] = 0.0;
 
+  __shared__ double red[NT / 64];
+  if (blockIdx.x == 0 && blockIdx.y == 0 && a.prev_out) {
+    for (int t = 0; t < a.prev_levels; ++t) {
+      const double r = psk::block_sum<NT>(a.prev_partials + (int64_t)t * a.prev_blocks, a.prev_blocks, red, tid);
+      if (tid == 0) a.prev_out[t] = r;
+    }
+  }
+
   // One march step s (J = (s - s0) mod Q).
   auto step = [&](auto jc, const int s) {
     constexpr int J = decltype(jc)::value;
@@ -275,7 +299,6 @@ HimenoTB(TBArgs a) {
     }, std::make_integer_sequence<int, Q>{});
   }
 
-  __shared__ double red[NT / 64];
   const int blocks = gridDim.x * gridDim.y;
   const int bid = blockIdx.x + blockIdx.y * gridDim.x;
 #pragma unroll
@@ -342,17 +365,23 @@ int EnqueueTyped(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega
   for (int n = 0; n < NCOEF; ++n) a.c[n] = static_cast<const st *>(grids[n]->p0);
   for (int d = 0; d < 3; ++d) a.dim[d] = p0->dim[d];
   a.omega = omega;
-  a.partials = partials;
+  a.prev_blocks = nblocks;
+  // Partials alternate between two halves of the buffer; each sweep's
+  // workgroup 0 sums the previous sweep's half, and one kernel sums the last.
   for (int i = 0; i < sweeps; ++i) {
     a.p = static_cast<const st *>((i & 1) ? p1->p0 : p0->p0);
     a.q = static_cast<st *>((i & 1) ? p0->p0 : p1->p0);
+    a.partials = partials + (size_t)(i & 1) * nblocks;
+    a.prev_partials = partials + (size_t)((i + 1) & 1) * nblocks;
+    a.prev_out = gosa_dev && i > 0 ? gosa_dev + i - 1 : nullptr;
     hipError_t e = v.by == 4 ? psk::LaunchZMarch<P, 64, 4>(a, box, v.zchunk, true, s)
                              : psk::LaunchZMarch<P, 64, 8>(a, box, v.zchunk, true, s);
     if (e != hipSuccess) return PS_HIP_ERR_DEVICE;
-    if (gosa_dev && nblocks > 0) {
-      hipLaunchKernelGGL(SumPartials, dim3(1), dim3(256), 0, s, partials, nblocks, gosa_dev + i);
-      if (hipGetLastError() != hipSuccess) return PS_HIP_ERR_DEVICE;
-    }
+  }
+  if (gosa_dev && nblocks > 0 && sweeps > 0) {
+    hipLaunchKernelGGL(SumPartials, dim3(1), dim3(256), 0, s, partials + (size_t)((sweeps - 1) & 1) * nblocks, nblocks,
+                       gosa_dev + sweeps - 1);
+    if (hipGetLastError() != hipSuccess) return PS_HIP_ERR_DEVICE;
   }
   return PS_HIP_OK;
 }
@@ -364,7 +393,7 @@ int RunTyped(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, in
   hipStream_t s = rt.stream();
   const int nblocks = Blocks(v, InteriorBox(p0));
   const size_t nres = sweeps > 0 ? (size_t)sweeps : 1;
-  double *scratch = static_cast<double *>(rt.Scratch((nres + (size_t)nblocks + 1) * sizeof(double)));
+  double *scratch = static_cast<double *>(rt.Scratch((nres + 2 * (size_t)nblocks + 1) * sizeof(double)));
   double *gosa_dev = scratch;
   double *partials = scratch + nres;
   int st = EnqueueTyped<S>(p0, p1, grids, omega, sweeps, gosa ? gosa_dev : nullptr, partials, v, s);
@@ -441,15 +470,27 @@ bool TBSupported(const TBConfig &c) {
          TBKernelFor(PS_HIP_STORAGE_NATIVE, 1, c.by, c.rpw, 0) != nullptr;
 }
 
-// Scratch doubles a run with configuration c needs for its partials.
-int TBPartials(const __PSGrid *p0, const TBConfig &c) {
+// Scratch doubles a run with configuration c needs for its partials: two
+// halves, each large enough for one launch.
+int TBHalf(const __PSGrid *p0, const TBConfig &c) {
   const TBConfig one{1, c.by, c.rpw, c.zchunk};
   return std::max(c.T * TBGridFor(p0->dim, c).Blocks(), TBGridFor(p0->dim, one).Blocks());
 }
+int TBPartials(const __PSGrid *p0, const TBConfig &c) { return 2 * TBHalf(p0, c); }
 
-// One pass of T sweeps: u_{k0+T} = F^T(in) into out (interior only).
+// Partials of the launch before the current one, still to be summed.
+struct Pending {
+  const double *partials = nullptr;
+  int levels = 0, blocks = 0;
+  double *out = nullptr;
+};
+
+// One pass of T sweeps: u_{k0+T} = F^T(in) into out (interior only).  Its
+// workgroup 0 sums the pending partials of the previous pass; this pass's
+// own partials (in `partials`) become pending.
 int TBPass(int storage, const TBConfig &c, int T, const void *in, void *out, __PSGrid *A, __PSGrid *B,
-           __PSGrid *const *grids, float omega, int k0, double *gosa_dev, double *partials, hipStream_t s) {
+           __PSGrid *const *grids, float omega, int k0, double *gosa_dev, double *partials, Pending &pend,
+           hipStream_t s) {
   const TBConfig cc{T, c.by, c.rpw, c.zchunk};
   const TBGrid g = TBGridFor(A->dim, cc);
   if (g.Blocks() == 0) return PS_HIP_OK;
@@ -468,12 +509,19 @@ int TBPass(int storage, const TBConfig &c, int T, const void *in, void *out, __P
   a.tiles_x = g.tiles_x;
   a.omega = omega;
   a.partials = partials;
+  a.prev_partials = pend.partials;
+  a.prev_levels = pend.levels;
+  a.prev_blocks = pend.blocks;
+  a.prev_out = pend.out;
   void *args[] = {&a};
   if (hipLaunchKernel(k, dim3(g.tiles_x * g.tiles_y, g.chunks), dim3(64, c.by / c.rpw), args, 0, s) != hipSuccess)
     return PS_HIP_ERR_DEVICE;
+  pend = Pending();
   if (gosa_dev) {
-    hipLaunchKernelGGL(SumPartialsLevels, dim3(T), dim3(256), 0, s, partials, g.Blocks(), gosa_dev + k0);
-    if (hipGetLastError() != hipSuccess) return PS_HIP_ERR_DEVICE;
+    pend.partials = partials;
+    pend.levels = T;
+    pend.blocks = g.Blocks();
+    pend.out = gosa_dev + k0;
   }
   return PS_HIP_OK;
 }
@@ -490,13 +538,17 @@ int TBRun(const TBConfig &c, __PSGrid *A, __PSGrid *B, void *X, __PSGrid *const 
   const int storage = Rec(A)->storage;
   if (Rec(A)->bytes > 0xffffffffull) return PS_HIP_ERR_UNSUPPORTED;  // 32-bit byte offsets
   auto P = [&](int j) -> void * { return (j & 1) ? B->p0 : A->p0; };
+  const int half = TBHalf(A, c);
+  Pending pend;
+  int launches = 0;
+  auto part = [&]() { return partials + (size_t)(launches++ & 1) * half; };
   void *cur = A->p0;
   int k = 0;
   if (c.T > 1) {
     if (!X) return PS_HIP_ERR_INVALID;
     while (sweeps - k >= c.T + 2) {
       void *dst = cur != X ? X : P(k + c.T);
-      const int st = TBPass(storage, c, c.T, cur, dst, A, B, grids, omega, k, gosa_dev, partials, s);
+      const int st = TBPass(storage, c, c.T, cur, dst, A, B, grids, omega, k, gosa_dev, part(), pend, s);
       if (st != PS_HIP_OK) return st;
       cur = dst;
       k += c.T;
@@ -504,16 +556,20 @@ int TBRun(const TBConfig &c, __PSGrid *A, __PSGrid *B, void *X, __PSGrid *const 
   }
   for (; k < sweeps; ++k) {
     void *dst = P(k + 1);
-    const int st = TBPass(storage, c, 1, cur, dst, A, B, grids, omega, k, gosa_dev, partials, s);
+    const int st = TBPass(storage, c, 1, cur, dst, A, B, grids, omega, k, gosa_dev, part(), pend, s);
     if (st != PS_HIP_OK) return st;
     cur = dst;
+  }
+  if (pend.out) {
+    hipLaunchKernelGGL(SumPartialsLevels, dim3(pend.levels), dim3(256), 0, s, pend.partials, pend.blocks, pend.out);
+    if (hipGetLastError() != hipSuccess) return PS_HIP_ERR_DEVICE;
   }
   return PS_HIP_OK;
 }
 
 }  // namespace
 
-int HimenoPartials(const __PSGrid *p0) { return Blocks(kVariants[0], InteriorBox(p0)); }
+int HimenoPartials(const __PSGrid *p0) { return 2 * Blocks(kVariants[0], InteriorBox(p0)); }
 
 int HimenoEnqueue(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
                   double *gosa_dev, double *partials, hipStream_t s) {
