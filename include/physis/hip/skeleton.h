@@ -46,11 +46,24 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+#include <utility>
+
 #include "physis/physis_hip.h"
 #include "physis/hip/storage.h"
 #include "physis/hip/wave_reduce.h"
 
 namespace psk {
+
+// Calls f(std::integral_constant<int, J>{}) for J = 0 .. N-1 in order.  The
+// z-march loops are unrolled with it by the depth of their register queues:
+// a plane then stays in the registers it was loaded into (renamed at
+// compile time), because a register copy of a load still in flight would
+// make the wave wait for that load.
+template <class F, int... J>
+__device__ __forceinline__ void StaticFor(F &&f, std::integer_sequence<int, J...>) {
+  (f(std::integral_constant<int, J>{}), ...);
+}
 
 // Half-open box [lo, hi) per axis.
 struct Box {

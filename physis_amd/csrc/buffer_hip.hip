@@ -44,11 +44,28 @@ fill_uniform(typename S::stored_t *dst, int64_t n, uint64_t seed, float lo, floa
 // yardstick that stencil kernels and HBM counter readings are scaled by.
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// Each thread keeps four 16-byte loads in flight (issued before any store)
+// and both directions stream past the caches: the fastest plain copy
+// measured on MI355X (about 6.3 TB/s read + write, against 5.4 TB/s for one
+// load in flight with cached loads).
+constexpr int kCopyUnroll = 4;
+
 __global__ void __launch_bounds__(256)
 stream_copy(v4f *__restrict__ dst, const v4f *__restrict__ src, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) __builtin_nontemporal_store(src[i], dst + i);
+  const int64_t stride = (int64_t)gridDim.x * 256 * kCopyUnroll;
+  for (int64_t base = (int64_t)blockIdx.x * 256 * kCopyUnroll + threadIdx.x; base < n; base += stride) {
+    v4f v[kCopyUnroll];
+#pragma unroll
+    for (int u = 0; u < kCopyUnroll; ++u) {
+      const int64_t i = base + u * 256;
+      if (i < n) v[u] = __builtin_nontemporal_load(src + i);
+    }
+#pragma unroll
+    for (int u = 0; u < kCopyUnroll; ++u) {
+      const int64_t i = base + u * 256;
+      if (i < n) __builtin_nontemporal_store(v[u], dst + i);
+    }
+  }
 }
 
 unsigned BlocksFor(int64_t n) {
@@ -91,7 +108,7 @@ extern "C" int PSHipStreamCopy(__PSGrid *dst, const __PSGrid *src) {
   const GridRec *rs = Rec(src);
   if (rs->bytes != Rec(dst)->bytes || rs->bytes % 16 != 0) return PS_HIP_ERR_INVALID;
   const int64_t n = (int64_t)(rs->bytes / 16);
-  int64_t b = (n + 255) / 256;
+  int64_t b = (n + 256 * kCopyUnroll - 1) / (256 * kCopyUnroll);
   if (b > 256 * 64) b = 256 * 64;
   hipLaunchKernelGGL(stream_copy, dim3((unsigned)b), dim3(256), 0, Runtime::Get().stream(),
                      static_cast<v4f *>(dst->p0), static_cast<const v4f *>(src->p0), n);

@@ -112,12 +112,6 @@ struct TBArgs {
   double *prev_out;
 };
 
-// Calls f(std::integral_constant<int, J>{}) for J = 0 .. N-1 in order.
-template <class F, int... J>
-__device__ __forceinline__ void StaticFor(F &&f, std::integer_sequence<int, J...>) {
-  (f(std::integral_constant<int, J>{}), ...);
-}
-
 // A workgroup is NW waves of 64 lanes; wave w holds the R rows w, w + NW,
 // ... of a BY = NW * R row tile (R independent points per thread keep more
 // loads and LDS reads in flight between barriers).  Prefetch distance D:
@@ -218,7 +212,7 @@ HimenoTB(TBArgs a) {
     load0(tmp, s0);
     store0(tmp, s0);
   }
-  StaticFor([&](auto dc) {
+  psk::StaticFor([&](auto dc) {
     constexpr int d = decltype(dc)::value;
     load0(pre[(d + 1) % Q], s0 + 1 + d);  // planes s0+1 .. s0+D
     loadc(cq[d % Q], s0 + d);             // planes s0 .. s0+D-1
@@ -243,7 +237,7 @@ HimenoTB(TBArgs a) {
     __syncthreads();
     if (s + 1 + D < ze + T) load0(pre[(J + 1 + D) % Q], s + 1 + D);
     loadc(cq[(J + D) % Q], s + D);
-    StaticFor([&](auto tc) {
+    psk::StaticFor([&](auto tc) {
       constexpr int t = decltype(tc)::value + 1;
       const int z = s - t + 1;
       // Needed on this level: lanes/rows [t-1, 64-(t-1)) x [t-1, BY-(t-1)),
@@ -293,7 +287,7 @@ HimenoTB(TBArgs a) {
   };
 
   for (int s = s0; s < send; s += Q) {
-    StaticFor([&](auto jc) {
+    psk::StaticFor([&](auto jc) {
       constexpr int J = decltype(jc)::value;
       if (s + J < send) step(jc, s + J);
     }, std::make_integer_sequence<int, Q>{});
