@@ -121,12 +121,17 @@ struct Ctx {
   }
 };
 
-template <class Plan, int BX, int BY>
+// R is the number of tile rows per thread: a workgroup of BX x BY threads
+// owns a BX x (BY * R) tile, thread (tx, ty) computing rows ty, ty + BY, ...
+// The R points are independent, so more loads and LDS reads are in flight
+// per barrier, and the y halo is shared by more rows.
+template <class Plan, int BX, int BY, int R = 1>
 __global__ void __launch_bounds__(BX * BY)
 ZMarch(typename Plan::Args a, Box box, Geometry geo) {
   using SV = typename Plan::SV;
   constexpr int RX = Plan::RX, RY = Plan::RY, RZ = Plan::RZ, NS = Plan::NS;
-  constexpr int TW = BX + 2 * RX, TH = BY + 2 * RY, PLANE = TW * TH;
+  constexpr int TY = BY * R;  // tile rows
+  constexpr int TW = BX + 2 * RX, TH = TY + 2 * RY, PLANE = TW * TH;
   constexpr int NSLOT = 2 * RZ + 2;
   constexpr int NT = BX * BY;
   constexpr int PER = (PLANE + NT - 1) / NT;
@@ -136,12 +141,11 @@ ZMarch(typename Plan::Args a, Box box, Geometry geo) {
   const int ntiles = geo.tiles_x * geo.tiles_y;
   const int b = geo.remap ? XcdTile(blockIdx.x, ntiles) : (int)blockIdx.x;
   const int tile_x = b % geo.tiles_x, tile_y = b / geo.tiles_x;
-  const int x0 = box.lo[0] + tile_x * BX, y0 = box.lo[1] + tile_y * BY;
+  const int x0 = box.lo[0] + tile_x * BX, y0 = box.lo[1] + tile_y * TY;
   const int zb = box.lo[2] + (int)blockIdx.y * geo.zchunk;
   const int ze = min(zb + geo.zchunk, box.hi[2]);
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
-  const int x = x0 + tx, y = y0 + ty;
-  const bool in_box = x < box.hi[0] && y < box.hi[1];
+  const int x = x0 + tx;
 
   SV buf[NSA][PER];
   int goff[PER];
@@ -195,25 +199,37 @@ ZMarch(typename Plan::Args a, Box box, Geometry geo) {
     if (blockIdx.x == 0 && blockIdx.y == 0) Plan::template fold_previous<NT>(a, fold, tid);
   }
 
-  Ctx<SV, NSLOT, PLANE, TW, RZ> c;
-  c.lds = lds;
-  c.center = (ty + RY) * TW + tx + RX;
+  Ctx<SV, NSLOT, PLANE, TW, RZ> c[R];
+  int y[R];
+  bool in_box[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    y[r] = y0 + ty + r * BY;
+    in_box[r] = x < box.hi[0] && y[r] < box.hi[1];
+    c[r].lds = lds;
+    c[r].center = (ty + r * BY + RY) * TW + tx + RX;
+  }
   double acc = 0.0;
   for (int z = zb; z < ze; ++z) {
-    // Red-black maps (reference translator/reference_translator.cc:462-477)
-    // update only the points of one colour.
-    const bool active = in_box && (geo.parity < 0 || ((x + y + z + geo.parity) & 1) == 0);
     if constexpr (NS > 0) {
       store_plane(z + RZ);
       __syncthreads();
       if (z + 1 < ze) load_plane(z + RZ + 1);
 #pragma unroll
-      for (int k = -RZ; k <= RZ; ++k) c.slot[k + RZ] = (z + k - zb + RZ) % NSLOT;
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int k = -RZ; k <= RZ; ++k) c[r].slot[k + RZ] = (z + k - zb + RZ) % NSLOT;
     }
-    if constexpr (Plan::kReduce) {
-      if (active) acc += Plan::point(c, a, x, y, z);
-    } else {
-      if (active) Plan::point(c, a, x, y, z);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      // Red-black maps (reference translator/reference_translator.cc:462-477)
+      // update only the points of one colour.
+      const bool active = in_box[r] && (geo.parity < 0 || ((x + y[r] + z + geo.parity) & 1) == 0);
+      if constexpr (Plan::kReduce) {
+        if (active) acc += Plan::point(c[r], a, x, y[r], z);
+      } else {
+        if (active) Plan::point(c[r], a, x, y[r], z);
+      }
     }
   }
   if constexpr (Plan::kReduce) {
@@ -225,23 +241,24 @@ ZMarch(typename Plan::Args a, Box box, Geometry geo) {
 
 // Number of workgroups LaunchZMarch uses for a box (partials needed by a
 // reducing plan).
-template <int BX, int BY>
+template <int BX, int BY, int R = 1>
 inline int ZMarchBlocks(const Box &box, int zchunk) {
   const int ex = box.hi[0] - box.lo[0], ey = box.hi[1] - box.lo[1], ez = box.hi[2] - box.lo[2];
   if (ex <= 0 || ey <= 0 || ez <= 0) return 0;
   if (zchunk <= 0 || zchunk > ez) zchunk = ez;
-  return ((ex + BX - 1) / BX) * ((ey + BY - 1) / BY) * ((ez + zchunk - 1) / zchunk);
+  return ((ex + BX - 1) / BX) * ((ey + BY * R - 1) / (BY * R)) * ((ez + zchunk - 1) / zchunk);
 }
 
-// Host-side launch of a plan over a box with tile BX x BY.
-template <class Plan, int BX, int BY>
+// Host-side launch of a plan over a box with BX x BY threads, each
+// computing R rows of a BX x (BY * R) tile.
+template <class Plan, int BX, int BY, int R = 1>
 inline hipError_t LaunchZMarch(const typename Plan::Args &a, const Box &box, int zchunk,
                                bool remap, hipStream_t stream, int parity = -1) {
   const int ex = box.hi[0] - box.lo[0], ey = box.hi[1] - box.lo[1], ez = box.hi[2] - box.lo[2];
   if (ex <= 0 || ey <= 0 || ez <= 0) return hipSuccess;  // empty domain: nothing to run
   Geometry geo;
   geo.tiles_x = (ex + BX - 1) / BX;
-  geo.tiles_y = (ey + BY - 1) / BY;
+  geo.tiles_y = (ey + BY * R - 1) / (BY * R);
   if (zchunk <= 0 || zchunk > ez) zchunk = ez;
   geo.zchunk = zchunk;
   geo.remap = remap ? 1 : 0;
@@ -249,7 +266,7 @@ inline hipError_t LaunchZMarch(const typename Plan::Args &a, const Box &box, int
   const int nchunks = (ez + zchunk - 1) / zchunk;
   dim3 grid(geo.tiles_x * geo.tiles_y, nchunks, 1);
   dim3 block(BX, BY, 1);
-  hipLaunchKernelGGL((ZMarch<Plan, BX, BY>), grid, block, 0, stream, a, box, geo);
+  hipLaunchKernelGGL((ZMarch<Plan, BX, BY, R>), grid, block, 0, stream, a, box, geo);
   return hipGetLastError();
 }
 

@@ -121,6 +121,7 @@ struct Variant {
   int bx, by;
   int zchunk;  // 0 = whole column
   bool remap;
+  int rows = 1;  // family 1: tile rows per thread
 };
 
 const Variant kVariants[] = {
@@ -149,6 +150,10 @@ const Variant kVariants[] = {
     {"tb2_w1_by16_z32", 4, 1, 16, 32, true},
     {"tb2_w1_by8_z32", 4, 1, 8, 32, true},
     {"tb2_w4_by4_z32", 4, 4, 4, 32, true},
+    {"skeleton64x4r2_z64", 1, 64, 4, 64, true, 2},
+    {"skeleton64x4r4_z64", 1, 64, 4, 64, true, 4},
+    {"skeleton64x8r2_z64", 1, 64, 8, 64, true, 2},
+    {"skeleton64x2r4_z64", 1, 64, 2, 64, true, 4},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 // Measured on MI355X at 512^3 FP32: star4_by8_z128 701 GLUPS one step per
@@ -220,7 +225,7 @@ hipError_t LaunchOne(const Variant &v, const void *in, void *out, const int *dim
   using st = typename S::stored_t;
   if (v.family == 3 && !psk::StarSupported(dim[0])) {
     // Rows that are not whole 16-byte vectors run on the LDS-ring skeleton.
-    static const Variant fallback = {"skeleton64x4_z64", 1, 64, 4, 64, true};
+    static const Variant fallback = {"skeleton64x8r2_z64", 1, 64, 8, 64, true, 2};
     return LaunchOne<S, kPeriodic>(fallback, in, out, dim, box, k, s);
   }
   if (v.family == 1 || v.family == 3) {
@@ -241,9 +246,10 @@ hipError_t LaunchOne(const Variant &v, const void *in, void *out, const int *dim
       }
       return hipErrorInvalidValue;
     }
-    if (v.bx == 64 && v.by == 4) return psk::LaunchZMarch<P, 64, 4>(a, box, v.zchunk, v.remap, s);
-    if (v.bx == 64 && v.by == 8) return psk::LaunchZMarch<P, 64, 8>(a, box, v.zchunk, v.remap, s);
-    if (v.bx == 64 && v.by == 2) return psk::LaunchZMarch<P, 64, 2>(a, box, v.zchunk, v.remap, s);
+#define PSH_RING(BY, R) \
+    if (v.bx == 64 && v.by == BY && v.rows == R) return psk::LaunchZMarch<P, 64, BY, R>(a, box, v.zchunk, v.remap, s);
+    PSH_RING(4, 1) PSH_RING(8, 1) PSH_RING(2, 1) PSH_RING(4, 2) PSH_RING(4, 4) PSH_RING(8, 2) PSH_RING(2, 4)
+#undef PSH_RING
     return hipErrorInvalidValue;
   }
   const int ex = box.hi[0] - box.lo[0], ey = box.hi[1] - box.lo[1], ez = box.hi[2] - box.lo[2];

@@ -324,7 +324,12 @@ __global__ void __launch_bounds__(256) SumPartialsLevels(const double *partials,
 struct Variant {
   const char *name;
   int bx, by, zchunk;
+  int rows = 1;  // tile rows per thread
 };
+// Two or more rows per thread (which the generic skeleton supports) were
+// 4-9x slower for this plan on MI355X: its twelve coefficient loads per point
+// are issued in the point body, and R points' worth of them exhaust the
+// registers.
 const Variant kVariants[] = {
     {"skeleton64x4_z32", 64, 4, 32},
     {"skeleton64x8_z32", 64, 8, 32},
@@ -342,7 +347,20 @@ psk::Box InteriorBox(const __PSGrid *p0) {
 }
 
 int Blocks(const Variant &v, const psk::Box &box) {
-  return v.by == 4 ? psk::ZMarchBlocks<64, 4>(box, v.zchunk) : psk::ZMarchBlocks<64, 8>(box, v.zchunk);
+  const int tile_rows = v.by * v.rows;
+  const int ex = box.hi[0] - box.lo[0], ey = box.hi[1] - box.lo[1], ez = box.hi[2] - box.lo[2];
+  if (ex <= 0 || ey <= 0 || ez <= 0) return 0;
+  const int zc = v.zchunk <= 0 || v.zchunk > ez ? ez : v.zchunk;
+  return ((ex + 63) / 64) * ((ey + tile_rows - 1) / tile_rows) * ((ez + zc - 1) / zc);
+}
+
+template <class P>
+hipError_t LaunchVariant(const Variant &v, const typename P::Args &a, const psk::Box &box, hipStream_t s) {
+#define PSH_RING(BY, R) \
+  if (v.by == BY && v.rows == R) return psk::LaunchZMarch<P, 64, BY, R>(a, box, v.zchunk, true, s);
+  PSH_RING(4, 1) PSH_RING(8, 1)
+#undef PSH_RING
+  return hipErrorInvalidValue;
 }
 
 // Queues `sweeps` sweeps p0 -> p1 -> p0 ... on stream s.  gosa_dev (may be
@@ -368,8 +386,7 @@ int EnqueueTyped(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega
     a.partials = partials + (size_t)(i & 1) * nblocks;
     a.prev_partials = partials + (size_t)((i + 1) & 1) * nblocks;
     a.prev_out = gosa_dev && i > 0 ? gosa_dev + i - 1 : nullptr;
-    hipError_t e = v.by == 4 ? psk::LaunchZMarch<P, 64, 4>(a, box, v.zchunk, true, s)
-                             : psk::LaunchZMarch<P, 64, 8>(a, box, v.zchunk, true, s);
+    hipError_t e = LaunchVariant<P>(v, a, box, s);
     if (e != hipSuccess) return PS_HIP_ERR_DEVICE;
   }
   if (gosa_dev && nblocks > 0 && sweeps > 0) {

@@ -925,6 +925,13 @@ std::string Translator::EmitPlan(const Kernel &k) {
     << ", RZ = " << k.rz << ";\n";
   o << "  static constexpr bool kWrap = " << (k.wrap ? "true" : "false") << ";\n";
   o << "  static constexpr bool kReduce = false;\n";
+  // Grids the body touches straight in HBM (not staged); a user-type grid
+  // counts as several streams.  The launcher gives a thread two tile rows
+  // only when this is small (each row's direct loads take registers).
+  int direct = 0;
+  for (const Param &p : k.all)
+    if (p.is_grid && p.name != k.staged) direct += p.gt.utype.empty() ? 1 : 4;
+  o << "  static constexpr int kDirectGrids = " << direct << ";\n";
   o << "  struct Args {\n";
   for (const Param &p : k.all) {
     if (p.is_grid && !p.gt.utype.empty()) o << "    struct __PSURef_" << p.gt.utype << " " << p.name << ";\n";
