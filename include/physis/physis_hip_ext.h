@@ -109,8 +109,11 @@ extern const char *PSHipDiffusion7VariantName(int variant);
 
 /* Himeno 19-point Jacobi sweep (reference examples/himeno/himenobmtxpa_physis.c:331-361).
  * grids[] = {a0,a1,a2,a3,b0,b1,b2,c0,c1,c2,bnd,wrk1}; p0/p1 ping-pong.
- * Runs `sweeps` sweeps over the interior [1,n-1) per axis; gosa[i] receives
- * the FP64 sum of ss^2 of sweep i when gosa is not NULL. */
+ * Runs `sweeps` sweeps over the interior [1,n-1) per axis, one sweep per
+ * launch; gosa[i] receives the FP64 sum of ss^2 of sweep i when gosa is not
+ * NULL.  variant: 1-3 generic-skeleton tiles, 4 the one-sweep shape of the
+ * fused kernel (coefficients loaded a plane ahead), 0 = 4 for grids of up
+ * to 8M interior points, else 1. */
 extern int PSHipHimenoRun(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids,
                           float omega, int sweeps, double *gosa, int variant);
 /* The same sweeps with `degree` (1..4) of them fused per HBM pass (temporal

@@ -55,7 +55,7 @@ def run_hip(rt, p, coefs, omega, sweeps, storage=0, variant=0):
 def test_himeno_bitwise_vs_oracle(rt, shape):
     p, coefs = himeno_inputs(shape, seed=100)
     want0, want1, wgosa = O.himeno(p, coefs, 0.8, 6)
-    for variant in (1, 2, 3):
+    for variant in (0, 1, 2, 3, 4):
         print("shape", shape, "variant", variant, flush=True)
         got0, got1, gosa = run_hip(rt, p, coefs, 0.8, 6, variant=variant)
         assert np.array_equal(got0, want0) and np.array_equal(got1, want1)
