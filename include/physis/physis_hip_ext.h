@@ -129,6 +129,11 @@ extern int PSHipHimenoRun(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids,
 extern int PSHipHimenoRunTB(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids,
                             float omega, int sweeps, double *gosa, int degree,
                             int by, int rows_per_wave, int zchunk, int prefetch);
+/* The pass schedule of PSHipHimenoRunTB (host only, no device work): pass i
+ * is passes[4i..4i+3] = {first sweep index, sweeps in the pass, source,
+ * destination} with buffers 0 = p0, 1 = p1, 2 = the third buffer.  Returns
+ * the number of passes, or -1 if max_passes is too small. */
+extern int PSHipHimenoTBSchedule(int sweeps, int degree, int *passes, int max_passes);
 
 /* ---- batched many-grid diffusion (no reference counterpart) -------------
  * Grid pairs (a[i], b[i]) of any shape (nx <= 512), FP32 or BF16 storage

@@ -12,6 +12,7 @@
 #include "physis/hip/storage.h"
 
 #include <algorithm>
+#include <vector>
 #include <type_traits>
 #include <utility>
 
@@ -489,6 +490,35 @@ int TBHalf(const __PSGrid *p0, const TBConfig &c) {
 }
 int TBPartials(const __PSGrid *p0, const TBConfig &c) { return 2 * TBHalf(p0, c); }
 
+// The passes of a run of `sweeps` sweeps with degree T: pass i is
+// out[4i .. 4i+3] = {first iterate k, sweeps in the pass, source buffer,
+// destination buffer} with buffers 0 = A (p0), 1 = B (p1), 2 = the third
+// buffer.  Fused passes (T sweeps) run while at least T + 2 sweeps remain,
+// writing to the third buffer when reading A or B, else to the buffer of the
+// output's parity; every other pass is one sweep into the buffer of its
+// output's parity.  Returns the number of passes, or -1 if max is too small.
+int TBSchedule(int sweeps, int T, int *out, int max) {
+  auto parity = [](int j) { return j & 1; };
+  int cur = 0, k = 0, n = 0;
+  auto emit = [&](int t, int dst) {
+    if (n >= max) return false;
+    out[4 * n] = k;
+    out[4 * n + 1] = t;
+    out[4 * n + 2] = cur;
+    out[4 * n + 3] = dst;
+    ++n;
+    cur = dst;
+    k += t;
+    return true;
+  };
+  if (T > 1)
+    while (sweeps - k >= T + 2)
+      if (!emit(T, cur != 2 ? 2 : parity(k + T))) return -1;
+  while (k < sweeps)
+    if (!emit(1, parity(k + 1))) return -1;
+  return n;
+}
+
 // Partials of the launch before the current one, still to be summed.
 struct Pending {
   const double *partials = nullptr;
@@ -543,33 +573,25 @@ int TBPass(int storage, const TBConfig &c, int T, const void *in, void *out, __P
 // (input and output alternate between the third buffer X and the buffer of
 // the output's parity); the rest run one sweep per pass into the buffer of
 // their parity, so the last two always land in A and B.  gosa_dev[i] (may
-// be NULL) receives sweep i's FP64 sum of ss^2.
+// be NULL) receives sweep i's FP64 sum of ss^2.  The schedule itself is
+// TBSchedule (host code, exported for tests as PSHipHimenoTBSchedule).
 int TBRun(const TBConfig &c, __PSGrid *A, __PSGrid *B, void *X, __PSGrid *const *grids, float omega, int sweeps,
           double *gosa_dev, double *partials, hipStream_t s) {
   const int storage = Rec(A)->storage;
   if (Rec(A)->bytes > 0xffffffffull) return PS_HIP_ERR_UNSUPPORTED;  // 32-bit byte offsets
-  auto P = [&](int j) -> void * { return (j & 1) ? B->p0 : A->p0; };
+  if (c.T > 1 && !X) return PS_HIP_ERR_INVALID;
+  void *buf[3] = {A->p0, B->p0, X};
   const int half = TBHalf(A, c);
   Pending pend;
   int launches = 0;
-  auto part = [&]() { return partials + (size_t)(launches++ & 1) * half; };
-  void *cur = A->p0;
-  int k = 0;
-  if (c.T > 1) {
-    if (!X) return PS_HIP_ERR_INVALID;
-    while (sweeps - k >= c.T + 2) {
-      void *dst = cur != X ? X : P(k + c.T);
-      const int st = TBPass(storage, c, c.T, cur, dst, A, B, grids, omega, k, gosa_dev, part(), pend, s);
-      if (st != PS_HIP_OK) return st;
-      cur = dst;
-      k += c.T;
-    }
-  }
-  for (; k < sweeps; ++k) {
-    void *dst = P(k + 1);
-    const int st = TBPass(storage, c, 1, cur, dst, A, B, grids, omega, k, gosa_dev, part(), pend, s);
+  std::vector<int> plan(4 * (size_t)(sweeps + 1));
+  const int npass = TBSchedule(sweeps, c.T, plan.data(), (int)(plan.size() / 4));
+  if (npass < 0) return PS_HIP_ERR_INVALID;
+  for (int i = 0; i < npass; ++i) {
+    const int k = plan[4 * i], T = plan[4 * i + 1], src = plan[4 * i + 2], dst = plan[4 * i + 3];
+    double *part = partials + (size_t)(launches++ & 1) * half;
+    const int st = TBPass(storage, c, T, buf[src], buf[dst], A, B, grids, omega, k, gosa_dev, part, pend, s);
     if (st != PS_HIP_OK) return st;
-    cur = dst;
   }
   if (pend.out) {
     hipLaunchKernelGGL(SumPartialsLevels, dim3(pend.levels), dim3(256), 0, s, pend.partials, pend.blocks, pend.out);
@@ -646,6 +668,13 @@ bool SameGridSet(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids) {
 }
 
 }  // namespace
+
+// The pass schedule PSHipHimenoRunTB follows for `sweeps` sweeps of degree
+// `degree` (see TBSchedule); returns the number of passes or -1.
+extern "C" int PSHipHimenoTBSchedule(int sweeps, int degree, int *passes, int max_passes) {
+  if (sweeps < 0 || degree < 1 || !passes || max_passes < 0) return -1;
+  return TBSchedule(sweeps, degree, passes, max_passes);
+}
 
 // Himeno sweeps with `degree` sweeps fused per HBM pass (1..4), tiles of
 // 64 x `by` points (8 or 16), `rows_per_wave` tile rows per wave (1 or 2)
