@@ -83,20 +83,16 @@ inline int FloatStorage(const __PSGrid *const *grids, int n) {
 
 // Launch shape for the one-step LDS-ring skeleton by map rank.
 // parity: -1 for every point, 0/1 for the red/black points of a red-black map.
-// 3-D maps of plans with few grids read outside the ring run 64 x 8 threads
-// with two tile rows each (64 x 16 tiles): measured 35-40% faster than one
-// row per thread on MI355X for the 7-point update.  Plans with many direct
-// grids, or whose ring would not fit 64 KiB of LDS, keep one row per thread.
+// 3-D maps of large boxes may run two tile rows per thread (LaunchZMarch3D);
+// plans with many grids read outside the ring, or whose two-row ring would
+// not fit 64 KiB of LDS, keep one row per thread.
 template <class P, int RANK>
 inline hipError_t LaunchRing(const typename P::Args &a, const Box &box, hipStream_t s, int parity = -1) {
   if (RANK == 1) return LaunchZMarch<P, 64, 1>(a, box, 0, true, s, parity);
   if (RANK == 2) return LaunchZMarch<P, 64, 4>(a, box, 0, true, s, parity);
   constexpr size_t kRing2 = (size_t)(P::NS > 0 ? P::NS : 1) * (2 * P::RZ + 2) * (64 + 2 * P::RX) *
                             (16 + 2 * P::RY) * sizeof(typename P::SV);
-  if constexpr (P::kDirectGrids <= 2 && kRing2 <= 64 * 1024)
-    return LaunchZMarch<P, 64, 8, 2>(a, box, 64, true, s, parity);
-  else
-    return LaunchZMarch<P, 64, 4>(a, box, 64, true, s, parity);
+  return LaunchZMarch3D<P, (P::kDirectGrids <= 2 && kRing2 <= 64 * 1024)>(a, box, 64, true, s, parity);
 }
 
 // One step of a star plan: the vectorised skeleton when rows are whole

@@ -270,4 +270,24 @@ inline hipError_t LaunchZMarch(const typename Plan::Args &a, const Box &box, int
   return hipGetLastError();
 }
 
+// Workgroups at which a launch fills the MI355X (256 CUs, two per CU).
+constexpr int kFillWorkgroups = 512;
+
+// One step of a plan over a 3-D box: 64 x 8 threads with two tile rows each
+// when the box still yields enough of those workgroups to fill the device
+// (measured 35-40% faster than 64 x 4 threads with one row on 512^3), else
+// 64 x 4 with one row (small grids are latency-bound: more, shorter
+// workgroups win; two rows per thread made 32-256^3 grids up to 1.5x
+// slower).  kTwoRowsOk lets the caller rule the wide tile out (plans with
+// many directly read grids, or rings too large for LDS).
+template <class Plan, bool kTwoRowsOk>
+inline hipError_t LaunchZMarch3D(const typename Plan::Args &a, const Box &box, int zchunk, bool remap,
+                                 hipStream_t stream, int parity = -1) {
+  if constexpr (kTwoRowsOk) {
+    if (ZMarchBlocks<64, 8, 2>(box, zchunk) >= kFillWorkgroups)
+      return LaunchZMarch<Plan, 64, 8, 2>(a, box, zchunk, remap, stream, parity);
+  }
+  return LaunchZMarch<Plan, 64, 4>(a, box, zchunk, remap, stream, parity);
+}
+
 }  // namespace psk

@@ -224,9 +224,14 @@ hipError_t LaunchOne(const Variant &v, const void *in, void *out, const int *dim
                      const Coef7 &k, hipStream_t s) {
   using st = typename S::stored_t;
   if (v.family == 3 && !psk::StarSupported(dim[0])) {
-    // Rows that are not whole 16-byte vectors run on the LDS-ring skeleton.
-    static const Variant fallback = {"skeleton64x8r2_z64", 1, 64, 8, 64, true, 2};
-    return LaunchOne<S, kPeriodic>(fallback, in, out, dim, box, k, s);
+    // Rows that are not whole 16-byte vectors run on the LDS-ring skeleton
+    // (two rows per thread when the grid is large enough, as generated code).
+    typename DiffusionPlan<S, kPeriodic>::Args a;
+    a.g1 = static_cast<const st *>(in);
+    a.g2 = static_cast<st *>(out);
+    for (int d = 0; d < 3; ++d) a.dim[d] = dim[d];
+    a.k = k;
+    return psk::LaunchZMarch3D<DiffusionPlan<S, kPeriodic>, true>(a, box, 64, true, s);
   }
   if (v.family == 1 || v.family == 3) {
     typename DiffusionPlan<S, kPeriodic>::Args a;
