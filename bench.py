@@ -76,7 +76,7 @@ def cpu_baseline(n):
 
 
 def batched_bench(rt, n_grids=256, steps=200, seed=2024):
-    """Config 4: n_grids independent FP32 7-point grids, edges log-uniform in
+    """Batched workload: n_grids independent FP32 7-point grids, edges log-uniform in
     [32, 256] per axis (seeded), 200 steps each.  Batched (tuned launch shape,
     one hipGraph for the run) vs a sequential loop of plain single-grid runs
     (PSHipDiffusion7Run per grid) over the same grids."""
@@ -122,7 +122,7 @@ def batched_bench(rt, n_grids=256, steps=200, seed=2024):
 
 
 def served_bench(rt, n_req=24, rate=40.0, seed=7, budget_gib=48, tick=50):
-    """Config 5: Poisson arrivals at a fixed rate of mixed requests (70%
+    """Served workload: Poisson arrivals at a fixed rate of mixed requests (70%
     7-point diffusion, 30% 19-point Jacobi), cubic edges uniform in [32, 384],
     FP32 or BF16 storage per request, each run to residual <= 1e-5 or 2000
     steps; grids resident on the device, admission by HBM budget.  The same
@@ -160,7 +160,7 @@ HIMENO_FUSED_SHAPES = [(2, 8, 1, 32), (2, 16, 1, 32), (3, 16, 1, 32), (1, 8, 1, 
 
 
 def himeno_bench(rt, sweeps=400, seed=300):
-    """Config 3: 19-point Jacobi (Himeno form) on 257x129x129, coefficients
+    """Himeno workload: 19-point Jacobi on 257x129x129, coefficients
     a0..a2, b0..b2, c0..c2 uniform [0.9, 1.1], a3 uniform [0.15, 0.18], bnd 1,
     wrk1 0, p = (z/(nz-1))^2, omega 0.8, 400 sweeps with the FP64 sum of ss^2
     per sweep; FP32 and BF16 storage, one sweep per pass and the fastest

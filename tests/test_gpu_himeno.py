@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 
 def himeno_inputs(shape, seed):
-    """Config-3 inputs: coefficients uniform in [0.9, 1.1] (a3 in [0.15,
+    """Inputs of the 257x129x129 Himeno workload: coefficients uniform in [0.9, 1.1] (a3 in [0.15,
     0.18]), bnd = 1, wrk1 = 0, p = (z/(nz-1))^2."""
     nx, ny, nz = shape
     coefs = []
@@ -75,7 +75,7 @@ def test_himeno_config3_shape_400_sweeps_vs_fp64(rt):
 
 
 def test_himeno_convergent_form_400_sweeps_vs_fp64(rt):
-    """With the config-3 coefficient ranges (b-terms ~1) the relaxation
+    """With the workload's coefficient ranges (b-terms ~1) the relaxation
     diverges (|p| ~ 3e3 after 40 sweeps, ~1e63 after 400 in FP64), so 400
     sweeps are checked on the original Himeno coefficients (a=c=1, b=0,
     a3=1/6), which converge."""
