@@ -122,7 +122,7 @@ extern int PSHipHimenoRun(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids,
  * rows_per_wave tile rows per wave: (by, rows_per_wave) = (8, 1) or (8, 2)
  * for degree <= 3, (16, 1) or (16, 2) for any degree; zchunk = output
  * planes per workgroup (0: all); prefetch = march steps between a plane's
- * load and its first use (0: the shape's default; 1..3 for some shapes,
+ * load and its first use (0: the default, 1; 2..4 for some shapes,
  * PS_HIP_ERR_UNSUPPORTED otherwise).  p0,
  * p1 and gosa end exactly as after PSHipHimenoRun; degree > 1 uses a third
  * runtime-owned buffer of the grid's size; grids must be below 4 GiB. */

@@ -453,10 +453,11 @@ TBGrid TBGridFor(const int *dim, const TBConfig &c) {
 // tile (T = 1) exist for every tile the fused shapes use.
 template <class S>
 const void *TBKernel(int T, int nw, int r, int pf) {
-  if (pf == 0) pf = T == 1 ? 2 : 1;
+  if (pf == 0) pf = 1;
 #define PSH_TB(TT, NW, RR, DD) \
   if (T == TT && nw == NW && r == RR && pf == DD) return reinterpret_cast<const void *>(&HimenoTB<S, TT, NW, RR, DD>);
-  PSH_TB(1, 8, 1, 2) PSH_TB(1, 8, 1, 3) PSH_TB(1, 8, 1, 4) PSH_TB(1, 4, 2, 2) PSH_TB(1, 8, 2, 2) PSH_TB(1, 16, 1, 2)
+  PSH_TB(1, 8, 1, 1) PSH_TB(1, 8, 1, 2) PSH_TB(1, 8, 1, 3) PSH_TB(1, 8, 1, 4) PSH_TB(1, 4, 1, 1) PSH_TB(1, 4, 1, 2)
+  PSH_TB(1, 4, 2, 1) PSH_TB(1, 8, 2, 1) PSH_TB(1, 16, 1, 1)
   PSH_TB(2, 8, 1, 1) PSH_TB(2, 8, 1, 2) PSH_TB(2, 8, 1, 3) PSH_TB(2, 4, 2, 1) PSH_TB(2, 8, 2, 1) PSH_TB(2, 16, 1, 1)
   PSH_TB(3, 8, 1, 1) PSH_TB(3, 4, 2, 1) PSH_TB(3, 8, 2, 1) PSH_TB(3, 16, 1, 1) PSH_TB(3, 16, 1, 2)
   PSH_TB(4, 8, 2, 1) PSH_TB(4, 16, 1, 1)
@@ -601,10 +602,9 @@ int TBRun(const TBConfig &c, __PSGrid *A, __PSGrid *B, void *X, __PSGrid *const 
 }
 
 // One sweep per launch on the fused kernel's shape (T = 1): its coefficient
-// loads are issued a plane ahead, which measured 15-19% faster than the
-// generic skeleton on grids of up to ~8M interior points (Himeno's
-// 257x129x129, 160^3) and 3-17% slower on larger ones (236^3, 384^3), where
-// the generic skeleton's smaller tiles keep more waves resident.
+// loads are issued a plane ahead, which measured 15-40% faster than the
+// generic skeleton on grids of up to ~8M interior points (116^3, 160^3,
+// Himeno's 257x129x129) and 0-5% slower on larger ones (236^3, 384^3).
 const TBConfig kSweepShape{1, 8, 1, 16};
 constexpr int64_t kSweepKernelMaxPoints = 8ll << 20;
 

@@ -109,7 +109,7 @@ struct JacobiShape {
   int degree, by, rpw, zchunk;
 };
 // Candidates the tuner times (all compiled shapes of himeno19.hip).
-const JacobiShape kJacobiCandidates[] = {{1, 8, 1, 16}, {2, 8, 1, 32}, {2, 16, 1, 32}, {3, 16, 1, 32}};
+const JacobiShape kJacobiCandidates[] = {{1, 8, 1, 16}, {2, 8, 1, 16}, {2, 8, 1, 32}, {2, 16, 1, 32}, {3, 16, 1, 32}};
 constexpr int kTuneSweeps = 8;
 
 double Seconds(std::chrono::steady_clock::time_point t0) {

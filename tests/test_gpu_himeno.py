@@ -167,3 +167,27 @@ def test_himeno_fused_sweeps_16bit_equal_single_sweeps(rt, storage):
     for got in results:
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
         np.testing.assert_allclose(got[2], ref[2], rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("cfg", [(1, 8, 1, 16, 2), (1, 8, 1, 0, 3), (1, 8, 1, 5, 4), (1, 4, 1, 16, 2),
+                                 (2, 8, 1, 32, 2), (2, 8, 1, 3, 3), (3, 16, 1, 0, 2)])
+def test_himeno_prefetch_distances_bitwise_vs_oracle(rt, cfg):
+    """Deeper coefficient prefetch only changes when loads are issued."""
+    shape = (70, 40, 23)
+    p, coefs = himeno_inputs(shape, seed=600)
+    want0, want1, wgosa = O.himeno(p, coefs, 0.8, 9)
+    g0, g1 = rt.Grid(shape), rt.Grid(shape)
+    g0.copyin(p)
+    g1.copyin(p)
+    grids = []
+    for c in coefs:
+        g = rt.Grid(shape)
+        g.copyin(c)
+        grids.append(g)
+    print("config", cfg, flush=True)
+    gosa = rt.himeno_run_tb(g0, g1, grids, 0.8, 9, *cfg)
+    got0, got1 = g0.copyout(), g1.copyout()
+    for g in grids + [g0, g1]:
+        g.free()
+    assert np.array_equal(got0, want0) and np.array_equal(got1, want1)
+    np.testing.assert_allclose(gosa, wgosa, rtol=1e-12, atol=1e-300)
