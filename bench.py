@@ -222,6 +222,29 @@ def himeno_bench(rt, sweeps=400, seed=300):
     return out
 
 
+def reference_benchmark(n, steps):
+    """The reference's diffusion benchmark (examples/diffusion-benchmark):
+    its Physis program translated by physisc-hip and linked with its own
+    harness, run unmodified at n^3 for `steps` steps.  Its timer covers
+    PSGridCopyin, the PSStencilRun and PSGridCopyout (host buffers)."""
+    exe = os.path.join(ROOT, "tests", "_corpus_bin", "diffusion_benchmark_physis")
+    if not os.path.exists(exe):
+        return None
+    try:
+        out = subprocess.run([exe, "--size", str(n), "--count", str(steps)], capture_output=True, text=True,
+                             timeout=120, check=True).stdout
+    except (subprocess.SubprocessError, OSError) as e:
+        print("reference benchmark failed: %s" % e, file=sys.stderr)
+        return None
+    vals = dict(re.findall(r"^(\w+)[^:\n]*:\s*([0-9.e+-]+)", out, flags=re.M))
+    sec = float(vals["Elapsed"])
+    return {"workload": "reference examples/diffusion-benchmark, Physis variant, %d^3, %d steps, translated by "
+                        "physisc-hip; timer includes PSGridCopyin/Copyout" % (n, steps),
+            "elapsed_s": sec, "glups_incl_copies": round(n ** 3 * steps / sec / 1e9, 1),
+            "harness_gflops": float(vals["FLOPS"]), "harness_gbs": float(vals["Throughput"]),
+            "accuracy_rms_vs_analytic": float(vals["Accuracy"])}
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "diffusion7_512_traffic.json")
     try:
@@ -294,6 +317,8 @@ def main():
             file=sys.stderr, flush=True)
     g1.free()
     g2.free()
+    refbench = reference_benchmark(n, args.steps)
+    print("reference benchmark: %s" % json.dumps(refbench), file=sys.stderr, flush=True)
     batched = None
     if not args.no_batched:
         batched = batched_bench(rt)
@@ -329,6 +354,7 @@ def main():
                    "periodic_glups": round(n ** 3 * args.steps / results[1][0] / 1e9, 2),
                    "hbm_roofline_pct": round(100 * achieved / HBM_PEAK_GBS, 2),
                    "stream_copy_gbs": round(copy_gbs, 1),
+                   "reference_benchmark": refbench,
                    "batched": batched,
                    "himeno": himeno,
                    "served": served},

@@ -1300,9 +1300,12 @@ std::string Translator::Run(const std::vector<std::string> &includes) {
     pos = sk.second;
   }
   *dst += RewriteHost(pos, t_.size());
-  o << host_before << "\n" << generated;
+  // The generated templates and runs keep C++ linkage even where the
+  // program's kernels sit inside an extern "C" block (a C file compiled as
+  // C++ for a C++ harness, as the reference's benchmarks are).
+  o << host_before << "\nextern \"C++\" {\n" << generated;
   for (const std::string &r : runs_) o << r;
-  o << host_after << "\n";
+  o << "}  // extern \"C++\"\n" << host_after << "\n";
   return o.str();
 }
 

@@ -7,7 +7,10 @@ where tests/test_gpu_corpus.py runs them and compares their stdout with the
 SHA-256 of the reference's own manual.ref outputs in manual_ref.json).
 Programs this target does not support would be listed in SKIPPED with the
 reason; there are none.  A module test (test_*.module.c, a DSL unit without
-main) is linked with its plain C driver test_*.module_base.c.
+main) is linked with its plain C driver test_*.module_base.c.  The
+reference's diffusion benchmark (examples/diffusion-benchmark) is built the
+same way: its Physis program translated, its C++ harness compiled for the
+Physis variant and linked in (tests/_corpus_bin/diffusion_benchmark_physis).
 """
 import json
 import os
@@ -21,6 +24,25 @@ CASES = "/root/reference/tests/system_tests/test_cases"
 OUT = os.path.join(ROOT, "tests", "_corpus_bin")
 
 SKIPPED = {}
+
+# The diffusion benchmark of the reference (examples/diffusion-benchmark):
+# its Physis program, translated by physisc-hip, linked with the benchmark's
+# own C++ harness built for the Physis variant (main.cc with -DPHYSIS).
+BENCH_DIR = "/root/reference/examples/diffusion-benchmark"
+BENCH_EXE = os.path.join(OUT, "diffusion_benchmark_physis")
+
+
+def build_benchmark(phc):
+    if not os.path.isdir(BENCH_DIR):
+        return True
+    try:
+        phc.build(os.path.join(BENCH_DIR, "diffusion3d_physis.c"), BENCH_EXE,
+                  extra_cxx=[os.path.join(BENCH_DIR, f) for f in ("main.cc", "diffusion3d.cc", "baseline.cc")],
+                  cxx_flags=["-DPHYSIS", "-I" + BENCH_DIR, "-I" + os.path.join(ROOT, "include"), "-w"])
+        os.remove(BENCH_EXE + ".hip")
+        return True
+    except subprocess.CalledProcessError:
+        return False
 
 
 def main(jobs=8):
@@ -54,6 +76,7 @@ def main(jobs=8):
         results = list(pool.map(one, stems))
     built = [s for s, ok in results if ok]
     failed = [s for s, ok in results if not ok]
+    (built if build_benchmark(phc) else failed).append("diffusion_benchmark_physis")
     with open(os.path.join(OUT, "manifest.json"), "w") as f:
         json.dump({"built": built, "failed": failed}, f, indent=1)
     print("built %d, failed %d %s" % (len(built), len(failed), failed), file=sys.stderr)

@@ -68,3 +68,35 @@ def test_self_checking_program_passes(name):
     print("running", name, flush=True)
     out = subprocess.run([exe(name)], capture_output=True, timeout=120)
     assert out.returncode == 0, (out.stdout.decode()[-1000:], out.stderr.decode()[-2000:])
+
+
+def run_benchmark(path, size, count, cwd):
+    out = subprocess.run([path, "--size", str(size), "--count", str(count), "--dump"], capture_output=True,
+                         timeout=120, cwd=cwd)
+    assert out.returncode == 0, out.stderr.decode()[-2000:]
+    acc = [l for l in out.stdout.decode().splitlines() if l.startswith("Accuracy")]
+    return acc
+
+
+@pytest.mark.parametrize("size,count", [(64, 100), (48, 20), (100, 8)])
+def test_reference_diffusion_benchmark_matches_its_cpu_baseline(tmp_path, size, count):
+    """The reference's diffusion benchmark (examples/diffusion-benchmark): its
+    Physis program translated by physisc-hip and linked with the benchmark's
+    own harness (main.cc built for the Physis variant) dumps the same field,
+    to the last printed digit of every element, as the same harness built
+    for its sequential CPU baseline (diffusion3d.cc), and reports the same
+    accuracy against the analytic solution."""
+    bench = exe("diffusion_benchmark_physis")
+    base = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "diffusion_baseline.exe")
+    if not os.path.exists(base):
+        pytest.skip("the reference's CPU baseline is not built (oracle/Makefile ref)")
+    print("benchmark", size, count, flush=True)
+    d_hip, d_cpu = tmp_path / "hip", tmp_path / "cpu"
+    d_hip.mkdir()
+    d_cpu.mkdir()
+    acc_hip = run_benchmark(bench, size, count, d_hip)
+    acc_cpu = run_benchmark(base, size, count, d_cpu)
+    assert acc_hip and acc_hip == acc_cpu
+    got = (d_hip / "diffusion3d_result.physis.out").read_bytes()
+    want = (d_cpu / "diffusion3d_result.baseline.out").read_bytes()
+    assert len(got) == len(want) and got == want
