@@ -156,7 +156,7 @@ def served_bench(rt, n_req=24, rate=40.0, seed=7, budget_gib=48, tick=50):
 
 
 # (degree, tile rows, rows per wave, z chunk) of the fused Himeno kernel.
-HIMENO_FUSED_SHAPES = [(2, 8, 1, 32), (2, 16, 1, 32), (3, 16, 1, 32), (1, 8, 1, 16)]
+HIMENO_FUSED_SHAPES = [(2, 8, 1, 8), (2, 8, 1, 16), (2, 8, 1, 32), (2, 16, 1, 32), (3, 16, 1, 32)]
 
 
 def himeno_bench(rt, sweeps=400, seed=300):

@@ -41,6 +41,11 @@
 //   static void stage_dims(const Args&, int&, int&, int&);   staged grid extents
 //   static SV stage_load(const Args&, int s, int64_t off);    element of staged grid s
 //   template <class C> static void point(const C&, const Args&, int x, int y, int z);
+// and optionally, with NS == 1,
+//   using Storage;                    storage of the staged grid
+//   static const Storage::stored_t *stage_ptr(const Args&);   its elements
+// in which case the next plane is held in registers as stored elements and
+// widened only when written to LDS (see Stager).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -109,6 +114,31 @@ __device__ __forceinline__ int FixIndex(int i, int n) {
   return i < 0 ? 0 : (i >= n ? n - 1 : i);
 }
 
+// How ZMarch loads the next plane into registers.  A plan that names its
+// staged grid's Storage and pointer is staged raw: registers hold the stored
+// elements until the LDS write, where they are widened.  Widening at the
+// load instead would make the wave wait for the load right there (a 16-bit
+// element is converted by a shift of the loaded register), so the next
+// plane's latency would not overlap the current plane's compute.
+template <class Plan, class = void>
+struct Stager {
+  using raw_t = typename Plan::SV;
+  __device__ __forceinline__ static raw_t load(const typename Plan::Args &a, int s, int64_t off) {
+    return Plan::stage_load(a, s, off);
+  }
+  __device__ __forceinline__ static typename Plan::SV widen(raw_t v) { return v; }
+};
+template <class Plan>
+struct Stager<Plan, std::void_t<typename Plan::Storage, std::enable_if_t<Plan::NS == 1>,
+                                decltype(Plan::stage_ptr(std::declval<const typename Plan::Args &>()))>> {
+  using S = typename Plan::Storage;
+  using raw_t = typename S::stored_t;
+  __device__ __forceinline__ static raw_t load(const typename Plan::Args &a, int, int64_t off) {
+    return Plan::stage_ptr(a)[off];
+  }
+  __device__ __forceinline__ static typename Plan::SV widen(raw_t v) { return (typename Plan::SV)S::widen(v); }
+};
+
 // Per-point view of the staged planes handed to Plan::point.
 template <class SV, int NSLOT, int PLANE, int TW, int RZ>
 struct Ctx {
@@ -147,7 +177,8 @@ ZMarch(typename Plan::Args a, Box box, Geometry geo) {
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
   const int x = x0 + tx;
 
-  SV buf[NSA][PER];
+  using Stage = Stager<Plan>;
+  typename Stage::raw_t buf[NSA][PER];
   int goff[PER];
   int nx = 1, ny = 1, nz = 1;
   if constexpr (NS > 0) {
@@ -170,7 +201,7 @@ ZMarch(typename Plan::Args a, Box box, Geometry geo) {
       for (int s = 0; s < NS; ++s)
 #pragma unroll
         for (int j = 0; j < PER; ++j)
-          if (PLANE % NT == 0 || tid + j * NT < PLANE) buf[s][j] = Plan::stage_load(a, s, goff[j] + zo);
+          if (PLANE % NT == 0 || tid + j * NT < PLANE) buf[s][j] = Stage::load(a, s, goff[j] + zo);
     }
   };
   auto store_plane = [&](int p) {
@@ -181,7 +212,7 @@ ZMarch(typename Plan::Args a, Box box, Geometry geo) {
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
           const int e = tid + j * NT;
-          if (PLANE % NT == 0 || e < PLANE) lds[(s * NSLOT + sl) * PLANE + e] = buf[s][j];
+          if (PLANE % NT == 0 || e < PLANE) lds[(s * NSLOT + sl) * PLANE + e] = Stage::widen(buf[s][j]);
         }
     }
   };

@@ -21,9 +21,12 @@ namespace {
 
 enum { A0, A1, A2, A3, B0, B1, B2, C0, C1, C2, BND, WRK1, NCOEF };
 
-template <class S>
+// kOff32: every grid is below 4 GiB, so coefficient addresses are a uniform
+// base plus a 32-bit byte offset (one VGPR per load instead of a 64-bit add).
+template <class S, bool kOff32>
 struct HimenoPlan {
   using SV = float;
+  using Storage = S;
   using st = typename S::stored_t;
   static constexpr int NS = 1, RX = 1, RY = 1, RZ = 1;
   static constexpr bool kWrap = false;
@@ -53,10 +56,17 @@ struct HimenoPlan {
     nz = a.dim[2];
   }
   __device__ static SV stage_load(const Args &a, int, int64_t off) { return S::load(a.p + off); }
+  __host__ __device__ static const st *stage_ptr(const Args &a) { return a.p; }
   template <class C>
   __device__ static double point(const C &P, const Args &a, const int i, const int j, const int k) {
     const int64_t o = i + (int64_t)j * a.dim[0] + (int64_t)k * a.dim[0] * a.dim[1];
-    auto g = [&](int n) { return S::load(a.c[n] + o); };
+    const uint32_t ob = (uint32_t)(i + j * a.dim[0] + k * a.dim[0] * a.dim[1]) * (uint32_t)sizeof(st);
+    auto g = [&](int n) {
+      if constexpr (kOff32)
+        return S::load(reinterpret_cast<const st *>(reinterpret_cast<const char *>(a.c[n]) + ob));
+      else
+        return S::load(a.c[n] + o);
+    };
     float s0, ss;
     s0 = g(A0) * P.at(0, 0, 0, 1) + g(A1) * P.at(0, 0, 1, 0) + g(A2) * P.at(0, 1, 0, 0) +
          g(B0) * (P.at(0, 0, 1, 1) - P.at(0, 0, -1, 1) - P.at(0, 0, 1, -1) + P.at(0, 0, -1, -1)) +
@@ -65,7 +75,10 @@ struct HimenoPlan {
          g(C0) * P.at(0, 0, 0, -1) + g(C1) * P.at(0, 0, -1, 0) + g(C2) * P.at(0, -1, 0, 0) + g(WRK1);
     ss = (s0 * g(A3) - P.at(0, 0, 0, 0)) * g(BND);
     const float v = P.at(0, 0, 0, 0) + a.omega * ss;
-    S::store(a.q + o, v);
+    if constexpr (kOff32)
+      S::store(reinterpret_cast<st *>(reinterpret_cast<char *>(a.q) + ob), v);
+    else
+      S::store(a.q + o, v);
     return (double)ss * (double)ss;
   }
 };
@@ -335,6 +348,9 @@ const Variant kVariants[] = {
     {"skeleton64x4_z32", 64, 4, 32},
     {"skeleton64x8_z32", 64, 8, 32},
     {"skeleton64x4_zall", 64, 4, 0},
+    {"skeleton64x4_z16", 64, 4, 16},
+    {"skeleton64x4_z64", 64, 4, 64},
+    {"skeleton64x2_z32", 64, 2, 32},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -359,7 +375,7 @@ template <class P>
 hipError_t LaunchVariant(const Variant &v, const typename P::Args &a, const psk::Box &box, hipStream_t s) {
 #define PSH_RING(BY, R) \
   if (v.by == BY && v.rows == R) return psk::LaunchZMarch<P, 64, BY, R>(a, box, v.zchunk, true, s);
-  PSH_RING(4, 1) PSH_RING(8, 1)
+  PSH_RING(4, 1) PSH_RING(8, 1) PSH_RING(2, 1)
 #undef PSH_RING
   return hipErrorInvalidValue;
 }
@@ -367,10 +383,10 @@ hipError_t LaunchVariant(const Variant &v, const typename P::Args &a, const psk:
 // Queues `sweeps` sweeps p0 -> p1 -> p0 ... on stream s.  gosa_dev (may be
 // NULL) receives the FP64 sum of ss^2 of each sweep; partials needs
 // Blocks(v, box) doubles.
-template <class S>
-int EnqueueTyped(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
-                 double *gosa_dev, double *partials, const Variant &v, hipStream_t s) {
-  using P = HimenoPlan<S>;
+template <class S, bool kOff32>
+int EnqueuePlan(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
+                double *gosa_dev, double *partials, const Variant &v, hipStream_t s) {
+  using P = HimenoPlan<S, kOff32>;
   using st = typename S::stored_t;
   const psk::Box box = InteriorBox(p0);
   const int nblocks = Blocks(v, box);
@@ -396,6 +412,15 @@ int EnqueueTyped(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega
     if (hipGetLastError() != hipSuccess) return PS_HIP_ERR_DEVICE;
   }
   return PS_HIP_OK;
+}
+
+// The 14 grids of a run have one shape and storage, so p0's size decides.
+template <class S>
+int EnqueueTyped(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
+                 double *gosa_dev, double *partials, const Variant &v, hipStream_t s) {
+  if (Rec(p0)->bytes <= 0xffffffffull)
+    return EnqueuePlan<S, true>(p0, p1, grids, omega, sweeps, gosa_dev, partials, v, s);
+  return EnqueuePlan<S, false>(p0, p1, grids, omega, sweeps, gosa_dev, partials, v, s);
 }
 
 template <class S>
@@ -601,29 +626,41 @@ int TBRun(const TBConfig &c, __PSGrid *A, __PSGrid *B, void *X, __PSGrid *const 
   return PS_HIP_OK;
 }
 
-// One sweep per launch on the fused kernel's shape (T = 1): its coefficient
-// loads are issued a plane ahead, which measured 15-40% faster than the
-// generic skeleton on grids of up to ~8M interior points (116^3, 160^3,
-// Himeno's 257x129x129) and 0-5% slower on larger ones (236^3, 384^3).
+// One sweep per launch on the fused kernel's shape (T = 1, coefficient
+// loads issued a plane ahead): variant 4 of PSHipHimenoRun.
 const TBConfig kSweepShape{1, 8, 1, 16};
-constexpr int64_t kSweepKernelMaxPoints = 8ll << 20;
 
-bool UseSweepKernel(const __PSGrid *p0) {
-  int64_t interior = 1;
-  for (int d = 0; d < 3; ++d) interior *= std::max(p0->dim[d] - 2, 0);
-  return interior <= kSweepKernelMaxPoints && Rec(p0)->bytes <= 0xffffffffull;
+// The default one-sweep launch: tiles of 64 x 4 points and the largest
+// z-chunk from the storage's list that still gives kTileWorkgroups
+// workgroups (the smallest chunk when none does).  Timed on MI355X over
+// 48^3 .. 513x257x257 against 21 tiles (by 2/4/8 x chunks 2..32): FP32 is
+// fastest at 4 planes (2 on grids too small to fill the device), 16-bit
+// storage at 6-8 planes; the rule is within 6% of the best tile on every
+// grid and 1.05-4x faster than 32-plane chunks on grids below 257x129x129.
+constexpr int kTileWorkgroups = 2048;
+
+Variant AutoTile(const __PSGrid *p0) {
+  static const int kChunksFP32[] = {4, 2};
+  static const int kChunks16[] = {8, 4, 2};
+  const bool wide = Rec(p0)->storage == PS_HIP_STORAGE_NATIVE;
+  const int *chunks = wide ? kChunksFP32 : kChunks16;
+  const int n = wide ? 2 : 3;
+  const psk::Box box = InteriorBox(p0);
+  Variant v{"skeleton64x4_auto", 64, 4, chunks[n - 1]};
+  for (int i = 0; i < n; ++i) {
+    const Variant c{"skeleton64x4_auto", 64, 4, chunks[i]};
+    if (Blocks(c, box) >= kTileWorkgroups) return c;
+  }
+  return v;
 }
 
 }  // namespace
 
-int HimenoPartials(const __PSGrid *p0) {
-  return std::max(2 * Blocks(kVariants[0], InteriorBox(p0)), TBPartials(p0, kSweepShape));
-}
+int HimenoPartials(const __PSGrid *p0) { return 2 * Blocks(AutoTile(p0), InteriorBox(p0)); }
 
 int HimenoEnqueue(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
                   double *gosa_dev, double *partials, hipStream_t s) {
-  if (UseSweepKernel(p0)) return TBRun(kSweepShape, p0, p1, nullptr, grids, omega, sweeps, gosa_dev, partials, s);
-  const Variant &v = kVariants[0];
+  const Variant v = AutoTile(p0);
   switch (Rec(p0)->storage) {
     case PS_HIP_STORAGE_NATIVE:
       return EnqueueTyped<psk::Native<float>>(p0, p1, grids, omega, sweeps, gosa_dev, partials, v, s);
@@ -713,16 +750,42 @@ extern "C" int PSHipHimenoRunTB(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *gri
   return PS_HIP_OK;
 }
 
+// One sweep per launch on the generic skeleton with a tile of 64 x `by`
+// points (by = 2, 4 or 8) and `zchunk` planes per workgroup (0: all).
+extern "C" int PSHipHimenoRunTile(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
+                                  double *gosa, int by, int zchunk) {
+  if (!SameGridSet(p0, p1, grids) || sweeps < 0 || zchunk < 0) return PS_HIP_ERR_INVALID;
+  if (by != 2 && by != 4 && by != 8) return PS_HIP_ERR_UNSUPPORTED;
+  const Variant v{"tile", 64, by, zchunk};
+  switch (Rec(p0)->storage) {
+    case PS_HIP_STORAGE_NATIVE: return RunTyped<psk::Native<float>>(p0, p1, grids, omega, sweeps, gosa, v);
+    case PS_HIP_STORAGE_BF16: return RunTyped<psk::BF16>(p0, p1, grids, omega, sweeps, gosa, v);
+    case PS_HIP_STORAGE_FP16: return RunTyped<psk::FP16>(p0, p1, grids, omega, sweeps, gosa, v);
+  }
+  return PS_HIP_ERR_INVALID;
+}
+
 extern "C" int PSHipHimenoRun(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega,
                               int sweeps, double *gosa, int variant) {
   if (!p0 || !p1 || !grids || sweeps < 0) return PS_HIP_ERR_INVALID;
-  // Variant kNumVariants + 1: the fused kernel's one-sweep shape; variant 0
-  // picks it or variant 1 by grid size.
-  if (variant == 0) variant = SameGridSet(p0, p1, grids) && UseSweepKernel(p0) ? kNumVariants + 1 : 1;
-  if (variant == kNumVariants + 1)
+  // Ids 1-3 are kVariants[0..2], id 4 the fused kernel's one-sweep shape,
+  // ids 5.. the remaining kVariants; variant 0 is the 64 x 4 tile with the
+  // z-chunk of AutoTile.
+  constexpr int kSweepId = 4;
+  if (variant == 0) {
+    if (!SameGridSet(p0, p1, grids) || sweeps < 0) return PS_HIP_ERR_INVALID;
+    const Variant v = AutoTile(p0);
+    switch (Rec(p0)->storage) {
+      case PS_HIP_STORAGE_NATIVE: return RunTyped<psk::Native<float>>(p0, p1, grids, omega, sweeps, gosa, v);
+      case PS_HIP_STORAGE_BF16: return RunTyped<psk::BF16>(p0, p1, grids, omega, sweeps, gosa, v);
+      case PS_HIP_STORAGE_FP16: return RunTyped<psk::FP16>(p0, p1, grids, omega, sweeps, gosa, v);
+    }
+    return PS_HIP_ERR_INVALID;
+  }
+  if (variant == kSweepId)
     return PSHipHimenoRunTB(p0, p1, grids, omega, sweeps, gosa, kSweepShape.T, kSweepShape.by, kSweepShape.rpw,
                             kSweepShape.zchunk, 0);
-  if (variant < 1 || variant > kNumVariants) return PS_HIP_ERR_INVALID;
+  if (variant < 1 || variant > kNumVariants + 1) return PS_HIP_ERR_INVALID;
   const int storage = Rec(p0)->storage;
   auto same = [&](const __PSGrid *g) {
     if (!g || g->num_dims != 3 || g->elm_size != 4 || !g->p0 || Rec(g)->storage != storage) return false;
@@ -733,7 +796,7 @@ extern "C" int PSHipHimenoRun(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids
   if (!same(p0) || !same(p1)) return PS_HIP_ERR_INVALID;
   for (int n = 0; n < NCOEF; ++n)
     if (!same(grids[n])) return PS_HIP_ERR_INVALID;
-  const Variant &v = kVariants[variant - 1];
+  const Variant &v = kVariants[variant < kSweepId ? variant - 1 : variant - 2];
   switch (storage) {
     case PS_HIP_STORAGE_NATIVE: return RunTyped<psk::Native<float>>(p0, p1, grids, omega, sweeps, gosa, v);
     case PS_HIP_STORAGE_BF16: return RunTyped<psk::BF16>(p0, p1, grids, omega, sweeps, gosa, v);

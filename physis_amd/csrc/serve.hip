@@ -103,14 +103,22 @@ const float kDiffCoef[7] = {0.1f, 0.1f, 0.1f, 0.1f, 0.1f, 0.1f, 0.4f};
 constexpr float kOmega = 0.8f;
 constexpr int kSideStreams = 4;
 
-// A launch shape of the fused Jacobi kernel: degree, tile rows, rows per
-// wave, z chunk.
+// A launch shape of the Jacobi sweeps: degree 0 is one sweep per launch on
+// the default tile (HimenoEnqueue); degree >= 1 the fused kernel with that
+// many sweeps per pass, tile rows, rows per wave and z chunk.
 struct JacobiShape {
   int degree, by, rpw, zchunk;
 };
 // Candidates the tuner times (all compiled shapes of himeno19.hip).
-const JacobiShape kJacobiCandidates[] = {{1, 8, 1, 16}, {2, 8, 1, 16}, {2, 8, 1, 32}, {2, 16, 1, 32}, {3, 16, 1, 32}};
+const JacobiShape kJacobiCandidates[] = {{0, 0, 0, 0},    {1, 8, 1, 16},  {2, 8, 1, 8},
+                                         {2, 8, 1, 16},   {2, 8, 1, 32},  {2, 16, 1, 32}, {3, 16, 1, 32}};
 constexpr int kTuneSweeps = 8;
+
+int EnqueueJacobi(const JacobiShape &c, __PSGrid *A, __PSGrid *B, void *aux, __PSGrid *const *coef, int sweeps,
+                  double *gosa, double *partials, hipStream_t s) {
+  if (c.degree == 0) return HimenoEnqueue(A, B, coef, kOmega, sweeps, gosa, partials, s);
+  return HimenoEnqueueTB(A, B, aux, coef, kOmega, sweeps, c.degree, c.by, c.rpw, c.zchunk, gosa, partials, s);
+}
 
 double Seconds(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -219,11 +227,9 @@ JacobiShape Server::TuneJacobi(Live &lv, hipStream_t s) {
   JacobiShape best = kJacobiCandidates[0];
   float best_ms = 1e30f;
   for (const JacobiShape &c : kJacobiCandidates) {
-    double *gosa = lv.dev, *partials = lv.dev + tick_;
+    double *gosa = lv.dev, *partials = lv.dev + std::max(tick_, kTuneSweeps);
     CheckHip(hipEventRecord(e0, s), "record");
-    if (HimenoEnqueueTB(lv.A, lv.B, lv.aux, lv.coef, kOmega, kTuneSweeps, c.degree, c.by, c.rpw, c.zchunk, gosa,
-                        partials, s) != PS_HIP_OK)
-      continue;
+    if (EnqueueJacobi(c, lv.A, lv.B, lv.aux, lv.coef, kTuneSweeps, gosa, partials, s) != PS_HIP_OK) continue;
     CheckHip(hipEventRecord(e1, s), "record");
     CheckHip(hipEventSynchronize(e1), "sync");
     float ms = 0.f;
@@ -260,7 +266,7 @@ int Server::Admit(int idx, double now, hipStream_t s, bool batched) {
     if (st == PS_HIP_OK) {
       int parts = HimenoPartials(lv.A);
       for (const JacobiShape &c : kJacobiCandidates)
-        parts = std::max(parts, HimenoTBPartials(lv.A, c.degree, c.by, c.rpw, c.zchunk));
+        if (c.degree > 0) parts = std::max(parts, HimenoTBPartials(lv.A, c.degree, c.by, c.rpw, c.zchunk));
       lv.dev_doubles = std::max(tick_, kTuneSweeps) + parts + (int)ReduceScratchDoubles() + 1;
     }
   }
@@ -392,9 +398,7 @@ void Server::AdvanceTick(std::vector<Live *> &act, bool batched) {
         CheckHip(hipStreamWaitEvent(js, ready_, 0), "wait");
         used.push_back(js);
       }
-      const JacobiShape &c = lv->shape;
-      if (HimenoEnqueueTB(lv->A, lv->B, lv->aux, lv->coef, kOmega, tick_, c.degree, c.by, c.rpw, c.zchunk, gosa,
-                          partials, js) != PS_HIP_OK)
+      if (EnqueueJacobi(lv->shape, lv->A, lv->B, lv->aux, lv->coef, tick_, gosa, partials, js) != PS_HIP_OK)
         Fatal("serve", "jacobi run failed");
     } else if (HimenoEnqueue(lv->A, lv->B, lv->coef, kOmega, tick_, gosa, partials, js) != PS_HIP_OK) {
       Fatal("serve", "jacobi run failed");

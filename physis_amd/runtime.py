@@ -233,6 +233,20 @@ def himeno_run(p0, p1, coef_grids, omega, sweeps, with_gosa=True, variant=0):
     return np.array(gosa[:sweeps], np.float64)
 
 
+def himeno_run_tile(p0, p1, coef_grids, omega, sweeps, by, zchunk, with_gosa=True):
+    """Himeno sweeps, one per launch, on the generic skeleton with tiles of
+    64 x by points (by = 2, 4 or 8) and zchunk planes per workgroup (0 =
+    all).  Returns the FP64 sum of ss^2 per sweep."""
+    L = lib()
+    L.PSHipHimenoRunTile.argtypes = [_GP, _GP, ctypes.POINTER(_GP), ctypes.c_float, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int]
+    arr = (_GP * 12)(*[g.handle for g in coef_grids])
+    gosa = (ctypes.c_double * max(sweeps, 1))()
+    check(L.PSHipHimenoRunTile(p0.handle, p1.handle, arr, omega, sweeps, gosa if with_gosa else None, by, zchunk),
+          "PSHipHimenoRunTile")
+    return np.array(gosa[:sweeps], np.float64)
+
+
 def himeno_run_tb(p0, p1, coef_grids, omega, sweeps, degree, by=16, rows_per_wave=1, zchunk=0, prefetch=0,
                   with_gosa=True):
     """Himeno sweeps with `degree` sweeps fused per HBM pass (tiles of 64 x by

@@ -55,7 +55,7 @@ def run_hip(rt, p, coefs, omega, sweeps, storage=0, variant=0):
 def test_himeno_bitwise_vs_oracle(rt, shape):
     p, coefs = himeno_inputs(shape, seed=100)
     want0, want1, wgosa = O.himeno(p, coefs, 0.8, 6)
-    for variant in (0, 1, 2, 3, 4):
+    for variant in range(8):
         print("shape", shape, "variant", variant, flush=True)
         got0, got1, gosa = run_hip(rt, p, coefs, 0.8, 6, variant=variant)
         assert np.array_equal(got0, want0) and np.array_equal(got1, want1)
@@ -191,3 +191,37 @@ def test_himeno_prefetch_distances_bitwise_vs_oracle(rt, cfg):
         g.free()
     assert np.array_equal(got0, want0) and np.array_equal(got1, want1)
     np.testing.assert_allclose(gosa, wgosa, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("tile", [(2, 1), (2, 8), (4, 3), (4, 8), (4, 0), (8, 2), (8, 16)])
+@pytest.mark.parametrize("storage", [0, 1])
+def test_himeno_tile_shapes_vs_oracle(rt, tile, storage):
+    """Every tile height and z-chunk of the generic skeleton gives the same
+    fields: bitwise for FP32, equal to variant 1 for BF16 storage."""
+    shape = (70, 21, 37)
+    by, zchunk = tile
+    p, coefs = himeno_inputs(shape, seed=700)
+    g0, g1 = rt.Grid(shape, storage=storage), rt.Grid(shape, storage=storage)
+    grids = []
+    for c in coefs:
+        g = rt.Grid(shape, storage=storage)
+        g.copyin(c)
+        grids.append(g)
+    outs = []
+    for mode in ("tile", "variant1"):
+        g0.copyin(p)
+        g1.copyin(p)
+        print("tile", tile, "storage", storage, mode, flush=True)
+        if mode == "tile":
+            gosa = rt.himeno_run_tile(g0, g1, grids, 0.8, 7, by, zchunk)
+        else:
+            gosa = rt.himeno_run(g0, g1, grids, 0.8, 7, variant=1)
+        outs.append((g0.copyout(), g1.copyout(), gosa))
+    for g in grids + [g0, g1]:
+        g.free()
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    np.testing.assert_allclose(outs[0][2], outs[1][2], rtol=1e-12, atol=1e-300)
+    if storage == 0:
+        want0, want1, wgosa = O.himeno(p, coefs, 0.8, 7)
+        assert np.array_equal(outs[0][0], want0) and np.array_equal(outs[0][1], want1)
+        np.testing.assert_allclose(outs[0][2], wgosa, rtol=1e-12, atol=1e-300)
