@@ -111,16 +111,18 @@ extern const char *PSHipDiffusion7VariantName(int variant);
  * grids[] = {a0,a1,a2,a3,b0,b1,b2,c0,c1,c2,bnd,wrk1}; p0/p1 ping-pong.
  * Runs `sweeps` sweeps over the interior [1,n-1) per axis, one sweep per
  * launch; gosa[i] receives the FP64 sum of ss^2 of sweep i when gosa is not
- * NULL.  variant: 0 = the default tile (64 x 4 points; z-chunk of 4 or 2
- * planes for FP32, 8, 4 or 2 for 16-bit storage: the largest that still
- * gives 2048 workgroups), 1-3 and 5-7 fixed generic-skeleton tiles, 4 the
+ * NULL.  variant: 0 = the default tile (256 points, 256/128/64 wide: the
+ * widest that leaves at most 10% of a row's lanes idle; z-chunk of 4 or 2
+ * planes for FP32, 16, 8, 4 or 2 for 16-bit storage: the largest that still
+ * gives 1024 workgroups), 1-3 and 5-7 fixed generic-skeleton tiles, 4 the
  * one-sweep shape of the fused kernel (coefficients loaded a plane ahead). */
 extern int PSHipHimenoRun(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids,
                           float omega, int sweeps, double *gosa, int variant);
-/* The same sweeps on a generic-skeleton tile of 64 x by points (by = 2, 4
- * or 8) with zchunk planes per workgroup (0: all); for tuning. */
+/* The same sweeps on a generic-skeleton tile of bx x by points (64 x 2/4/8,
+ * 128 x 2/4, 256 x 1/2) with zchunk planes per workgroup (0: all); for
+ * tuning.  PS_HIP_ERR_UNSUPPORTED for other tiles. */
 extern int PSHipHimenoRunTile(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids,
-                              float omega, int sweeps, double *gosa, int by, int zchunk);
+                              float omega, int sweeps, double *gosa, int bx, int by, int zchunk);
 /* The same sweeps with `degree` (1..4) of them fused per HBM pass (temporal
  * blocking; no reference counterpart): each coefficient element is loaded
  * once per pass instead of once per sweep.  Tiles of 64 x by points with

@@ -368,14 +368,15 @@ int Blocks(const Variant &v, const psk::Box &box) {
   const int ex = box.hi[0] - box.lo[0], ey = box.hi[1] - box.lo[1], ez = box.hi[2] - box.lo[2];
   if (ex <= 0 || ey <= 0 || ez <= 0) return 0;
   const int zc = v.zchunk <= 0 || v.zchunk > ez ? ez : v.zchunk;
-  return ((ex + 63) / 64) * ((ey + tile_rows - 1) / tile_rows) * ((ez + zc - 1) / zc);
+  return ((ex + v.bx - 1) / v.bx) * ((ey + tile_rows - 1) / tile_rows) * ((ez + zc - 1) / zc);
 }
 
 template <class P>
 hipError_t LaunchVariant(const Variant &v, const typename P::Args &a, const psk::Box &box, hipStream_t s) {
-#define PSH_RING(BY, R) \
-  if (v.by == BY && v.rows == R) return psk::LaunchZMarch<P, 64, BY, R>(a, box, v.zchunk, true, s);
-  PSH_RING(4, 1) PSH_RING(8, 1) PSH_RING(2, 1)
+#define PSH_RING(BX, BY) \
+  if (v.bx == BX && v.by == BY && v.rows == 1) return psk::LaunchZMarch<P, BX, BY, 1>(a, box, v.zchunk, true, s);
+  PSH_RING(64, 4) PSH_RING(64, 8) PSH_RING(64, 2) PSH_RING(128, 2) PSH_RING(128, 4) PSH_RING(256, 1)
+  PSH_RING(256, 2)
 #undef PSH_RING
   return hipErrorInvalidValue;
 }
@@ -630,28 +631,40 @@ int TBRun(const TBConfig &c, __PSGrid *A, __PSGrid *B, void *X, __PSGrid *const 
 // loads issued a plane ahead): variant 4 of PSHipHimenoRun.
 const TBConfig kSweepShape{1, 8, 1, 16};
 
-// The default one-sweep launch: tiles of 64 x 4 points and the largest
-// z-chunk from the storage's list that still gives kTileWorkgroups
-// workgroups (the smallest chunk when none does).  Timed on MI355X over
-// 48^3 .. 513x257x257 against 21 tiles (by 2/4/8 x chunks 2..32): FP32 is
-// fastest at 4 planes (2 on grids too small to fill the device), 16-bit
-// storage at 6-8 planes; the rule is within 6% of the best tile on every
-// grid and 1.05-4x faster than 32-plane chunks on grids below 257x129x129.
-constexpr int kTileWorkgroups = 2048;
+// The default one-sweep launch: a tile of 256 points whose width is the
+// widest of 256, 128 and 64 that covers the interior row with at most 10%
+// of its lanes idle (wider tiles read longer runs of each of the 14 rows
+// per step), and the largest z-chunk from the storage's list that still
+// gives kTileWorkgroups workgroups (the smallest chunk when none does).
+// Timed on MI355X over 48^3 .. 513x257x257 against 35 tiles (64 x 2/4/8,
+// 128 x 2/4, 256 x 1/2, chunks 2..32): FP32 is fastest at 2-4 planes,
+// 16-bit storage at 8-16; the rule is within 5% of the best tile on every
+// grid, 4-15% faster than 64 x 4 tiles on 256^3-513x257x257 and 1.05-4x
+// faster than 32-plane chunks on grids below 257x129x129.
+constexpr int kTileWorkgroups = 1024;
 
 Variant AutoTile(const __PSGrid *p0) {
   static const int kChunksFP32[] = {4, 2};
-  static const int kChunks16[] = {8, 4, 2};
+  static const int kChunks16[] = {16, 8, 4, 2};
   const bool wide = Rec(p0)->storage == PS_HIP_STORAGE_NATIVE;
   const int *chunks = wide ? kChunksFP32 : kChunks16;
-  const int n = wide ? 2 : 3;
+  const int n = wide ? 2 : 4;
   const psk::Box box = InteriorBox(p0);
-  Variant v{"skeleton64x4_auto", 64, 4, chunks[n - 1]};
+  const int ex = std::max(box.hi[0] - box.lo[0], 1);
+  int bx = 64;
+  for (int w : {256, 128}) {
+    const int covered = (ex + w - 1) / w * w;
+    if (10 * ex >= 9 * covered) {
+      bx = w;
+      break;
+    }
+  }
+  const int by = 256 / bx;
   for (int i = 0; i < n; ++i) {
-    const Variant c{"skeleton64x4_auto", 64, 4, chunks[i]};
+    const Variant c{"skeleton_auto", bx, by, chunks[i]};
     if (Blocks(c, box) >= kTileWorkgroups) return c;
   }
-  return v;
+  return Variant{"skeleton_auto", bx, by, chunks[n - 1]};
 }
 
 }  // namespace
@@ -750,13 +763,16 @@ extern "C" int PSHipHimenoRunTB(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *gri
   return PS_HIP_OK;
 }
 
-// One sweep per launch on the generic skeleton with a tile of 64 x `by`
-// points (by = 2, 4 or 8) and `zchunk` planes per workgroup (0: all).
+// One sweep per launch on the generic skeleton with a tile of bx x by
+// points (64 x 2/4/8, 128 x 2/4, 256 x 1/2) and `zchunk` planes per
+// workgroup (0: all).
 extern "C" int PSHipHimenoRunTile(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
-                                  double *gosa, int by, int zchunk) {
+                                  double *gosa, int bx, int by, int zchunk) {
   if (!SameGridSet(p0, p1, grids) || sweeps < 0 || zchunk < 0) return PS_HIP_ERR_INVALID;
-  if (by != 2 && by != 4 && by != 8) return PS_HIP_ERR_UNSUPPORTED;
-  const Variant v{"tile", 64, by, zchunk};
+  const bool ok = (bx == 64 && (by == 2 || by == 4 || by == 8)) || (bx == 128 && (by == 2 || by == 4)) ||
+                  (bx == 256 && (by == 1 || by == 2));
+  if (!ok) return PS_HIP_ERR_UNSUPPORTED;
+  const Variant v{"tile", bx, by, zchunk};
   switch (Rec(p0)->storage) {
     case PS_HIP_STORAGE_NATIVE: return RunTyped<psk::Native<float>>(p0, p1, grids, omega, sweeps, gosa, v);
     case PS_HIP_STORAGE_BF16: return RunTyped<psk::BF16>(p0, p1, grids, omega, sweeps, gosa, v);
@@ -769,8 +785,7 @@ extern "C" int PSHipHimenoRun(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids
                               int sweeps, double *gosa, int variant) {
   if (!p0 || !p1 || !grids || sweeps < 0) return PS_HIP_ERR_INVALID;
   // Ids 1-3 are kVariants[0..2], id 4 the fused kernel's one-sweep shape,
-  // ids 5.. the remaining kVariants; variant 0 is the 64 x 4 tile with the
-  // z-chunk of AutoTile.
+  // ids 5.. the remaining kVariants; variant 0 is the tile of AutoTile.
   constexpr int kSweepId = 4;
   if (variant == 0) {
     if (!SameGridSet(p0, p1, grids) || sweeps < 0) return PS_HIP_ERR_INVALID;

@@ -233,17 +233,17 @@ def himeno_run(p0, p1, coef_grids, omega, sweeps, with_gosa=True, variant=0):
     return np.array(gosa[:sweeps], np.float64)
 
 
-def himeno_run_tile(p0, p1, coef_grids, omega, sweeps, by, zchunk, with_gosa=True):
+def himeno_run_tile(p0, p1, coef_grids, omega, sweeps, bx, by, zchunk, with_gosa=True):
     """Himeno sweeps, one per launch, on the generic skeleton with tiles of
-    64 x by points (by = 2, 4 or 8) and zchunk planes per workgroup (0 =
-    all).  Returns the FP64 sum of ss^2 per sweep."""
+    bx x by points (64 x 2/4/8, 128 x 2/4, 256 x 1/2) and zchunk planes per
+    workgroup (0 = all).  Returns the FP64 sum of ss^2 per sweep."""
     L = lib()
     L.PSHipHimenoRunTile.argtypes = [_GP, _GP, ctypes.POINTER(_GP), ctypes.c_float, ctypes.c_int,
-                                     ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int]
+                                     ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int, ctypes.c_int]
     arr = (_GP * 12)(*[g.handle for g in coef_grids])
     gosa = (ctypes.c_double * max(sweeps, 1))()
-    check(L.PSHipHimenoRunTile(p0.handle, p1.handle, arr, omega, sweeps, gosa if with_gosa else None, by, zchunk),
-          "PSHipHimenoRunTile")
+    check(L.PSHipHimenoRunTile(p0.handle, p1.handle, arr, omega, sweeps, gosa if with_gosa else None, bx, by,
+                               zchunk), "PSHipHimenoRunTile")
     return np.array(gosa[:sweeps], np.float64)
 
 
