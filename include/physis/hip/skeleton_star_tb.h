@@ -15,21 +15,47 @@
 //   (double-buffered by step parity: one barrier per step), then
 //   u1(k)   = body(u0 planes k-1, k, k+1)          registers + LDS rows
 //   u2(k-1) = body(u1 planes k-2, k-1, k)          registers + LDS rows
-// Rows have four pad floats at each end.  Periodic plans fill the pads with
-// the wrapped row ends; for clamped plans the pads are never selected by the
-// body (it reads the point itself at a clamped face).  Intermediate points
-// outside the grid are evaluated at their wrapped/clamped coordinates.
-// Each u1 value is computed by the same body in FP32 as in the one-step
-// path, so the fused result is bitwise equal to two separate steps.
+// Rows have four pad floats at each end: periodic plans fill them with the
+// wrapped row ends, clamped plans with the replicated end points.  Every
+// neighbour a u1 value reads (and every x neighbour a u2 value reads) is
+// then the element at the wrapped/clamped coordinate, so a plan that sets
+// kExactNeighbours may skip its clamped-face selects there (compute_exact):
+// at a clamped face the DSL body selects the point itself, which is the same
+// element.  Intermediate points outside the grid are evaluated at their
+// wrapped/clamped coordinates but are not copies of in-grid ones, so u2
+// must not read them as neighbours: at a clamped y face it reads its own row
+// instead (a wave-uniform LDS row choice), and at a z face it keeps the
+// plan's select.  Each u1 value is computed by the same
+// body in FP32 as in the one-step path, so the fused result is bitwise equal
+// to two separate steps.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "physis/hip/skeleton.h"
 #include "physis/hip/skeleton_star.h"
 
 namespace psk {
+
+// Whether Plan provides compute_exact<kX, kY, kZ>: compute() for a point
+// whose x (kX), y (kY) and z (kZ) neighbours are the elements at the clamped
+// or wrapped coordinates.
+template <class Plan, class = void>
+struct ExactNeighbours : std::false_type {};
+template <class Plan>
+struct ExactNeighbours<Plan, std::void_t<decltype(Plan::kExactNeighbours)>>
+    : std::integral_constant<bool, Plan::kExactNeighbours> {};
+
+template <class Plan, bool kX, bool kY, bool kZ, class C>
+__device__ __forceinline__ float ComputeAt(const C &c, const typename Plan::Args &a, int x, int y, int z) {
+  if constexpr (ExactNeighbours<Plan>::value)
+    return Plan::template compute_exact<kX, kY, kZ>(c, a, x, y, z);
+  else
+    return Plan::compute(c, a, x, y, z);
+}
 
 struct TBGeometry {
   int tiles_y;  // y tiles, each writing BY-2 rows
@@ -67,9 +93,11 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
   const int64_t hrow_lo = (int64_t)FixIndex<W>(ya - 1, ny) * nx + xs;
   const int64_t hrow_hi = (int64_t)FixIndex<W>(ya + BY, ny) * nx + xs;
   const bool out_row = wy >= 1 && wy <= BY - 2 && y < ny;
-  // Pad writers: the lane holding the row's last four points fills the left
-  // pad, the lane holding the first four the right pad.
-  const bool pad_l = W && xs == nx - 4, pad_r = W && xs == 0;
+  // Pad writers (the pad element next to each row end).  Periodic: the lane
+  // holding x = nx-1 fills the left pad, the lane holding x = 0 the right
+  // pad.  Clamped: the lane holding x = 0 fills the left pad, the lane
+  // holding x = nx-1 the right pad.
+  const bool first = xs == 0, last = xs == nx - 4;
 
   auto zoff = [&](int z) { return (int64_t)FixIndex<W>(z, nz) * plane; };
   auto ld = [&](int64_t off, float (&v)[4]) {
@@ -82,8 +110,14 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
   auto put = [&](float *r, const float (&v)[4]) {
     if (!has) return;
     *reinterpret_cast<v4f_t *>(r + xs + 4) = v4f_t{v[0], v[1], v[2], v[3]};
-    if (pad_l) *reinterpret_cast<v4f_t *>(r) = v4f_t{v[0], v[1], v[2], v[3]};
-    if (pad_r) *reinterpret_cast<v4f_t *>(r + nx + 4) = v4f_t{v[0], v[1], v[2], v[3]};
+    // Only the pad element next to the row is ever read.
+    if (W) {
+      if (last) r[3] = v[3];
+      if (first) r[nx + 4] = v[0];
+    } else {
+      if (first) r[3] = v[0];
+      if (last) r[nx + 4] = v[3];
+    }
   };
   auto get4 = [&](const float *r, float (&v)[4]) {
     const v4f_t q = *reinterpret_cast<const v4f_t *>(r + xs + 4);
@@ -97,6 +131,11 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
   ld(zoff(zb) + row, t0);
   if (halo_lo) ld(zoff(zb - 1) + hrow_lo, hlo);
   if (halo_hi) ld(zoff(zb - 1) + hrow_hi, hhi);
+  // Wrapped/clamped plane indices of k, k + 1 and k + 2, advanced by one plane
+  // per step (an integer modulo per load would cost ~25 VALU instructions on
+  // this target, which has no scalar division).
+  int z0 = FixIndex<W>(zb - 1, nz), z1 = FixIndex<W>(zb, nz), z2 = FixIndex<W>(zb + 1, nz);
+  auto advance = [&](int z) { return W ? (z + 1 == nz ? 0 : z + 1) : min(z + 1, nz - 1); };
 
   for (int k = zb - 1; k <= ze; ++k) {
     const int buf = (k + 2) & 1;
@@ -105,9 +144,10 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
     if (halo_hi) put(&L0[buf][BY + 1][0], hhi);
     put(&L1[buf][wy][0], c1);
     if (k < ze) {
-      ld(zoff(k + 2) + row, p0);
-      if (halo_lo) ld(zoff(k + 1) + hrow_lo, nlo);
-      if (halo_hi) ld(zoff(k + 1) + hrow_hi, nhi);
+      const int64_t o2 = (int64_t)z2 * plane, o1 = (int64_t)z1 * plane;
+      ld(o2 + row, p0);
+      if (halo_lo) ld(o1 + hrow_lo, nlo);
+      if (halo_hi) ld(o1 + hrow_hi, nhi);
     }
     __syncthreads();
     if (has) {
@@ -116,7 +156,7 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
       get4(&L0[buf][wy][0], nv);
       get4(&L0[buf][wy + 2][0], sv);
       const float wl = L0[buf][wy + 1][xs + 3], er = L0[buf][wy + 1][xs + 8];
-      const int kf = FixIndex<W>(k, nz);
+      const int kf = z0;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         StarCtx c;
@@ -127,13 +167,17 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
         c.s = sv[j];
         c.b = b0[j];
         c.t = t0[j];
-        t1[j] = RoundTrip<S>(Plan::compute(c, a1, xs + j, yf, kf));
+        t1[j] = RoundTrip<S>(ComputeAt<Plan, true, true, true>(c, a1, xs + j, yf, kf));
       }
       // u2 at plane k-1 on the output rows.
       if (out_row && k - 1 >= zb) {
         float n1[4], s1[4], v[4];
-        get4(&L1[buf][wy - 1][0], n1);
-        get4(&L1[buf][wy + 1][0], s1);
+        // Clamped plans: at a y face the row's own u1 values stand in for
+        // the missing neighbour row (a wave-uniform choice of LDS row), so
+        // only the z faces need selects in u2.
+        const int rn = !W && y == 0 ? wy : wy - 1, rs = !W && y == ny - 1 ? wy : wy + 1;
+        get4(&L1[buf][rn][0], n1);
+        get4(&L1[buf][rs][0], s1);
         const float w1 = L1[buf][wy][xs + 3], e1 = L1[buf][wy][xs + 8];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -145,7 +189,7 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
           c.s = s1[j];
           c.b = b1[j];
           c.t = t1[j];
-          v[j] = Plan::compute(c, a2, xs + j, y, k - 1);
+          v[j] = ComputeAt<Plan, true, true, false>(c, a2, xs + j, y, k - 1);
         }
         Store4<S>(out + (int64_t)(k - 1) * plane + (int64_t)y * nx + xs, v);
       }
@@ -160,6 +204,9 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
       b1[j] = c1[j];
       c1[j] = t1[j];
     }
+    z0 = z1;
+    z1 = z2;
+    z2 = advance(z2);
   }
 }
 
