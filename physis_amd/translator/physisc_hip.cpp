@@ -558,6 +558,9 @@ struct Kernel {
   bool wrap = false;
   bool star = false;    // vectorised star plan (compute() available)
   std::string out_grid; // the single emitted grid when star
+  // Locals assigned once from PSGridDim(staged grid, d): name -> d.
+  std::map<std::string, int> dimvar;
+  int clamp_ifs = 0;    // clamped-face reads of the staged grid (ClampIf)
 };
 
 // Parses "x", "x + 3", "x - 1", "2 + x", "(x-1)" against index name `name`.
@@ -601,7 +604,8 @@ class Translator {
  private:
   void CollectGridVars();
   void AnalyzeKernel(Kernel &k);
-  std::string RewriteBody(const Kernel &k, size_t b, size_t e, bool compute_form);
+  std::string RewriteBody(const Kernel &k, size_t b, size_t e, bool compute_form, bool exact = false);
+  int ClampIf(const Kernel &k, size_t i, size_t e) const;
   std::string EmitPlan(const Kernel &k);
   std::string EmitMapAndLaunch(const Kernel &k);
   std::string RewriteHost(size_t b, size_t e);
@@ -788,9 +792,93 @@ void Translator::AnalyzeKernel(Kernel &k) {
       k.out_grid = *k.emitted.begin();
     }
   }
+  if (k.star && !k.wrap) {
+    // Locals that hold a dimension of the staged grid: "n = PSGridDim(g, d);"
+    // as the only assignment to n.
+    // Writes to a local: any assignment operator or ++/-- next to it, or its
+    // address taken.
+    auto is_assign = [](const std::string &op) {
+      return op == "++" || op == "--" ||
+             (!op.empty() && op.back() == '=' && op != "==" && op != "<=" && op != ">=" && op != "!=");
+    };
+    std::map<std::string, int> assigned;
+    for (size_t i = f.lbrace + 1; i < f.rbrace; ++i) {
+      if (t_[i].kind != Tok::kIdent) continue;
+      const bool after = i + 1 < f.rbrace && is_assign(t_[i + 1].s);
+      const bool before = i > f.lbrace + 1 && (t_[i - 1].s == "++" || t_[i - 1].s == "--" || t_[i - 1].s == "&");
+      if (after || before) ++assigned[t_[i].s];
+    }
+    for (size_t i = f.lbrace + 1; i + 9 < f.rbrace; ++i) {
+      if (t_[i].kind == Tok::kIdent && t_[i + 1].s == "=" && t_[i + 2].s == "PSGridDim" && t_[i + 3].s == "(" &&
+          t_[i + 4].s == k.staged && t_[i + 5].s == "," && t_[i + 6].kind == Tok::kNum && t_[i + 7].s == ")" &&
+          t_[i + 8].s == ";" && assigned[t_[i].s] == 1) {
+        const int d = atoi(t_[i + 6].s.c_str());
+        if (d >= 0 && d < 3) k.dimvar[t_[i].s] = d;
+      }
+    }
+    for (size_t i = f.lbrace + 1; i < f.rbrace; ++i)
+      if (t_[i].s == "if" && ClampIf(k, i, f.rbrace) >= 0) ++k.clamp_ifs;
+  }
 }
 
-std::string Translator::RewriteBody(const Kernel &k, size_t b, size_t e, bool compute_form) {
+// Recognises a clamped-face read of the staged grid along axis d:
+//   if (i == 0)     v = PSGridGet(g, i, j, k); else v = PSGridGet(g, i-1, j, k);
+//   if (i == n - 1) v = PSGridGet(g, i, j, k); else v = PSGridGet(g, i+1, j, k);
+// (n a local holding PSGridDim(g, d), or PSGridDim(g, d) itself), i.e.
+// v = g at the clamped coordinate i-1 / i+1.  Returns d, or -1.
+int Translator::ClampIf(const Kernel &k, size_t i, size_t e) const {
+  if (k.staged.empty() || k.rank != 3 || i + 1 >= e || t_[i].s != "if" || t_[i + 1].s != "(") return -1;
+  const size_t cl = MatchClose(t_, i + 1);
+  if (cl >= e) return -1;
+  size_t c = i + 2;
+  int d = -1;
+  for (int a = 0; a < 3; ++a)
+    if (t_[c].kind == Tok::kIdent && t_[c].s == k.idx[a]) d = a;
+  if (d < 0 || t_[c + 1].s != "==") return -1;
+  c += 2;
+  bool low;
+  if (cl - c == 1 && t_[c].kind == Tok::kNum && t_[c].s == "0") {
+    low = true;
+  } else if (cl - c == 3 && t_[c].kind == Tok::kIdent && k.dimvar.count(t_[c].s) && k.dimvar.at(t_[c].s) == d &&
+             t_[c + 1].s == "-" && t_[c + 2].s == "1") {
+    low = false;
+  } else if (cl - c == 8 && t_[c].s == "PSGridDim" && t_[c + 1].s == "(" && t_[c + 2].s == k.staged &&
+             t_[c + 3].s == "," && t_[c + 4].kind == Tok::kNum && atoi(t_[c + 4].s.c_str()) == d &&
+             t_[c + 5].s == ")" && t_[c + 6].s == "-" && t_[c + 7].s == "1") {
+    low = false;
+  } else {
+    return -1;
+  }
+  // "v = PSGridGet(g, ...);" with the given offsets; returns the index after
+  // the ';' or 0.
+  auto assign = [&](size_t q, const std::string &v, const int (&want)[3], std::string *name) -> size_t {
+    if (q + 3 >= e || t_[q].kind != Tok::kIdent || t_[q + 1].s != "=" || t_[q + 2].s != "PSGridGet" ||
+        t_[q + 3].s != "(")
+      return 0;
+    if (!v.empty() && t_[q].s != v) return 0;
+    *name = t_[q].s;
+    const size_t close = MatchClose(t_, q + 3);
+    if (close + 1 >= e || t_[close + 1].s != ";") return 0;
+    auto args = SplitArgs(t_, q + 4, close);
+    if (args.size() != 4 || Text(t_, args[0].first, args[0].second) != k.staged) return 0;
+    for (int a = 0; a < 3; ++a) {
+      int off;
+      if (!RelativeIndex(t_, args[a + 1].first, args[a + 1].second, k.idx[a], &off) || off != want[a]) return 0;
+    }
+    return close + 2;
+  };
+  const int zero[3] = {0, 0, 0};
+  int nb[3] = {0, 0, 0};
+  nb[d] = low ? -1 : 1;
+  std::string v;
+  size_t q = assign(cl + 1, "", zero, &v);
+  if (!q || q >= e || t_[q].s != "else") return -1;
+  std::string v2;
+  if (!assign(q + 1, v, nb, &v2)) return -1;
+  return d;
+}
+
+std::string Translator::RewriteBody(const Kernel &k, size_t b, size_t e, bool compute_form, bool exact) {
   std::string out;
   int pdepth = 0;
   auto add = [&](const std::string &s) {
@@ -824,6 +912,20 @@ std::string Translator::RewriteBody(const Kernel &k, size_t b, size_t e, bool co
   };
   for (size_t i = b; i < e; ++i) {
     const Tok &tk = t_[i];
+    if (exact && tk.kind == Tok::kIdent && tk.s == "if") {
+      const int d = ClampIf(k, i, e);
+      if (d >= 0) {
+        // The skeleton supplies the clamped-coordinate neighbour along axis
+        // d when __kExact<d> is set, so the condition folds to false there.
+        add("if");
+        add("(");
+        add(std::string("!__kExact") + "XYZ"[d]);
+        add("&&");
+        ++pdepth;  // stands for the condition's own "(" skipped here
+        ++i;
+        continue;
+      }
+    }
     if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" && tk.s == "PSGridEmitUtype") {
       size_t close = MatchClose(t_, i + 1);
       auto args = SplitArgs(t_, i + 2, close);
@@ -976,6 +1078,17 @@ std::string Translator::EmitPlan(const Kernel &k) {
     o << "  template <class C>\n  __device__ static float compute(" << params << ") {\n";
     o << "    (void)__ctx;\n" << prologue();
     o << "    {\n" << RewriteBody(k, f.lbrace + 1, f.rbrace, true) << "    }\n  }\n";
+  }
+  if (k.star && k.clamp_ifs > 0) {
+    // compute() for a point whose neighbours along the flagged axes are the
+    // staged grid's elements at clamped coordinates (skeleton_star_tb.h):
+    // the clamped-face reads then take the neighbour directly.
+    o << "  static constexpr bool kExactNeighbours = true;\n";
+    o << "  template <bool __kExactX, bool __kExactY, bool __kExactZ, class C>\n  __device__ static float "
+         "compute_exact("
+      << params << ") {\n";
+    o << "    (void)__ctx;\n" << prologue();
+    o << "    {\n" << RewriteBody(k, f.lbrace + 1, f.rbrace, true, true) << "    }\n  }\n";
   }
   o << "};\n\n";
   return o.str();

@@ -1,0 +1,115 @@
+"""physisc-hip (CPU only): the clamped-face read recognition behind a star
+plan's compute_exact.
+
+A clamped-face read `if (x == 0) w = PSGridGet(g, x, y, z); else w =
+PSGridGet(g, x-1, y, z);` equals the staged grid at the clamped coordinate,
+so where the fused skeleton supplies clamped-coordinate neighbours the
+translator may fold the condition away (`!__kExactX && ...`).  Anything that
+is not exactly that form must keep its condition: the tests below translate
+small programs and check which conditionals were rewritten.  The numerics of
+the rewritten plans are checked on the GPU by the corpus tests (the
+reference's Neumann-condition test and diffusion benchmark)."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PHYSISC = os.path.join(ROOT, "physis_amd", "bin", "physisc-hip")
+
+PROGRAM = """
+#include "physis/physis.h"
+static void kernel(const int x, const int y, const int z, PSGrid3DFloat g1, PSGrid3DFloat g2, float k) {
+  int nx, ny, nz;
+  nx = PSGridDim(g1, 0);
+  ny = PSGridDim(g1, 1);
+  nz = PSGridDim(g1, 2);
+  %(extra)s
+  float c, w, e, n, s, b, t;
+  c = PSGridGet(g1, x, y, z);
+  %(body)s
+  PSGridEmit(g2, k * (c + w + e + n + s + b + t));
+  return;
+}
+int main(int argc, char *argv[]) {
+  PSInit(&argc, &argv, 3, 16, 16, 16);
+  PSGrid3DFloat g1 = PSGrid3DFloatNew(16, 16, 16);
+  PSGrid3DFloat g2 = PSGrid3DFloatNew(16, 16, 16);
+  PSDomain3D d = PSDomain3DNew(0, 16, 0, 16, 0, 16);
+  PSStencilRun(PSStencilMap(kernel, d, g1, g2, 0.1f), PSStencilMap(kernel, d, g2, g1, 0.1f), 2);
+  PSGridFree(g1);
+  PSGridFree(g2);
+  PSFinalize();
+  return 0;
+}
+"""
+
+CLAMPED = """
+  if (x == 0) w = PSGridGet(g1, x, y, z); else w = PSGridGet(g1, x-1, y, z);
+  if (x == nx-1) e = PSGridGet(g1, x, y, z); else e = PSGridGet(g1, x+1, y, z);
+  if (y == 0) n = PSGridGet(g1, x, y, z); else n = PSGridGet(g1, x, y-1, z);
+  if (y == PSGridDim(g1, 1) - 1) s = PSGridGet(g1, x, y, z); else s = PSGridGet(g1, x, y+1, z);
+  if (z == 0) b = PSGridGet(g1, x, y, z); else b = PSGridGet(g1, x, y, z-1);
+  if (z == nz-1) t = PSGridGet(g1, x, y, z); else t = PSGridGet(g1, x, y, z+1);
+"""
+
+
+def translate(tmp_path, body, extra=""):
+    if not os.path.exists(PHYSISC):
+        pytest.skip("physisc-hip is not built (make -C physis_amd/translator)")
+    src = tmp_path / "k.c"
+    src.write_text(PROGRAM % {"body": body, "extra": extra})
+    out = tmp_path / "k.hip"
+    subprocess.run([PHYSISC, "-o", str(out), str(src)], check=True, capture_output=True)
+    return out.read_text()
+
+
+def exact_body(hip):
+    """The text of compute_exact, or None when the plan has none."""
+    at = hip.find("compute_exact(")
+    if at < 0:
+        return None
+    return hip[at:hip.find("\n  }\n", at)]
+
+
+def test_all_six_clamped_faces_fold(tmp_path):
+    body = exact_body(translate(tmp_path, CLAMPED))
+    assert body is not None
+    for axis, n in (("X", 2), ("Y", 2), ("Z", 2)):
+        assert body.count("!__kExact" + axis) == n, axis
+    # compute() itself keeps every condition unchanged.
+    hip = translate(tmp_path, CLAMPED)
+    plain = hip[hip.find("compute(const C"):hip.find("kExactNeighbours")]
+    assert "__kExact" not in plain and plain.count("if") == 6
+
+
+@pytest.mark.parametrize("body", [
+    # offset on the wrong side of the face
+    "if (x == 0) w = PSGridGet(g1, x, y, z); else w = PSGridGet(g1, x+1, y, z);",
+    # the face value is not the point itself
+    "if (x == 0) w = PSGridGet(g1, x+1, y, z); else w = PSGridGet(g1, x-1, y, z);",
+    # another axis in the condition than in the offset
+    "if (y == 0) w = PSGridGet(g1, x, y, z); else w = PSGridGet(g1, x-1, y, z);",
+    # upper face tested against another axis' extent
+    "if (x == ny-1) w = PSGridGet(g1, x, y, z); else w = PSGridGet(g1, x+1, y, z);",
+    # different targets in the two branches
+    "if (x == 0) w = PSGridGet(g1, x, y, z); else e = PSGridGet(g1, x-1, y, z);",
+    # a face other than 0 or n-1
+    "if (x == 1) w = PSGridGet(g1, x, y, z); else w = PSGridGet(g1, x-1, y, z);",
+])
+def test_other_conditionals_are_kept(tmp_path, body):
+    hip = translate(tmp_path, body + "\n  e = n = s = b = t = c;")
+    assert exact_body(hip) is None
+
+
+def test_extent_variable_assigned_twice_is_not_trusted(tmp_path):
+    body = "if (x == nx-1) e = PSGridGet(g1, x, y, z); else e = PSGridGet(g1, x+1, y, z);\n  w = n = s = b = t = c;"
+    assert exact_body(translate(tmp_path, body)) is not None
+    for extra in ("nx = nx - 1;", "nx += 1;", "nx++;", "--nx;", "int *p = &nx; (void)p;"):
+        assert exact_body(translate(tmp_path, body, extra=extra)) is None, extra
+
+
+def test_periodic_kernels_have_no_exact_form(tmp_path):
+    body = ("if (x == 0) w = PSGridGet(g1, x, y, z); else w = PSGridGet(g1, x-1, y, z);\n"
+            "  e = PSGridGetPeriodic(g1, x+1, y, z);\n  n = s = b = t = c;")
+    assert exact_body(translate(tmp_path, body)) is None
