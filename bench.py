@@ -155,6 +155,25 @@ def served_bench(rt, n_req=24, rate=40.0, seed=7, budget_gib=48, tick=50):
     return out
 
 
+def storage16_bench(rt, n, steps=200):
+    """The same clamped 7-point diffusion with BF16 and FP16 grid storage
+    (FP32 arithmetic; 4 algorithmic bytes per lattice update instead of 8):
+    seeded uniform [0,1) field, 10 warmup steps, then `steps` timed steps
+    through PSHipDiffusion7Run.  Error against FP64 is checked by the tests."""
+    out = {"workload": "7-point diffusion %d^3 clamped, %d steps, 16-bit storage, FP32 arithmetic" % (n, steps)}
+    for storage, name in ((rt.STORAGE_BF16, "bf16"), (rt.STORAGE_FP16, "fp16")):
+        g1, g2 = rt.Grid((n, n, n), storage=storage), rt.Grid((n, n, n), storage=storage)
+        g1.fill_uniform(1)
+        rt.diffusion7_run(g1, g2, COEF, 0, 10)
+        wall, dev = time_steps(rt, g1, g2, 0, steps, 0)
+        gbs = 4.0 * n ** 3 * steps / dev / 1e9
+        out[name] = {"glups": round(n ** 3 * steps / wall / 1e9, 1), "algorithmic_gbs": round(gbs, 1),
+                     "hbm_roofline_pct": round(100 * gbs / HBM_PEAK_GBS, 1)}
+        g1.free()
+        g2.free()
+    return out
+
+
 # (degree, tile rows, rows per wave, z chunk) of the fused Himeno kernel.
 HIMENO_FUSED_SHAPES = [(2, 8, 1, 8), (2, 8, 1, 16), (2, 8, 1, 32), (2, 16, 1, 32), (3, 16, 1, 32)]
 
@@ -317,6 +336,8 @@ def main():
             file=sys.stderr, flush=True)
     g1.free()
     g2.free()
+    storage16 = storage16_bench(rt, n)
+    print("16-bit storage: %s" % json.dumps(storage16), file=sys.stderr, flush=True)
     refbench = reference_benchmark(n, args.steps)
     print("reference benchmark: %s" % json.dumps(refbench), file=sys.stderr, flush=True)
     batched = None
@@ -354,6 +375,7 @@ def main():
                    "periodic_glups": round(n ** 3 * args.steps / results[1][0] / 1e9, 2),
                    "hbm_roofline_pct": round(100 * achieved / HBM_PEAK_GBS, 2),
                    "stream_copy_gbs": round(copy_gbs, 1),
+                   "storage16": storage16,
                    "reference_benchmark": refbench,
                    "batched": batched,
                    "himeno": himeno,

@@ -33,29 +33,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include <type_traits>
-
 #include "physis/hip/skeleton.h"
 #include "physis/hip/skeleton_star.h"
 
 namespace psk {
-
-// Whether Plan provides compute_exact<kX, kY, kZ>: compute() for a point
-// whose x (kX), y (kY) and z (kZ) neighbours are the elements at the clamped
-// or wrapped coordinates.
-template <class Plan, class = void>
-struct ExactNeighbours : std::false_type {};
-template <class Plan>
-struct ExactNeighbours<Plan, std::void_t<decltype(Plan::kExactNeighbours)>>
-    : std::integral_constant<bool, Plan::kExactNeighbours> {};
-
-template <class Plan, bool kX, bool kY, bool kZ, class C>
-__device__ __forceinline__ float ComputeAt(const C &c, const typename Plan::Args &a, int x, int y, int z) {
-  if constexpr (ExactNeighbours<Plan>::value)
-    return Plan::template compute_exact<kX, kY, kZ>(c, a, x, y, z);
-  else
-    return Plan::compute(c, a, x, y, z);
-}
 
 struct TBGeometry {
   int tiles_y;  // y tiles, each writing BY-2 rows

@@ -8,7 +8,13 @@
 // publishes the row to an LDS buffer (double-buffered by step parity, four
 // pad floats at each end) from which x-1/x+1 and y-1/y+1 are read.  Periodic
 // plans write the wrapped row ends into the pads; clamped plans never select
-// them.
+// them.  Halo rows and planes are loaded at wrapped/clamped coordinates, so
+// the y and z neighbours a one-step point or a fused step's u1 point reads
+// are the elements there: plans with kExactNeighbours drop their y/z
+// clamped-face selects (ComputeAt).  A fused step's u2 reads its own row at
+// a clamped y face (intermediate halo rows are not copies of in-grid ones)
+// and keeps the z-face selects.  (Filling clamped pads per row too, to drop
+// the x selects, made the four- and eight-point kernels spill.)
 //
 // These are tile-level device functions: the single-grid launchers and the
 // batched launchers (many grids in one launch) call them per work item.
@@ -51,6 +57,13 @@ __device__ __forceinline__ void StridePut(float *r, const float (&v)[NP], int la
   }
 }
 
+// Wrapped/clamped index of plane z + 1 given that of plane z (no integer
+// modulo, which costs ~25 VALU instructions on this target).
+template <bool W>
+__device__ __forceinline__ int NextPlane(int z, int nz) {
+  return W ? (z + 1 == nz ? 0 : z + 1) : min(z + 1, nz - 1);
+}
+
 template <class S, int NP>
 __device__ __forceinline__ void StrideLoad(const typename S::stored_t *row, float (&v)[NP], int lane, int nx) {
 #pragma unroll
@@ -86,12 +99,15 @@ __device__ __forceinline__ void StrideStep(const typename Plan::Args &a, int nx,
   StrideLoad<S, NP>(in + zo(zb + 1) + row, t, lane, nx);
   if (halo_lo) StrideLoad<S, NP>(in + zo(zb) + hlo_row, hlo, lane, nx);
   if (halo_hi) StrideLoad<S, NP>(in + zo(zb) + hhi_row, hhi, lane, nx);
+  int z1 = FixIndex<W>(zb + 1, nz), z2 = FixIndex<W>(zb + 2, nz);  // planes z+1, z+2
   for (int z = zb; z < ze; ++z) {
     if (z + 1 < ze) {
-      StrideLoad<S, NP>(in + zo(z + 2) + row, p, lane, nx);
-      if (halo_lo) StrideLoad<S, NP>(in + zo(z + 1) + hlo_row, nlo, lane, nx);
-      if (halo_hi) StrideLoad<S, NP>(in + zo(z + 1) + hhi_row, nhi, lane, nx);
+      StrideLoad<S, NP>(in + (int64_t)z2 * plane + row, p, lane, nx);
+      if (halo_lo) StrideLoad<S, NP>(in + (int64_t)z1 * plane + hlo_row, nlo, lane, nx);
+      if (halo_hi) StrideLoad<S, NP>(in + (int64_t)z1 * plane + hhi_row, nhi, lane, nx);
     }
+    z1 = z2;
+    z2 = NextPlane<W>(z2, nz);
     auto &R = L.step[z & 1];
     StridePut<W, NP>(R[wy + 1], c, lane, nx);
     if (halo_lo) StridePut<W, NP>(R[0], hlo, lane, nx);
@@ -111,7 +127,7 @@ __device__ __forceinline__ void StrideStep(const typename Plan::Args &a, int nx,
           cx.s = R[wy + 2][x + 4];
           cx.b = b[k];
           cx.t = t[k];
-          S::store(o + x, Plan::compute(cx, a, x, y, z));
+          S::store(o + x, ComputeAt<Plan, false, true, true>(cx, a, x, y, z));
         }
       }
     }
@@ -155,6 +171,8 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
   StrideLoad<S, NP>(in + zo(zb) + row, t0, lane, nx);
   if (halo_lo) StrideLoad<S, NP>(in + zo(zb - 1) + hlo_row, hlo, lane, nx);
   if (halo_hi) StrideLoad<S, NP>(in + zo(zb - 1) + hhi_row, hhi, lane, nx);
+  // Wrapped/clamped indices of planes k, k+1, k+2.
+  int z0 = FixIndex<W>(zb - 1, nz), z1 = FixIndex<W>(zb, nz), z2 = FixIndex<W>(zb + 1, nz);
   for (int k = zb - 1; k <= ze; ++k) {
     const int buf = (k + 2) & 1;
     auto &R0 = L.u0[buf];
@@ -164,12 +182,15 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
     if (halo_hi) StridePut<W, NP>(R0[BY + 1], hhi, lane, nx);
     StridePut<W, NP>(R1[wy], c1, lane, nx);
     if (k < ze) {
-      StrideLoad<S, NP>(in + zo(k + 2) + row, p0, lane, nx);
-      if (halo_lo) StrideLoad<S, NP>(in + zo(k + 1) + hlo_row, nlo, lane, nx);
-      if (halo_hi) StrideLoad<S, NP>(in + zo(k + 1) + hhi_row, nhi, lane, nx);
+      StrideLoad<S, NP>(in + (int64_t)z2 * plane + row, p0, lane, nx);
+      if (halo_lo) StrideLoad<S, NP>(in + (int64_t)z1 * plane + hlo_row, nlo, lane, nx);
+      if (halo_hi) StrideLoad<S, NP>(in + (int64_t)z1 * plane + hhi_row, nhi, lane, nx);
     }
     __syncthreads();
-    const int kf = FixIndex<W>(k, nz);
+    const int kf = z0;
+    z0 = z1;
+    z1 = z2;
+    z2 = NextPlane<W>(z2, nz);
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
       const int x = lane + 64 * q;
@@ -182,11 +203,12 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
         cx.s = R0[wy + 2][x + 4];
         cx.b = b0[q];
         cx.t = t0[q];
-        t1[q] = RoundTrip<S>(Plan::compute(cx, a1, x, yf, kf));
+        t1[q] = RoundTrip<S>(ComputeAt<Plan, false, true, true>(cx, a1, x, yf, kf));
       }
     }
     if (out_row && k - 1 >= zb) {
       st *o = out + (int64_t)(k - 1) * plane + (int64_t)y * nx;
+      const int rn = !W && y == 0 ? wy : wy - 1, rs = !W && y == ny - 1 ? wy : wy + 1;
 #pragma unroll
       for (int q = 0; q < NP; ++q) {
         const int x = lane + 64 * q;
@@ -195,11 +217,11 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
           cx.c = c1[q];
           cx.w = R1[wy][x + 3];
           cx.e = R1[wy][x + 5];
-          cx.n = R1[wy - 1][x + 4];
-          cx.s = R1[wy + 1][x + 4];
+          cx.n = R1[rn][x + 4];
+          cx.s = R1[rs][x + 4];
           cx.b = b1[q];
           cx.t = t1[q];
-          S::store(o + x, Plan::compute(cx, a2, x, y, k - 1));
+          S::store(o + x, ComputeAt<Plan, false, true, false>(cx, a2, x, y, k - 1));
         }
       }
     }

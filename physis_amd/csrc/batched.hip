@@ -17,6 +17,7 @@
 //
 // No reference counterpart: the reference runs one grid per PSStencilRun.
 #include "runtime_hip.h"
+#include "diffusion7_body.h"
 #include "physis/hip/skeleton.h"
 #include "physis/hip/skeleton_star.h"
 #include "physis/hip/skeleton_stride.h"
@@ -31,10 +32,6 @@
 
 namespace physis_hip {
 namespace {
-
-struct Coef7 {
-  float ce, cw, cn, cs, ct, cb, cc;
-};
 
 // The diffusion body (reference examples/diffusion-benchmark/diffusion3d_physis.c:29-58;
 // periodic form test_7-pt-periodic.c:12-24), one value per point.
@@ -57,26 +54,14 @@ struct BatchPlan {
   __host__ __device__ static st *out_ptr(const Args &a) { return static_cast<st *>(a.out); }
   template <class C>
   __device__ static float compute(const C &ctx, const Args &a, const int x, const int y, const int z) {
-    const int nx = a.dim[0], ny = a.dim[1], nz = a.dim[2];
-    float c, w, e, n, s, b, t;
-    c = ctx.at(0, 0, 0, 0);
-    if (kPeriodic) {
-      w = ctx.at(0, -1, 0, 0);
-      e = ctx.at(0, 1, 0, 0);
-      n = ctx.at(0, 0, -1, 0);
-      s = ctx.at(0, 0, 1, 0);
-      b = ctx.at(0, 0, 0, -1);
-      t = ctx.at(0, 0, 0, 1);
-    } else {
-      if (x == 0) w = ctx.at(0, 0, 0, 0); else w = ctx.at(0, -1, 0, 0);
-      if (x == nx - 1) e = ctx.at(0, 0, 0, 0); else e = ctx.at(0, 1, 0, 0);
-      if (y == 0) n = ctx.at(0, 0, 0, 0); else n = ctx.at(0, 0, -1, 0);
-      if (y == ny - 1) s = ctx.at(0, 0, 0, 0); else s = ctx.at(0, 0, 1, 0);
-      if (z == 0) b = ctx.at(0, 0, 0, 0); else b = ctx.at(0, 0, 0, -1);
-      if (z == nz - 1) t = ctx.at(0, 0, 0, 0); else t = ctx.at(0, 0, 0, 1);
-    }
-    const Coef7 &k = a.k;
-    return k.cc * c + k.cw * w + k.ce * e + k.cs * s + k.cn * n + k.cb * b + k.ct * t;
+    return Diffusion7Body<kPeriodic, false, false, false>(ctx, a.k, x, y, z, a.dim[0], a.dim[1], a.dim[2]);
+  }
+  // compute() where the stride skeleton supplies clamped-coordinate
+  // neighbours along the flagged axes.
+  static constexpr bool kExactNeighbours = true;
+  template <bool kX, bool kY, bool kZ, class C>
+  __device__ static float compute_exact(const C &ctx, const Args &a, const int x, const int y, const int z) {
+    return Diffusion7Body<kPeriodic, kX, kY, kZ>(ctx, a.k, x, y, z, a.dim[0], a.dim[1], a.dim[2]);
   }
 };
 

@@ -7,6 +7,7 @@
 // left to right in FP32 with no contraction into FMA, exactly as the
 // sequential ref target does, so results are bitwise equal to it.
 #include "runtime_hip.h"
+#include "diffusion7_body.h"
 #include "physis/hip/skeleton.h"
 #include "physis/hip/skeleton_star.h"
 #include "physis/hip/skeleton_star_tb.h"
@@ -16,10 +17,6 @@
 
 namespace physis_hip {
 namespace {
-
-struct Coef7 {
-  float ce, cw, cn, cs, ct, cb, cc;
-};
 
 // ---- variant family 1: the generic skeleton with an emitted body ---------
 // Plan for map(kernel, dom, g_in, g_out, ce, cw, cn, cs, ct, cb, cc).
@@ -52,61 +49,14 @@ struct DiffusionPlan {
   }
   template <class C>
   __device__ static float compute(const C &ctx, const Args &a, const int x, const int y, const int z) {
-    const int nx = a.dim[0], ny = a.dim[1], nz = a.dim[2];
-    float c, w, e, n, s, b, t;
-    c = ctx.at(0, 0, 0, 0);
-    if (kPeriodic) {
-      w = ctx.at(0, -1, 0, 0);
-      e = ctx.at(0, 1, 0, 0);
-      n = ctx.at(0, 0, -1, 0);
-      s = ctx.at(0, 0, 1, 0);
-      b = ctx.at(0, 0, 0, -1);
-      t = ctx.at(0, 0, 0, 1);
-    } else {
-      if (x == 0) w = ctx.at(0, 0, 0, 0); else w = ctx.at(0, -1, 0, 0);
-      if (x == nx - 1) e = ctx.at(0, 0, 0, 0); else e = ctx.at(0, 1, 0, 0);
-      if (y == 0) n = ctx.at(0, 0, 0, 0); else n = ctx.at(0, 0, -1, 0);
-      if (y == ny - 1) s = ctx.at(0, 0, 0, 0); else s = ctx.at(0, 0, 1, 0);
-      if (z == 0) b = ctx.at(0, 0, 0, 0); else b = ctx.at(0, 0, 0, -1);
-      if (z == nz - 1) t = ctx.at(0, 0, 0, 0); else t = ctx.at(0, 0, 0, 1);
-    }
-    const Coef7 &k = a.k;
-    return k.cc * c + k.cw * w + k.ce * e + k.cs * s + k.cn * n + k.cb * b + k.ct * t;
+    return Diffusion7Body<kPeriodic, false, false, false>(ctx, a.k, x, y, z, a.dim[0], a.dim[1], a.dim[2]);
   }
   // compute() for a point whose x (kX), y (kY) and z (kZ) neighbours in ctx
-  // are the elements at the clamped coordinates (skeleton_star_tb.h): at a
-  // clamped face the select above picks the point itself, which is that same
-  // element, so the select is dropped.  Same expression, same result.
+  // are the elements at the clamped coordinates (skeleton_star_tb.h).
   static constexpr bool kExactNeighbours = true;
   template <bool kX, bool kY, bool kZ, class C>
   __device__ static float compute_exact(const C &ctx, const Args &a, const int x, const int y, const int z) {
-    if (kPeriodic) return compute(ctx, a, x, y, z);
-    const int nx = a.dim[0], ny = a.dim[1], nz = a.dim[2];
-    float c, w, e, n, s, b, t;
-    c = ctx.at(0, 0, 0, 0);
-    if (kX) {
-      w = ctx.at(0, -1, 0, 0);
-      e = ctx.at(0, 1, 0, 0);
-    } else {
-      if (x == 0) w = c; else w = ctx.at(0, -1, 0, 0);
-      if (x == nx - 1) e = c; else e = ctx.at(0, 1, 0, 0);
-    }
-    if (kY) {
-      n = ctx.at(0, 0, -1, 0);
-      s = ctx.at(0, 0, 1, 0);
-    } else {
-      if (y == 0) n = c; else n = ctx.at(0, 0, -1, 0);
-      if (y == ny - 1) s = c; else s = ctx.at(0, 0, 1, 0);
-    }
-    if (kZ) {
-      b = ctx.at(0, 0, 0, -1);
-      t = ctx.at(0, 0, 0, 1);
-    } else {
-      if (z == 0) b = c; else b = ctx.at(0, 0, 0, -1);
-      if (z == nz - 1) t = c; else t = ctx.at(0, 0, 0, 1);
-    }
-    const Coef7 &k = a.k;
-    return k.cc * c + k.cw * w + k.ce * e + k.cs * s + k.cn * n + k.cb * b + k.ct * t;
+    return Diffusion7Body<kPeriodic, kX, kY, kZ>(ctx, a.k, x, y, z, a.dim[0], a.dim[1], a.dim[2]);
   }
 };
 
