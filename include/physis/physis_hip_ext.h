@@ -118,11 +118,13 @@ extern const char *PSHipDiffusion7VariantName(int variant);
  * one-sweep shape of the fused kernel (coefficients loaded a plane ahead). */
 extern int PSHipHimenoRun(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids,
                           float omega, int sweeps, double *gosa, int variant);
-/* The same sweeps on a generic-skeleton tile of bx x by points (64 x 2/4/8,
- * 128 x 2/4, 256 x 1/2) with zchunk planes per workgroup (0: all); for
- * tuning.  PS_HIP_ERR_UNSUPPORTED for other tiles. */
+/* The same sweeps on the generic skeleton with bx x by threads computing
+ * `rows` tile rows each (64 x 2/4/8, 128 x 2/4, 256 x 1/2 with one row;
+ * 64 x 4, 128 x 2, 256 x 1 with two) and zchunk planes per workgroup (0:
+ * all); for tuning.  PS_HIP_ERR_UNSUPPORTED for other tiles. */
 extern int PSHipHimenoRunTile(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids,
-                              float omega, int sweeps, double *gosa, int bx, int by, int zchunk);
+                              float omega, int sweeps, double *gosa, int bx, int by, int rows,
+                              int zchunk);
 /* The same sweeps with `degree` (1..4) of them fused per HBM pass (temporal
  * blocking; no reference counterpart): each coefficient element is loaded
  * once per pass instead of once per sweep.  Tiles of 64 x by points with

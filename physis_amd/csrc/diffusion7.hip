@@ -236,9 +236,10 @@ hipError_t LaunchOne(const Variant &v, const void *in, void *out, const int *dim
       }
       return hipErrorInvalidValue;
     }
-#define PSH_RING(BY, R) \
-    if (v.bx == 64 && v.by == BY && v.rows == R) return psk::LaunchZMarch<P, 64, BY, R>(a, box, v.zchunk, v.remap, s);
-    PSH_RING(4, 1) PSH_RING(8, 1) PSH_RING(2, 1) PSH_RING(4, 2) PSH_RING(4, 4) PSH_RING(8, 2) PSH_RING(2, 4)
+#define PSH_RING(BX, BY, R) \
+    if (v.bx == BX && v.by == BY && v.rows == R) return psk::LaunchZMarch<P, BX, BY, R>(a, box, v.zchunk, v.remap, s);
+    PSH_RING(64, 4, 1) PSH_RING(64, 8, 1) PSH_RING(64, 2, 1) PSH_RING(64, 4, 2) PSH_RING(64, 4, 4) PSH_RING(64, 8, 2)
+    PSH_RING(64, 2, 4)
 #undef PSH_RING
     return hipErrorInvalidValue;
   }

@@ -373,10 +373,10 @@ int Blocks(const Variant &v, const psk::Box &box) {
 
 template <class P>
 hipError_t LaunchVariant(const Variant &v, const typename P::Args &a, const psk::Box &box, hipStream_t s) {
-#define PSH_RING(BX, BY) \
-  if (v.bx == BX && v.by == BY && v.rows == 1) return psk::LaunchZMarch<P, BX, BY, 1>(a, box, v.zchunk, true, s);
-  PSH_RING(64, 4) PSH_RING(64, 8) PSH_RING(64, 2) PSH_RING(128, 2) PSH_RING(128, 4) PSH_RING(256, 1)
-  PSH_RING(256, 2)
+#define PSH_RING(BX, BY, R) \
+  if (v.bx == BX && v.by == BY && v.rows == R) return psk::LaunchZMarch<P, BX, BY, R>(a, box, v.zchunk, true, s);
+  PSH_RING(64, 4, 1) PSH_RING(64, 8, 1) PSH_RING(64, 2, 1) PSH_RING(128, 2, 1) PSH_RING(128, 4, 1) PSH_RING(256, 1, 1)
+  PSH_RING(256, 2, 1) PSH_RING(256, 1, 2) PSH_RING(128, 2, 2) PSH_RING(64, 4, 2)
 #undef PSH_RING
   return hipErrorInvalidValue;
 }
@@ -763,16 +763,19 @@ extern "C" int PSHipHimenoRunTB(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *gri
   return PS_HIP_OK;
 }
 
-// One sweep per launch on the generic skeleton with a tile of bx x by
-// points (64 x 2/4/8, 128 x 2/4, 256 x 1/2) and `zchunk` planes per
-// workgroup (0: all).
+// One sweep per launch on the generic skeleton with bx x by threads, each
+// computing `rows` tile rows (tiles of bx x by*rows points): 64 x 2/4/8,
+// 128 x 2/4, 256 x 1/2 with one row, 64 x 4, 128 x 2, 256 x 1 with two;
+// `zchunk` planes per workgroup (0: all).
 extern "C" int PSHipHimenoRunTile(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
-                                  double *gosa, int bx, int by, int zchunk) {
+                                  double *gosa, int bx, int by, int rows, int zchunk) {
   if (!SameGridSet(p0, p1, grids) || sweeps < 0 || zchunk < 0) return PS_HIP_ERR_INVALID;
-  const bool ok = (bx == 64 && (by == 2 || by == 4 || by == 8)) || (bx == 128 && (by == 2 || by == 4)) ||
-                  (bx == 256 && (by == 1 || by == 2));
-  if (!ok) return PS_HIP_ERR_UNSUPPORTED;
-  const Variant v{"tile", bx, by, zchunk};
+  const bool one = (bx == 64 && (by == 2 || by == 4 || by == 8)) || (bx == 128 && (by == 2 || by == 4)) ||
+                   (bx == 256 && (by == 1 || by == 2));
+  const bool two = (bx == 64 && by == 4) || (bx == 128 && by == 2) || (bx == 256 && by == 1);
+  if (!((rows == 1 && one) || (rows == 2 && two))) return PS_HIP_ERR_UNSUPPORTED;
+  Variant v{"tile", bx, by, zchunk};
+  v.rows = rows;
   switch (Rec(p0)->storage) {
     case PS_HIP_STORAGE_NATIVE: return RunTyped<psk::Native<float>>(p0, p1, grids, omega, sweeps, gosa, v);
     case PS_HIP_STORAGE_BF16: return RunTyped<psk::BF16>(p0, p1, grids, omega, sweeps, gosa, v);

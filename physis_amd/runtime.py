@@ -233,17 +233,18 @@ def himeno_run(p0, p1, coef_grids, omega, sweeps, with_gosa=True, variant=0):
     return np.array(gosa[:sweeps], np.float64)
 
 
-def himeno_run_tile(p0, p1, coef_grids, omega, sweeps, bx, by, zchunk, with_gosa=True):
-    """Himeno sweeps, one per launch, on the generic skeleton with tiles of
-    bx x by points (64 x 2/4/8, 128 x 2/4, 256 x 1/2) and zchunk planes per
+def himeno_run_tile(p0, p1, coef_grids, omega, sweeps, bx, by, zchunk, rows=1, with_gosa=True):
+    """Himeno sweeps, one per launch, on the generic skeleton with bx x by
+    threads of `rows` tile rows each (64 x 2/4/8, 128 x 2/4, 256 x 1/2 with
+    one row; 64 x 4, 128 x 2, 256 x 1 with two) and zchunk planes per
     workgroup (0 = all).  Returns the FP64 sum of ss^2 per sweep."""
     L = lib()
     L.PSHipHimenoRunTile.argtypes = [_GP, _GP, ctypes.POINTER(_GP), ctypes.c_float, ctypes.c_int,
-                                     ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int, ctypes.c_int]
+                                     ctypes.POINTER(ctypes.c_double)] + [ctypes.c_int] * 4
     arr = (_GP * 12)(*[g.handle for g in coef_grids])
     gosa = (ctypes.c_double * max(sweeps, 1))()
     check(L.PSHipHimenoRunTile(p0.handle, p1.handle, arr, omega, sweeps, gosa if with_gosa else None, bx, by,
-                               zchunk), "PSHipHimenoRunTile")
+                               rows, zchunk), "PSHipHimenoRunTile")
     return np.array(gosa[:sweeps], np.float64)
 
 

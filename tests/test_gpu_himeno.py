@@ -194,13 +194,15 @@ def test_himeno_prefetch_distances_bitwise_vs_oracle(rt, cfg):
 
 
 @pytest.mark.parametrize("tile", [(64, 2, 1), (64, 2, 8), (64, 4, 3), (64, 4, 8), (64, 4, 0), (64, 8, 2), (64, 8, 16),
-                                  (128, 2, 4), (128, 4, 5), (256, 1, 8), (256, 2, 0)])
+                                  (128, 2, 4), (128, 4, 5), (256, 1, 8), (256, 2, 0),
+                                  (64, 4, 5, 2), (128, 2, 3, 2), (256, 1, 0, 2)])
 @pytest.mark.parametrize("storage", [0, 1])
 def test_himeno_tile_shapes_vs_oracle(rt, tile, storage):
     """Every tile height and z-chunk of the generic skeleton gives the same
     fields: bitwise for FP32, equal to variant 1 for BF16 storage."""
     shape = (70, 21, 37)
-    bx, by, zchunk = tile
+    bx, by, zchunk = tile[:3]
+    rows = tile[3] if len(tile) > 3 else 1
     p, coefs = himeno_inputs(shape, seed=700)
     g0, g1 = rt.Grid(shape, storage=storage), rt.Grid(shape, storage=storage)
     grids = []
@@ -214,7 +216,7 @@ def test_himeno_tile_shapes_vs_oracle(rt, tile, storage):
         g1.copyin(p)
         print("tile", tile, "storage", storage, mode, flush=True)
         if mode == "tile":
-            gosa = rt.himeno_run_tile(g0, g1, grids, 0.8, 7, bx, by, zchunk)
+            gosa = rt.himeno_run_tile(g0, g1, grids, 0.8, 7, bx, by, zchunk, rows=rows)
         else:
             gosa = rt.himeno_run(g0, g1, grids, 0.8, 7, variant=1)
         outs.append((g0.copyout(), g1.copyout(), gosa))
