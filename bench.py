@@ -155,6 +155,46 @@ def served_bench(rt, n_req=24, rate=40.0, seed=7, budget_gib=48, tick=50):
     return out
 
 
+def reduce_and_copy_bench(rt, n, reps=10):
+    """PSReduce over the n^3 FP32 grid (one HBM read pass, FP64 accumulation;
+    the call returns the value, so each includes a device sync and an 8-byte
+    device-to-host copy), and the host-buffer transfers PSGridCopyin /
+    PSGridCopyout of one n^3 FP32 grid from and to pageable numpy memory."""
+    import numpy as np
+    g = rt.Grid((n, n, n))
+    g.fill_uniform(5)
+    nbytes = 4 * n ** 3
+    out = {"workload": "%d^3 FP32 grid (%d MiB)" % (n, nbytes >> 20)}
+    for op, name in ((rt.PS_SUM, "sum"), (rt.PS_MAX, "max")):
+        g.reduce(op)
+        rt.synchronize()
+        timer = rt.Timer()
+        t0 = time.perf_counter()
+        timer.begin()
+        for _ in range(reps):
+            g.reduce(op)
+        timer.end()
+        wall = time.perf_counter() - t0
+        dev = timer.elapsed_ms() / 1e3
+        out["reduce_" + name] = {"gbs_device": round(nbytes * reps / dev / 1e9, 1),
+                                 "us_per_call_wall": round(wall / reps * 1e6, 1)}
+    host = np.empty((n, n, n), np.float32)
+    host[...] = 1.0
+    g.copyin(host)
+    t0 = time.perf_counter()
+    g.copyin(host)
+    rt.synchronize()
+    cin = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    back = g.copyout()
+    cout = time.perf_counter() - t0
+    out["copyin_gbs_pageable"] = round(nbytes / cin / 1e9, 2)
+    out["copyout_gbs_pageable"] = round(nbytes / cout / 1e9, 2)
+    del host, back
+    g.free()
+    return out
+
+
 def storage16_bench(rt, n, steps=200):
     """The same clamped 7-point diffusion with BF16 and FP16 grid storage
     (FP32 arithmetic; 4 algorithmic bytes per lattice update instead of 8):
@@ -338,6 +378,8 @@ def main():
     g2.free()
     storage16 = storage16_bench(rt, n)
     print("16-bit storage: %s" % json.dumps(storage16), file=sys.stderr, flush=True)
+    aux = reduce_and_copy_bench(rt, n)
+    print("reduce and host copies: %s" % json.dumps(aux), file=sys.stderr, flush=True)
     refbench = reference_benchmark(n, args.steps)
     print("reference benchmark: %s" % json.dumps(refbench), file=sys.stderr, flush=True)
     batched = None
@@ -376,6 +418,7 @@ def main():
                    "hbm_roofline_pct": round(100 * achieved / HBM_PEAK_GBS, 2),
                    "stream_copy_gbs": round(copy_gbs, 1),
                    "storage16": storage16,
+                   "reduce_and_host_copy": aux,
                    "reference_benchmark": refbench,
                    "batched": batched,
                    "himeno": himeno,
