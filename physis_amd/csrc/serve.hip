@@ -5,20 +5,25 @@
 // request's grids resident on the device and advances all of them in
 // "ticks" of a fixed number of steps:
 //   - arrived requests wait in FIFO order and are admitted as soon as their
-//     grids fit the HBM budget (memory is returned when a request retires);
+//     grids fit the HBM budget;
 //   - all active diffusion requests of one storage kind advance together in
 //     one batched launch per step (batched.hip) on the main stream, while the
 //     Jacobi requests sweep on a pool of side streams (request i on stream
 //     i mod 4), so the two kinds, and Jacobi requests among themselves,
-//     overlap.  Jacobi requests run several sweeps per HBM pass (temporal
-//     blocking, himeno19.hip) with a launch shape an in-process tuner picks
-//     per (storage, size class) by timing the candidates on the first
+//     overlap.  Jacobi requests run with a launch shape (one sweep per
+//     launch, or several per HBM pass, himeno19.hip) an in-process tuner
+//     picks per (storage, size class) by timing the candidates on the first
 //     admitted request of that class;
-//   - after each tick every request's residual is computed on the device
-//     (diffusion: ||u_n - u_{n-1}|| / ||u_n||, FP64; Jacobi: rms of ss over
-//     the interior from the fused FP64 epilogue) and finished requests
-//     (residual <= tol, or max_steps) retire: an FP64 checksum of the final
-//     field is taken and the grids are freed.
+//   - each diffusion batch and each Jacobi request starts its next tick as
+//     soon as its previous one has completed (polled with events), so small
+//     requests are not held to the pace of large ones; at the end of a tick
+//     the request's residual has been computed on the device (diffusion:
+//     ||u_n - u_{n-1}|| / ||u_n||, FP64; Jacobi: rms of ss over the interior
+//     from the fused FP64 epilogue) and finished requests (residual <= tol,
+//     or max_steps) retire: an FP64 checksum of the final field is taken.
+//     Their memory stays counted against the budget until it is freed, which
+//     happens when admission needs it or at the end of the trace (freeing
+//     device memory waits for the whole device).
 // Every request runs the same number of steps and the same arithmetic alone
 // or in any mix (batching does not change per-grid results, reductions have
 // a fixed order), so results are bitwise independent of the mix.
@@ -37,6 +42,7 @@
 #include <chrono>
 #include <cmath>
 #include <deque>
+#include <list>
 #include <map>
 #include <thread>
 #include <vector>
@@ -102,6 +108,7 @@ void DiffNorm(const __PSGrid *a, const __PSGrid *b, double *partials, double *ou
 const float kDiffCoef[7] = {0.1f, 0.1f, 0.1f, 0.1f, 0.1f, 0.1f, 0.4f};
 constexpr float kOmega = 0.8f;
 constexpr int kSideStreams = 4;
+constexpr size_t kLargeJacobiBytes = size_t(256) << 20;
 
 // A launch shape of the Jacobi sweeps: degree 0 is one sweep per launch on
 // the default tile (HimenoEnqueue); degree >= 1 the fused kernel with that
@@ -140,6 +147,12 @@ namespace {
 
 struct Live {
   int idx = -1;
+  bool inflight = false;      // a tick has been enqueued and not yet seen complete
+  bool started = false;       // Jacobi: first tick enqueued (it waits for `init`)
+  bool retired = false;
+  hipStream_t js = nullptr;   // Jacobi: side stream
+  hipEvent_t ev = nullptr;    // Jacobi: end of the in-flight tick
+  hipEvent_t init = nullptr;  // Jacobi: end of the initial fills on the main stream
   const PSHipRequest *rq = nullptr;
   __PSGrid *A = nullptr, *B = nullptr;  // diffusion u_n, u_{n-1} / Jacobi p0, p1
   void *aux = nullptr;                  // third buffer (fused steps / sweeps)
@@ -172,6 +185,9 @@ class Server {
   }
   ~Server() {
     for (auto &b : batches_) (void)PSHipBatchDestroy(b.second.handle);
+    for (void *h : dead_batches_) (void)PSHipBatchDestroy(h);
+    for (auto &g : groups_)
+      if (g.ev) (void)hipEventDestroy(g.ev);
     for (auto &st : side_) {
       (void)hipStreamSynchronize(st);
       (void)hipStreamDestroy(st);
@@ -181,10 +197,14 @@ class Server {
   }
 
   int Run(bool sequential);
+  int RunContinuous();
 
  private:
   int Admit(int idx, double now, hipStream_t s, bool batched);
-  void Retire(Live &lv, double now);
+  void Retire(Live &lv, double now, hipStream_t s);
+  void FreeRetired();
+  void Finish(Live &lv);  // residual and step count after a completed tick
+  void LaunchDiffusion(int storage, std::vector<Live *> &members);
   void AdvanceTick(std::vector<Live *> &act, bool batched);
   void Release(Live &lv);
   JacobiShape TuneJacobi(Live &lv, hipStream_t s);
@@ -203,7 +223,16 @@ class Server {
   hipEvent_t ready_ = nullptr;  // orders side-stream work after the main stream
   double *host_ = nullptr;
   int64_t used_ = 0;
-  std::vector<Live> live_;
+  std::vector<Live> live_;               // sequential mode
+  std::list<Live> clive_;                // continuous mode (stable addresses)
+  std::vector<Live> retired_;            // allocations awaiting release
+  std::vector<void *> dead_batches_;     // replaced batch handles awaiting release
+  struct Group {
+    bool inflight = false;
+    hipEvent_t ev = nullptr;
+    std::vector<Live *> members;
+  };
+  Group groups_[2];                      // continuous-mode diffusion batches per storage kind
   std::map<int, BatchEntry> batches_;  // key: storage kind
   std::chrono::steady_clock::time_point t0_;
 };
@@ -316,20 +345,25 @@ void Server::Release(Live &lv) {
   if (lv.dev) (void)hipFree(lv.dev);
   lv.aux = nullptr;
   lv.dev = nullptr;
+  for (hipEvent_t *e : {&lv.ev, &lv.init}) {
+    if (*e) (void)hipEventDestroy(*e);
+    *e = nullptr;
+  }
 }
 
-void Server::Retire(Live &lv, double now) {
+void Server::Retire(Live &lv, double now, hipStream_t s) {
   // Checksum of the final field (A holds u_n for diffusion; after an even
-  // number of sweeps p0 holds the Jacobi field).
+  // number of sweeps p0 holds the Jacobi field), on the stream that computed
+  // it.
   double *res = lv.dev;
-  CheckHip((hipError_t)(ReduceAsync(lv.A->p0, lv.A->num_elms, 4, Rec(lv.A)->storage, PS_SUM, res, nullptr, main_) ==
+  CheckHip((hipError_t)(ReduceAsync(lv.A->p0, lv.A->num_elms, 4, Rec(lv.A)->storage, PS_SUM, res, nullptr, s) ==
                                 PS_HIP_OK
                             ? hipSuccess
                             : hipErrorUnknown),
            "checksum");
   double *slot = host_ + 2 * (size_t)n_;  // past the per-request residual slots
-  CheckHip(hipMemcpyAsync(slot, res, sizeof(double), hipMemcpyDeviceToHost, main_), "checksum copy");
-  CheckHip(hipStreamSynchronize(main_), "checksum sync");
+  CheckHip(hipMemcpyAsync(slot, res, sizeof(double), hipMemcpyDeviceToHost, s), "checksum copy");
+  CheckHip(hipStreamSynchronize(s), "checksum sync");
   PSHipResult &o = out_[lv.idx];
   o.status = PS_HIP_OK;
   o.steps = lv.steps;
@@ -337,8 +371,37 @@ void Server::Retire(Live &lv, double now) {
   o.admitted = lv.admitted;
   o.finished = now;
   o.checksum = *slot;
-  used_ -= (int64_t)lv.bytes;
-  Release(lv);
+  lv.retired = true;
+  // Freed later (FreeRetired): its bytes stay in used_ until then.
+  retired_.push_back(lv);
+  lv.A = lv.B = nullptr;
+  for (auto &c : lv.coef) c = nullptr;
+  lv.aux = nullptr;
+  lv.dev = nullptr;
+  lv.ev = lv.init = nullptr;
+}
+
+void Server::FreeRetired() {
+  if (retired_.empty() && dead_batches_.empty()) return;
+  CheckHip(hipDeviceSynchronize(), "device sync");
+  for (Live &lv : retired_) {
+    used_ -= (int64_t)lv.bytes;
+    Release(lv);
+  }
+  retired_.clear();
+  for (void *h : dead_batches_) (void)PSHipBatchDestroy(h);
+  dead_batches_.clear();
+}
+
+void Server::Finish(Live &lv) {
+  lv.steps += tick_;
+  const double *h = host_ + 2 * lv.idx;
+  if (lv.rq->kind == 0) {
+    lv.residual = h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0;
+  } else {
+    const double interior = std::pow((double)(lv.rq->n - 2), 3.0);
+    lv.residual = std::sqrt(h[0] / interior);
+  }
 }
 
 void Server::AdvanceTick(std::vector<Live *> &act, bool batched) {
@@ -408,16 +471,173 @@ void Server::AdvanceTick(std::vector<Live *> &act, bool batched) {
   }
   CheckHip(hipStreamSynchronize(ds), "tick sync");
   for (hipStream_t js : used) CheckHip(hipStreamSynchronize(js), "tick sync");
-  for (Live *lv : act) {
-    lv->steps += tick_;
-    const double *h = host_ + 2 * lv->idx;
-    if (lv->rq->kind == 0) {
-      lv->residual = h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0;
-    } else {
-      const double interior = std::pow((double)(lv->rq->n - 2), 3.0);
-      lv->residual = std::sqrt(h[0] / interior);
+  for (Live *lv : act) Finish(*lv);
+}
+
+// One tick of the diffusion batch of one storage kind on the main stream,
+// with each member's residual inputs copied to its host slots; the batch is
+// rebuilt when its membership changed (the old handle is released later).
+void Server::LaunchDiffusion(int storage, std::vector<Live *> &v) {
+  std::vector<int> ids;
+  for (Live *lv : v) ids.push_back(lv->idx);
+  BatchEntry &be = batches_[storage];
+  if (be.ids != ids) {
+    if (be.handle) dead_batches_.push_back(be.handle);
+    be.handle = nullptr;
+    std::vector<__PSGrid *> a, b;
+    std::vector<void *> aux;
+    std::vector<float> coef;
+    for (Live *lv : v) {
+      a.push_back(lv->A);
+      b.push_back(lv->B);
+      aux.push_back(lv->aux);
+      coef.insert(coef.end(), kDiffCoef, kDiffCoef + 7);
+    }
+    if (PSHipBatchCreateEx(&be.handle, a.data(), b.data(), aux.data(), (int)v.size(), coef.data(),
+                           PS_HIP_CLAMPED) != PS_HIP_OK)
+      Fatal("serve", "batch creation failed");
+    be.ids = ids;
+  }
+  if (PSHipBatchRunOn(be.handle, tick_, main_) != PS_HIP_OK) Fatal("serve", "batched run failed");
+  for (Live *lv : v) DiffNorm(lv->A, lv->B, lv->dev + 2, lv->dev, main_);
+  for (Live *lv : v)
+    CheckHip(hipMemcpyAsync(host_ + 2 * lv->idx, lv->dev, 2 * sizeof(double), hipMemcpyDeviceToHost, main_),
+             "norm copy");
+}
+
+// Continuous batching (mode 0): admission as in Run, then every diffusion
+// batch and every Jacobi request starts its next tick as soon as its last
+// one has completed.
+int Server::RunContinuous() {
+  t0_ = std::chrono::steady_clock::now();
+  for (int i = 0; i < n_; ++i) out_[i] = PSHipResult{PS_HIP_ERR_INVALID, 0, 0.0, 0.0, 0.0, 0.0};
+  for (auto &g : groups_) CheckHip(hipEventCreateWithFlags(&g.ev, hipEventDisableTiming), "hipEventCreate");
+  std::vector<int> order(n_);
+  for (int i = 0; i < n_; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return reqs_[a].arrival < reqs_[b].arrival; });
+  size_t next = 0;
+  std::deque<int> waiting;
+  int done = 0;
+  auto finished = [](const Live &lv) { return lv.residual <= lv.rq->tol || lv.steps >= lv.rq->max_steps; };
+  while (done < n_) {
+    const double now = Seconds(t0_);
+    while (next < order.size() && reqs_[order[next]].arrival <= now) waiting.push_back(order[next++]);
+    // Admission by HBM budget, FIFO; retired requests' memory is released
+    // first when the budget needs it.
+    while (!waiting.empty()) {
+      const int idx = waiting.front();
+      const int64_t need = (int64_t)RequestBytes(reqs_[idx]);
+      if (need > budget_) {
+        out_[idx].status = PS_HIP_ERR_NOMEM;  // can never fit
+        waiting.pop_front();
+        ++done;
+        continue;
+      }
+      if (used_ + need > budget_) {
+        if (!retired_.empty()) {
+          FreeRetired();
+          continue;
+        }
+        break;
+      }
+      const int st = Admit(idx, Seconds(t0_), main_, true);
+      if (st == PS_HIP_ERR_NOMEM && (!clive_.empty() || !retired_.empty())) {
+        FreeRetired();
+        if (!clive_.empty()) break;  // wait for memory to free up
+        continue;
+      }
+      waiting.pop_front();
+      if (st != PS_HIP_OK) {
+        out_[idx].status = st;
+        ++done;
+        continue;
+      }
+      clive_.push_back(live_.back());
+      live_.pop_back();
+      Live &lv = clive_.back();
+      if (lv.rq->kind == 1) {
+        lv.js = side_[lv.idx % kSideStreams];
+        CheckHip(hipEventCreateWithFlags(&lv.ev, hipEventDisableTiming), "hipEventCreate");
+        CheckHip(hipEventCreateWithFlags(&lv.init, hipEventDisableTiming), "hipEventCreate");
+        CheckHip(hipEventRecord(lv.init, main_), "record");
+      }
+    }
+    bool progressed = false;
+    // Diffusion batches, one per storage kind.
+    for (int sk = 0; sk < 2; ++sk) {
+      Group &g = groups_[sk];
+      if (g.inflight) {
+        const hipError_t q = hipEventQuery(g.ev);
+        if (q == hipErrorNotReady) continue;
+        CheckHip(q, "event query");
+        g.inflight = false;
+        progressed = true;
+        const double t = Seconds(t0_);
+        for (Live *lv : g.members) {
+          Finish(*lv);
+          if (finished(*lv)) {
+            Retire(*lv, t, main_);
+            ++done;
+          }
+        }
+      }
+      std::vector<Live *> members;
+      for (Live &lv : clive_)
+        if (!lv.retired && lv.rq->kind == 0 && lv.rq->storage == sk) members.push_back(&lv);
+      if (members.empty()) continue;
+      LaunchDiffusion(sk, members);
+      CheckHip(hipEventRecord(g.ev, main_), "record");
+      g.members = members;
+      g.inflight = true;
+    }
+    // Jacobi requests, each on its side stream.  Large ones (grids of
+    // kLargeJacobiBytes or more) sweep one at a time in arrival order: each
+    // one fills the device alone, and sharing it among them only delays the
+    // earliest (the trace's worst latency) without finishing the set sooner.
+    // Small ones and the diffusion batches overlap with them.
+    const Live *large_head = nullptr;
+    for (const Live &lv : clive_)
+      if (!lv.retired && lv.rq->kind == 1 && lv.bytes >= kLargeJacobiBytes) {
+        large_head = &lv;
+        break;
+      }
+    for (Live &lv : clive_) {
+      if (lv.retired || lv.rq->kind != 1) continue;
+      if (lv.inflight) {
+        const hipError_t q = hipEventQuery(lv.ev);
+        if (q == hipErrorNotReady) continue;
+        CheckHip(q, "event query");
+        lv.inflight = false;
+        progressed = true;
+        Finish(lv);
+        if (finished(lv)) {
+          Retire(lv, Seconds(t0_), lv.js);
+          ++done;
+          continue;
+        }
+      }
+      if (lv.bytes >= kLargeJacobiBytes && &lv != large_head) continue;
+      if (!lv.started) {
+        CheckHip(hipStreamWaitEvent(lv.js, lv.init, 0), "wait");
+        lv.started = true;
+      }
+      double *gosa = lv.dev;
+      double *partials = lv.dev + std::max(tick_, kTuneSweeps);
+      if (EnqueueJacobi(lv.shape, lv.A, lv.B, lv.aux, lv.coef, tick_, gosa, partials, lv.js) != PS_HIP_OK)
+        Fatal("serve", "jacobi run failed");
+      CheckHip(hipMemcpyAsync(host_ + 2 * lv.idx, gosa + tick_ - 1, sizeof(double), hipMemcpyDeviceToHost, lv.js),
+               "gosa copy");
+      CheckHip(hipEventRecord(lv.ev, lv.js), "record");
+      lv.inflight = true;
+    }
+    clive_.remove_if([](const Live &lv) { return lv.retired; });
+    if (!progressed) {
+      if (clive_.empty() && next >= order.size() && waiting.empty()) break;
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
   }
+  FreeRetired();
+  return PS_HIP_OK;
 }
 
 int Server::Run(bool sequential) {
@@ -466,7 +686,8 @@ int Server::Run(bool sequential) {
     std::vector<Live> keep;
     for (Live &lv : live_) {
       if (lv.residual <= lv.rq->tol || lv.steps >= lv.rq->max_steps) {
-        Retire(lv, t);
+        Retire(lv, t, main_);
+        FreeRetired();
         ++done;
       } else {
         keep.push_back(lv);
@@ -497,7 +718,7 @@ int PSHipServeTrace(const PSHipRequest *reqs, int n, int64_t budget_bytes, int t
   }
   Runtime::Get().EnsureDevice();
   Server srv(reqs, n, budget_bytes, tick_steps, results);
-  return srv.Run(mode == 1);
+  return mode == 1 ? srv.Run(true) : srv.RunContinuous();
 }
 
 }  // extern "C"

@@ -51,7 +51,9 @@ def run_hip(rt, p, coefs, omega, sweeps, storage=0, variant=0):
     return out0, out1, gosa
 
 
-@pytest.mark.parametrize("shape", [(65, 33, 33), (19, 7, 5), (3, 3, 3), (129, 17, 9)])
+# Default tile widths: 64 (65, 19, 3 wide), 128 (129 wide: 127 interior
+# points) and 256 (258 wide: 256 interior points).
+@pytest.mark.parametrize("shape", [(65, 33, 33), (19, 7, 5), (3, 3, 3), (129, 17, 9), (258, 6, 10)])
 def test_himeno_bitwise_vs_oracle(rt, shape):
     p, coefs = himeno_inputs(shape, seed=100)
     want0, want1, wgosa = O.himeno(p, coefs, 0.8, 6)
