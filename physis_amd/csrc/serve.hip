@@ -179,7 +179,6 @@ class Server {
     Runtime &rt = Runtime::Get();
     main_ = rt.stream();
     for (auto &st : side_) CheckHip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
-    CheckHip(hipEventCreateWithFlags(&ready_, hipEventDisableTiming), "hipEventCreate");
     CheckHip(hipHostMalloc(&host_, sizeof(double) * 4 * (size_t)std::max(n, 1), hipHostMallocDefault),
              "hipHostMalloc");
   }
@@ -192,11 +191,10 @@ class Server {
       (void)hipStreamSynchronize(st);
       (void)hipStreamDestroy(st);
     }
-    (void)hipEventDestroy(ready_);
     (void)hipHostFree(host_);
   }
 
-  int Run(bool sequential);
+  int RunSequential();
   int RunContinuous();
 
  private:
@@ -205,7 +203,7 @@ class Server {
   void FreeRetired();
   void Finish(Live &lv);  // residual and step count after a completed tick
   void LaunchDiffusion(int storage, std::vector<Live *> &members);
-  void AdvanceTick(std::vector<Live *> &act, bool batched);
+  void AdvanceTick(std::vector<Live *> &act);
   void Release(Live &lv);
   JacobiShape TuneJacobi(Live &lv, hipStream_t s);
 
@@ -220,7 +218,6 @@ class Server {
   int tick_;
   PSHipResult *out_;
   hipStream_t main_, side_[kSideStreams];
-  hipEvent_t ready_ = nullptr;  // orders side-stream work after the main stream
   double *host_ = nullptr;
   int64_t used_ = 0;
   std::vector<Live> live_;               // sequential mode
@@ -404,73 +401,27 @@ void Server::Finish(Live &lv) {
   }
 }
 
-void Server::AdvanceTick(std::vector<Live *> &act, bool batched) {
-  hipStream_t ds = main_;
-  // Diffusion: one batch per storage kind (rebuilt when its membership changes).
-  std::map<int, std::vector<Live *>> groups;
-  for (Live *lv : act)
-    if (lv->rq->kind == 0) groups[lv->rq->storage].push_back(lv);
-  for (auto &g : groups) {
-    std::vector<Live *> &v = g.second;
-    if (batched) {
-      std::vector<int> ids;
-      for (Live *lv : v) ids.push_back(lv->idx);
-      BatchEntry &be = batches_[g.first];
-      if (be.ids != ids) {
-        if (be.handle) PSHipBatchDestroy(be.handle);
-        std::vector<__PSGrid *> a, b;
-        std::vector<void *> aux;
-        std::vector<float> coef;
-        for (Live *lv : v) {
-          a.push_back(lv->A);
-          b.push_back(lv->B);
-          aux.push_back(lv->aux);
-          coef.insert(coef.end(), kDiffCoef, kDiffCoef + 7);
-        }
-        if (PSHipBatchCreateEx(&be.handle, a.data(), b.data(), aux.data(), (int)v.size(), coef.data(),
-                               PS_HIP_CLAMPED) != PS_HIP_OK)
-          Fatal("serve", "batch creation failed");
-        be.ids = ids;
-      }
-      if (PSHipBatchRunOn(be.handle, tick_, ds) != PS_HIP_OK) Fatal("serve", "batched run failed");
-    } else {
-      for (Live *lv : v) {
-        __PSDomain dom = PSDomain3DNew(0, lv->rq->n, 0, lv->rq->n, 0, lv->rq->n);
-        if (PSHipDiffusion7Run(lv->A, lv->B, &dom, kDiffCoef, PS_HIP_CLAMPED, tick_, 0) != PS_HIP_OK)
-          Fatal("serve", "diffusion run failed");
-      }
-    }
-    for (Live *lv : v) DiffNorm(lv->A, lv->B, lv->dev + 2, lv->dev, ds);
-    for (Live *lv : v)
-      CheckHip(hipMemcpyAsync(host_ + 2 * lv->idx, lv->dev, 2 * sizeof(double), hipMemcpyDeviceToHost, ds),
-               "norm copy");
-  }
-  // Jacobi requests: fused sweeps on the side streams (batched mode) or
-  // single sweeps one after another on the main stream (sequential mode).
-  std::vector<hipStream_t> used;
+// One tick of the request being served in sequential mode (mode 1), through
+// the plain single-grid paths on the main stream, then a sync.
+void Server::AdvanceTick(std::vector<Live *> &act) {
   for (Live *lv : act) {
-    if (lv->rq->kind != 1) continue;
-    double *gosa = lv->dev;
-    double *partials = lv->dev + std::max(tick_, kTuneSweeps);
-    hipStream_t js = batched ? side_[lv->idx % kSideStreams] : main_;
-    if (batched) {
-      // The side stream starts after the main stream's pending work (the
-      // request's initial fills).
-      if (std::find(used.begin(), used.end(), js) == used.end()) {
-        CheckHip(hipEventRecord(ready_, main_), "record");
-        CheckHip(hipStreamWaitEvent(js, ready_, 0), "wait");
-        used.push_back(js);
-      }
-      if (EnqueueJacobi(lv->shape, lv->A, lv->B, lv->aux, lv->coef, tick_, gosa, partials, js) != PS_HIP_OK)
+    if (lv->rq->kind == 0) {
+      __PSDomain dom = PSDomain3DNew(0, lv->rq->n, 0, lv->rq->n, 0, lv->rq->n);
+      if (PSHipDiffusion7Run(lv->A, lv->B, &dom, kDiffCoef, PS_HIP_CLAMPED, tick_, 0) != PS_HIP_OK)
+        Fatal("serve", "diffusion run failed");
+      DiffNorm(lv->A, lv->B, lv->dev + 2, lv->dev, main_);
+      CheckHip(hipMemcpyAsync(host_ + 2 * lv->idx, lv->dev, 2 * sizeof(double), hipMemcpyDeviceToHost, main_),
+               "norm copy");
+    } else {
+      double *gosa = lv->dev;
+      double *partials = lv->dev + std::max(tick_, kTuneSweeps);
+      if (HimenoEnqueue(lv->A, lv->B, lv->coef, kOmega, tick_, gosa, partials, main_) != PS_HIP_OK)
         Fatal("serve", "jacobi run failed");
-    } else if (HimenoEnqueue(lv->A, lv->B, lv->coef, kOmega, tick_, gosa, partials, js) != PS_HIP_OK) {
-      Fatal("serve", "jacobi run failed");
+      CheckHip(hipMemcpyAsync(host_ + 2 * lv->idx, gosa + tick_ - 1, sizeof(double), hipMemcpyDeviceToHost, main_),
+               "gosa copy");
     }
-    CheckHip(hipMemcpyAsync(host_ + 2 * lv->idx, gosa + tick_ - 1, sizeof(double), hipMemcpyDeviceToHost, js),
-             "gosa copy");
   }
-  CheckHip(hipStreamSynchronize(ds), "tick sync");
-  for (hipStream_t js : used) CheckHip(hipStreamSynchronize(js), "tick sync");
+  CheckHip(hipStreamSynchronize(main_), "tick sync");
   for (Live *lv : act) Finish(*lv);
 }
 
@@ -640,7 +591,9 @@ int Server::RunContinuous() {
   return PS_HIP_OK;
 }
 
-int Server::Run(bool sequential) {
+// Sequential mode (mode 1): one request at a time, admitted in arrival
+// order, one tick then a sync.
+int Server::RunSequential() {
   t0_ = std::chrono::steady_clock::now();
   for (int i = 0; i < n_; ++i) out_[i] = PSHipResult{PS_HIP_ERR_INVALID, 0, 0.0, 0.0, 0.0, 0.0};
   std::vector<int> order(n_);
@@ -653,10 +606,10 @@ int Server::Run(bool sequential) {
   while (done < n_) {
     const double now = Seconds(t0_);
     while (next < order.size() && reqs_[order[next]].arrival <= now) waiting.push_back(order[next++]);
-    // Admission by HBM budget, FIFO (sequential mode: one at a time).
+    // Admission in arrival order, one request at a time.
     while (!waiting.empty()) {
       const int idx = waiting.front();
-      if (sequential && !live_.empty()) break;
+      if (!live_.empty()) break;
       const int64_t need = (int64_t)RequestBytes(reqs_[idx]);
       if (need > budget_) {
         out_[idx].status = PS_HIP_ERR_NOMEM;  // can never fit
@@ -665,7 +618,7 @@ int Server::Run(bool sequential) {
         continue;
       }
       if (used_ + need > budget_) break;
-      const int st = Admit(idx, Seconds(t0_), main_, !sequential);
+      const int st = Admit(idx, Seconds(t0_), main_, false);
       if (st == PS_HIP_ERR_NOMEM && !live_.empty()) break;  // wait for memory to free up
       waiting.pop_front();
       if (st != PS_HIP_OK) {
@@ -680,7 +633,7 @@ int Server::Run(bool sequential) {
     }
     std::vector<Live *> act;
     for (Live &lv : live_) act.push_back(&lv);
-    AdvanceTick(act, !sequential);
+    AdvanceTick(act);
     // Retire finished requests.
     const double t = Seconds(t0_);
     std::vector<Live> keep;
@@ -718,7 +671,7 @@ int PSHipServeTrace(const PSHipRequest *reqs, int n, int64_t budget_bytes, int t
   }
   Runtime::Get().EnsureDevice();
   Server srv(reqs, n, budget_bytes, tick_steps, results);
-  return mode == 1 ? srv.Run(true) : srv.RunContinuous();
+  return mode == 1 ? srv.RunSequential() : srv.RunContinuous();
 }
 
 }  // extern "C"
