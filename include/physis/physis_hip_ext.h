@@ -188,9 +188,12 @@ typedef struct {
 } PSHipResult;
 
 /* Serves the requests in real time.  mode 0: continuous batching under an
- * HBM budget with batched diffusion and overlapped Jacobi streams; mode 1:
- * one request at a time through the single-grid paths.  tick_steps (even):
- * steps between residual checks. */
+ * HBM budget: diffusion requests of one storage kind advance together in
+ * batched launches, Jacobi requests on side streams (those with grids of
+ * 256 MiB or more one at a time in arrival order), each batch and request
+ * starting its next tick as soon as its previous one completes; mode 1: one
+ * request at a time through the single-grid paths.  tick_steps (even):
+ * steps between residual checks.  Results are the same in both modes. */
 extern int PSHipServeTrace(const PSHipRequest *reqs, int n, int64_t budget_bytes, int tick_steps, int mode,
                            PSHipResult *results);
 

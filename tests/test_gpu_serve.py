@@ -96,3 +96,26 @@ def test_fp32_results_match_oracle(rt, mixed):
             p0, _, _ = O.himeno(p, coefs, 0.8, steps)
             want = float(np.sum(p0.astype(np.float64)))
         assert abs(r["checksum"] - want) <= 1e-12 * max(1.0, abs(want)), (q, r["checksum"], want)
+
+
+def test_large_jacobi_requests_run_in_arrival_order_and_small_work_overlaps(rt):
+    """Large Jacobi requests (grids of 256 MiB or more: 15 FP32 grids of
+    176^3) sweep one at a time in arrival order, so the earlier one finishes
+    first; a small Jacobi request and a diffusion request that arrive with
+    them are served alongside and finish before either.  Results are the
+    same as served alone."""
+    reqs = [
+        dict(kind=1, n=176, storage=0, seed=31, arrival=0.000, max_steps=400, tol=0.0),
+        dict(kind=1, n=176, storage=0, seed=32, arrival=0.001, max_steps=400, tol=0.0),
+        dict(kind=1, n=40, storage=0, seed=33, arrival=0.002, max_steps=100, tol=0.0),
+        dict(kind=0, n=48, storage=0, seed=34, arrival=0.002, max_steps=100, tol=0.0),
+    ]
+    res = rt.serve_trace(reqs, BIG_BUDGET, TICK, rt.SERVE_BATCHED)
+    assert all(r["status"] == 0 for r in res)
+    assert [r["steps"] for r in res] == [400, 400, 100, 100]
+    print("finished", [round(r["finished"], 4) for r in res], flush=True)
+    assert res[0]["finished"] < res[1]["finished"]
+    assert max(res[2]["finished"], res[3]["finished"]) < res[0]["finished"]
+    for q, r in zip(reqs, res):
+        alone = rt.serve_trace([dict(q, arrival=0.0)], BIG_BUDGET, TICK, rt.SERVE_BATCHED)[0]
+        assert key(alone) == key(r), q
