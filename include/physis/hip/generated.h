@@ -108,11 +108,8 @@ inline hipError_t LaunchStarStep(const typename P::Args &a, const Box &box, int 
 template <class P>
 inline bool LaunchFusedPair(const typename P::Args &a1, const typename P::Args &a2, int nx, int ny, int nz,
                             hipStream_t s) {
-  hipError_t e;
-  if (StarTB2Supported<1>(nx, ny, nz)) e = LaunchZMarchStarTB2<P, 1, 16>(a1, a2, ny, nz, 32, true, s);
-  else if (StarTB2Supported<2>(nx, ny, nz)) e = LaunchZMarchStarTB2<P, 2, 8>(a1, a2, ny, nz, 32, true, s);
-  else if (StarTB2Supported<4>(nx, ny, nz)) e = LaunchZMarchStarTB2<P, 4, 4>(a1, a2, ny, nz, 32, true, s);
-  else return false;
+  const hipError_t e = LaunchZMarchStarTB2Auto<P>(a1, a2, nx, ny, nz, s);
+  if (e == hipErrorNotSupported) return false;
   CheckLaunch(e, "fused pair");
   return true;
 }

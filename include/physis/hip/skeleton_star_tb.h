@@ -197,6 +197,28 @@ inline bool StarTB2Supported(int nx, int ny, int nz) {
   return nx % 4 == 0 && nx <= 256 * NWX && nx >= 4 && ny >= 1 && nz >= 1;
 }
 
+// Launch shape of the fused kernel for a grid: NWX = ceil(nx / 256) waves
+// per row (whole rows per workgroup, so fewer idle lanes than a fixed
+// width), 8 rows per workgroup (4 with four waves per row), and 16-plane
+// z-chunks, or 8 when 16 leaves fewer than kTB2Waves waves to fill the
+// device.  Timed on MI355X (GLUPS, clamped): 128^3 258, 192^3 629, 256^3
+// 905, 384^3 966, 512^3 1196, each the best of 14 shapes (a fixed
+// 2 x 8 x 32-plane shape: 586 at 256^3, 946 at 384^3).
+struct TB2Shape {
+  int nwx, by, zchunk;
+};
+constexpr int kTB2Waves = 4096;
+
+inline TB2Shape ChooseTB2Shape(int nx, int ny, int nz) {
+  TB2Shape sh;
+  sh.nwx = nx <= 256 ? 1 : (nx <= 512 ? 2 : 4);
+  sh.by = sh.nwx == 4 ? 4 : 8;
+  const int tiles_y = (ny + sh.by - 3) / (sh.by - 2);
+  const int64_t waves16 = (int64_t)tiles_y * ((nz + 15) / 16) * sh.nwx * sh.by;
+  sh.zchunk = waves16 >= kTB2Waves ? 16 : 8;
+  return sh;
+}
+
 template <class Plan, int NWX, int BY>
 inline hipError_t LaunchZMarchStarTB2(const typename Plan::Args &a1, const typename Plan::Args &a2, int ny,
                                       int nz, int zchunk, bool remap, hipStream_t stream) {
@@ -208,6 +230,21 @@ inline hipError_t LaunchZMarchStarTB2(const typename Plan::Args &a1, const typen
   dim3 grid(geo.tiles_y, (nz + zchunk - 1) / zchunk, 1);
   hipLaunchKernelGGL((ZMarchStar4TB2<Plan, NWX, BY>), grid, dim3(64 * NWX, BY, 1), 0, stream, a1, a2, geo);
   return hipGetLastError();
+}
+
+// Two fused steps with the shape ChooseTB2Shape picks; hipErrorNotSupported
+// when the rows are not whole 16-byte vectors or wider than 1024 points.
+template <class Plan>
+inline hipError_t LaunchZMarchStarTB2Auto(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx,
+                                          int ny, int nz, hipStream_t stream) {
+  const TB2Shape sh = ChooseTB2Shape(nx, ny, nz);
+  if (sh.nwx == 1 && StarTB2Supported<1>(nx, ny, nz))
+    return LaunchZMarchStarTB2<Plan, 1, 8>(a1, a2, ny, nz, sh.zchunk, true, stream);
+  if (sh.nwx == 2 && StarTB2Supported<2>(nx, ny, nz))
+    return LaunchZMarchStarTB2<Plan, 2, 8>(a1, a2, ny, nz, sh.zchunk, true, stream);
+  if (sh.nwx == 4 && StarTB2Supported<4>(nx, ny, nz))
+    return LaunchZMarchStarTB2<Plan, 4, 4>(a1, a2, ny, nz, sh.zchunk, true, stream);
+  return hipErrorNotSupported;
 }
 
 }  // namespace psk

@@ -97,7 +97,9 @@ extern int PSHipTimerElapsed(float *ms);
  * (boundary = PS_HIP_PERIODIC).  coef = {ce, cw, cn, cs, ct, cb, cc}.
  * Runs `steps` steps alternating g1->g2 and g2->g1 (PSStencilRun with two
  * maps and count steps/2; an odd count ends with one g1->g2 map).
- * variant: 0 = auto-tuned default, otherwise a kernel id from
+ * variant: 0 = two fused steps per pass with the launch shape chosen from
+ * the grid size (waves per row from nx, z-chunk to fill the device; one step
+ * per pass where rows are not whole 16-byte vectors), otherwise a kernel id from
  * PSHipDiffusion7Variants. */
 enum PSHipBoundary { PS_HIP_CLAMPED = 0, PS_HIP_PERIODIC = 1 };
 extern int PSHipDiffusion7Run(__PSGrid *g1, __PSGrid *g2,

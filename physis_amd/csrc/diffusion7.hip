@@ -139,11 +139,17 @@ const Variant kVariants[] = {
     {"skeleton64x4r4_z64", 1, 64, 4, 64, true, 4},
     {"skeleton64x8r2_z64", 1, 64, 8, 64, true, 2},
     {"skeleton64x2r4_z64", 1, 64, 2, 64, true, 4},
+    {"tb2_w1_by8_z16", 4, 1, 8, 16, true},
+    {"tb2_w1_by8_z8", 4, 1, 8, 8, true},
+    {"tb2_w1_by16_z16", 4, 1, 16, 16, true},
+    {"tb2_w1_by16_z8", 4, 1, 16, 8, true},
+    {"tb2_w2_by8_z8", 4, 2, 8, 8, true},
+    {"tb2_w2_by6_z16", 4, 2, 6, 16, true},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
-// Measured on MI355X at 512^3 FP32: star4_by8_z128 701 GLUPS one step per
-// pass; tb2_w2_by8_z32 ~1100 GLUPS two steps per pass (bench.py --sweep).
-constexpr int kDefaultVariant = 17;
+// Measured on MI355X at 512^3 FP32 (bench.py --sweep): star4_by8_z128 ~710
+// GLUPS one step per pass; two steps per pass ~1190 GLUPS with the shape of
+// psk::ChooseTB2Shape, which variant 0 uses for every grid size.
 // One-step variant used by family-4 runs for unfused steps.
 constexpr int kStarVariant = 14;
 
@@ -298,9 +304,12 @@ int PSHipDiffusion7Run(__PSGrid *g1, __PSGrid *g2, const __PSDomain *dom, const 
     if (dom->local_min[d] < 0 || dom->local_max[d] > g1->dim[d]) return PS_HIP_ERR_INVALID;
   }
   if (Rec(g1)->storage != Rec(g2)->storage) return PS_HIP_ERR_INVALID;
-  if (variant == 0) variant = kDefaultVariant;
-  if (variant < 1 || variant > kNumVariants) return PS_HIP_ERR_INVALID;
-  const Variant &v = kVariants[variant - 1];
+  if (variant < 0 || variant > kNumVariants) return PS_HIP_ERR_INVALID;
+  // Variant 0: two fused steps per pass with the shape psk::ChooseTB2Shape
+  // picks for this grid.
+  const psk::TB2Shape sh = psk::ChooseTB2Shape(g1->dim[0], g1->dim[1], g1->dim[2]);
+  const Variant autov{"tb2_auto", 4, sh.nwx, sh.by, sh.zchunk, true};
+  const Variant &v = variant == 0 ? autov : kVariants[variant - 1];
   Coef7 k{coef[0], coef[1], coef[2], coef[3], coef[4], coef[5], coef[6]};
   psk::Box box = psk::BoxFromDomain(*dom, 3);
   hipStream_t s = Runtime::Get().stream();

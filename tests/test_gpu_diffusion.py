@@ -32,7 +32,9 @@ def run_hip(rt, field, coef, boundary, steps, variant=0, dom=None, storage=0):
 
 
 def variant_ids(rt):
-    return list(rt.diffusion7_variants().keys())
+    """0 (the default: fused steps with a shape chosen from the grid size)
+    and every fixed kernel variant."""
+    return [0] + list(rt.diffusion7_variants().keys())
 
 
 @pytest.mark.parametrize("boundary", [0, 1])
@@ -40,7 +42,7 @@ def test_64cubed_100_steps_bitwise_all_variants(rt, boundary):
     field = O.uniform_field((64, 64, 64), seed=1)
     want1, want2 = O.diffusion7(field, K_BENCH, periodic=bool(boundary), steps=100)
     for v in variant_ids(rt):
-        print("variant", v, rt.diffusion7_variants()[v], flush=True)
+        print("variant", v, rt.diffusion7_variants().get(v, "default"), flush=True)
         got1, got2 = run_hip(rt, field, K_BENCH, boundary, 100, variant=v)
         assert np.array_equal(got1, want1), "variant %d" % v
         assert np.array_equal(got2, want2), "variant %d" % v
