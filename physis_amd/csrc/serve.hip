@@ -109,6 +109,7 @@ const float kDiffCoef[7] = {0.1f, 0.1f, 0.1f, 0.1f, 0.1f, 0.1f, 0.4f};
 constexpr float kOmega = 0.8f;
 constexpr int kSideStreams = 4;
 constexpr size_t kLargeJacobiBytes = size_t(256) << 20;
+constexpr size_t kMaxDeadBatches = 64;
 
 // A launch shape of the Jacobi sweeps: degree 0 is one sweep per launch on
 // the default tile (HimenoEnqueue); degree >= 1 the fused kernel with that
@@ -582,6 +583,9 @@ int Server::RunContinuous() {
       lv.inflight = true;
     }
     clive_.remove_if([](const Live &lv) { return lv.retired; });
+    // Replaced batch handles hold device work lists; release them in bulk
+    // before they accumulate on a long trace.
+    if (dead_batches_.size() >= kMaxDeadBatches) FreeRetired();
     if (!progressed) {
       if (clive_.empty() && next >= order.size() && waiting.empty()) break;
       std::this_thread::sleep_for(std::chrono::microseconds(20));
