@@ -199,23 +199,33 @@ inline bool StarTB2Supported(int nx, int ny, int nz) {
 
 // Launch shape of the fused kernel for a grid: NWX = ceil(nx / 256) waves
 // per row (whole rows per workgroup, so fewer idle lanes than a fixed
-// width), 8 rows per workgroup (4 with four waves per row), and 16-plane
-// z-chunks, or 8 when 16 leaves fewer than kTB2Waves waves to fill the
-// device.  Timed on MI355X (GLUPS, clamped): 128^3 258, 192^3 629, 256^3
-// 905, 384^3 966, 512^3 1196, each the best of 14 shapes (a fixed
+// width), 8 rows per workgroup (4 with four waves per row), and the largest
+// z-chunk of 32, 16 and 8 planes that still gives kTB2WavesFull waves, else
+// the largest that gives kTB2Waves, else 8 (a chunk re-reads two halo
+// planes, so longer chunks move fewer bytes once the device is full).
+// Timed on MI355X (GLUPS, clamped): 128^3 258, 192^3 629, 256^3 905, 384^3
+// 966, 512^3 1186-1196, each within 1% of the best of 14 shapes (a fixed
 // 2 x 8 x 32-plane shape: 586 at 256^3, 946 at 384^3).
 struct TB2Shape {
   int nwx, by, zchunk;
 };
-constexpr int kTB2Waves = 4096;
+constexpr int kTB2Waves = 4096, kTB2WavesFull = 16384;
 
 inline TB2Shape ChooseTB2Shape(int nx, int ny, int nz) {
   TB2Shape sh;
   sh.nwx = nx <= 256 ? 1 : (nx <= 512 ? 2 : 4);
   sh.by = sh.nwx == 4 ? 4 : 8;
   const int tiles_y = (ny + sh.by - 3) / (sh.by - 2);
-  const int64_t waves16 = (int64_t)tiles_y * ((nz + 15) / 16) * sh.nwx * sh.by;
-  sh.zchunk = waves16 >= kTB2Waves ? 16 : 8;
+  auto waves = [&](int z) { return (int64_t)tiles_y * ((nz + z - 1) / z) * sh.nwx * sh.by; };
+  sh.zchunk = 8;
+  for (int want : {kTB2WavesFull, kTB2Waves}) {
+    for (int z : {32, 16, 8}) {
+      if (waves(z) >= want) {
+        sh.zchunk = z;
+        return sh;
+      }
+    }
+  }
   return sh;
 }
 
