@@ -17,6 +17,7 @@
 #include "physis/hip/skeleton.h"
 #include "physis/hip/skeleton_star.h"
 #include "physis/hip/skeleton_star_tb.h"
+#include "physis/hip/skeleton_stride.h"
 #include "physis/hip/storage.h"
 
 namespace psk {
@@ -103,12 +104,21 @@ inline hipError_t LaunchStarStep(const typename P::Args &a, const Box &box, int 
   return LaunchRing<P, 3>(a, box, s);
 }
 
+// Whether two steps of a star plan can be fused on a grid: rows of whole
+// 16-byte vectors up to 1024 points (vectorised skeleton), or any rows up
+// to 512 points (stride skeleton).
+inline bool FusedPairSupported(int nx, int ny, int nz) {
+  return StarTB2Supported<1>(nx, ny, nz) || StarTB2Supported<2>(nx, ny, nz) || StarTB2Supported<4>(nx, ny, nz) ||
+         (nx >= 1 && nx <= 512 && ny >= 1 && nz >= 1);
+}
+
 // Two fused steps src -> dst for a star plan; false if the grid does not fit
 // a supported row width.
 template <class P>
 inline bool LaunchFusedPair(const typename P::Args &a1, const typename P::Args &a2, int nx, int ny, int nz,
                             hipStream_t s) {
-  const hipError_t e = LaunchZMarchStarTB2Auto<P>(a1, a2, nx, ny, nz, s);
+  hipError_t e = LaunchZMarchStarTB2Auto<P>(a1, a2, nx, ny, nz, s);
+  if (e == hipErrorNotSupported) e = LaunchStrideTB2Grid<P>(a1, a2, nx, ny, nz, s);
   if (e == hipErrorNotSupported) return false;
   CheckLaunch(e, "fused pair");
   return true;
@@ -133,8 +143,7 @@ inline void RunPingPong(typename P::Args ab, typename P::Args ba, int iter, int 
   st *B = P::out_ptr(ab);
   const st *src = A;
   int fused = iter - 1;
-  if (fused > 0 && (StarTB2Supported<1>(nx, ny, nz) || StarTB2Supported<2>(nx, ny, nz) ||
-                    StarTB2Supported<4>(nx, ny, nz))) {
+  if (fused > 0 && FusedPairSupported(nx, ny, nz)) {
     st *aux = static_cast<st *>(__PSHipAuxBuffer(grid_bytes));
     if (!aux) {
       fprintf(stderr, "physis-hip: no memory for the fused-step buffer\n");

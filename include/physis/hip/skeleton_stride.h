@@ -238,4 +238,48 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
   }
 }
 
+// Two fused maps over one whole grid: workgroup (blockIdx.x, blockIdx.y) =
+// (y tile of BY - 2 output rows, z chunk); tiles are renumbered per XCD.
+template <class Plan, int NP, int BY>
+__global__ void __launch_bounds__(64 * BY)
+StrideTB2Grid(typename Plan::Args a1, typename Plan::Args a2, int nx, int ny, int nz, int zchunk) {
+  __shared__ StrideTB2Lds<NP, BY> L;
+  const int ty = XcdTile(blockIdx.x, gridDim.x);
+  const int zb = (int)blockIdx.y * zchunk;
+  StrideTB2<Plan, NP, BY>(a1, a2, nx, ny, nz, ty * (BY - 2) - 1, zb, min(zb + zchunk, nz), L);
+}
+
+// Host launch of StrideTB2Grid for rows of any length up to 512 points:
+// NP = the fewest points per lane that cover a row (1, 2, 4 or 8), 8 rows
+// per workgroup, and the longest z-chunk of 32, 16 and 8 planes that gives
+// at least 16384 waves, else 4096, else 8 (the rule of ChooseTB2Shape).
+// hipErrorNotSupported for wider rows.
+template <class Plan>
+inline hipError_t LaunchStrideTB2Grid(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx, int ny,
+                                      int nz, hipStream_t stream) {
+  constexpr int BY = 8;
+  if (nx < 1 || nx > 512 || ny < 1 || nz < 1) return hipErrorNotSupported;
+  const int tiles_y = (ny + BY - 3) / (BY - 2);
+  auto waves = [&](int z) { return (int64_t)tiles_y * ((nz + z - 1) / z) * BY; };
+  int zchunk = 8;
+  bool found = false;
+  for (int want : {16384, 4096}) {
+    for (int z : {32, 16, 8}) {
+      if (!found && waves(z) >= want) {
+        zchunk = z;
+        found = true;
+      }
+    }
+  }
+  const dim3 grid(tiles_y, (nz + zchunk - 1) / zchunk, 1), block(64, BY, 1);
+  if (nx <= 64) hipLaunchKernelGGL((StrideTB2Grid<Plan, 1, BY>), grid, block, 0, stream, a1, a2, nx, ny, nz, zchunk);
+  else if (nx <= 128)
+    hipLaunchKernelGGL((StrideTB2Grid<Plan, 2, BY>), grid, block, 0, stream, a1, a2, nx, ny, nz, zchunk);
+  else if (nx <= 256)
+    hipLaunchKernelGGL((StrideTB2Grid<Plan, 4, BY>), grid, block, 0, stream, a1, a2, nx, ny, nz, zchunk);
+  else
+    hipLaunchKernelGGL((StrideTB2Grid<Plan, 8, BY>), grid, block, 0, stream, a1, a2, nx, ny, nz, zchunk);
+  return hipGetLastError();
+}
+
 }  // namespace psk

@@ -11,6 +11,7 @@
 #include "physis/hip/skeleton.h"
 #include "physis/hip/skeleton_star.h"
 #include "physis/hip/skeleton_star_tb.h"
+#include "physis/hip/skeleton_stride.h"
 #include "physis/hip/storage.h"
 
 #include <cstdio>
@@ -182,6 +183,38 @@ hipError_t LaunchPair(const Variant &v, const void *src, void *dst, const int *d
   return hipErrorInvalidValue;
 }
 
+// A fused pair on the stride skeleton (rows of any length up to 512 points),
+// for whole-grid runs whose rows are not whole 16-byte vectors.
+template <class S, bool kPeriodic>
+hipError_t LaunchStridePair(const void *src, void *dst, const int *dim, const Coef7 &k, hipStream_t s) {
+  using P = DiffusionPlan<S, kPeriodic>;
+  using st = typename S::stored_t;
+  typename P::Args a1, a2;
+  a1.g1 = static_cast<const st *>(src);
+  a1.g2 = nullptr;
+  a2.g1 = nullptr;
+  a2.g2 = static_cast<st *>(dst);
+  for (int d = 0; d < 3; ++d) a1.dim[d] = a2.dim[d] = dim[d];
+  a1.k = a2.k = k;
+  return psk::LaunchStrideTB2Grid<P>(a1, a2, dim[0], dim[1], dim[2], s);
+}
+
+hipError_t LaunchStridePairStorage(int storage, bool periodic, const void *src, void *dst, const int *dim,
+                                   const Coef7 &k, hipStream_t s) {
+  switch (storage) {
+    case PS_HIP_STORAGE_NATIVE:
+      return periodic ? LaunchStridePair<psk::Native<float>, true>(src, dst, dim, k, s)
+                      : LaunchStridePair<psk::Native<float>, false>(src, dst, dim, k, s);
+    case PS_HIP_STORAGE_BF16:
+      return periodic ? LaunchStridePair<psk::BF16, true>(src, dst, dim, k, s)
+                      : LaunchStridePair<psk::BF16, false>(src, dst, dim, k, s);
+    case PS_HIP_STORAGE_FP16:
+      return periodic ? LaunchStridePair<psk::FP16, true>(src, dst, dim, k, s)
+                      : LaunchStridePair<psk::FP16, false>(src, dst, dim, k, s);
+  }
+  return hipErrorInvalidValue;
+}
+
 hipError_t LaunchPairStorage(const Variant &v, int storage, bool periodic, const void *src, void *dst,
                              const int *dim, const Coef7 &k, hipStream_t s) {
   switch (storage) {
@@ -320,15 +353,22 @@ int PSHipDiffusion7Run(__PSGrid *g1, __PSGrid *g2, const __PSDomain *dom, const 
   if (v.family == 4) {
     const Variant &one = kVariants[kStarVariant - 1];
     const int fused = steps / 2 - 1;  // the last pair stays unfused so g2 ends as in the ref target
-    if (fused > 0 && PairApplies(v, g1->dim, box)) {
+    // The default variant fuses rows that are not whole 16-byte vectors on
+    // the stride skeleton (whole-grid runs, rows up to 512 points).
+    const bool star_pair = PairApplies(v, g1->dim, box);
+    bool whole = true;
+    for (int d = 0; d < 3; ++d) whole = whole && box.lo[d] == 0 && box.hi[d] == g1->dim[d];
+    const bool stride_pair = !star_pair && variant == 0 && whole && g1->dim[0] <= 512;
+    if (fused > 0 && (star_pair || stride_pair)) {
       // Fused pairs ping-pong between g1 and a third buffer; the last pair then
       // reads whichever holds the field: map(src -> g2), map(g2 -> g1).
       void *aux = Runtime::Get().Aux(Rec(g1)->bytes);
       if (!aux) return PS_HIP_ERR_NOMEM;
       for (int i = 0; i < fused; ++i) {
         void *dst = (i & 1) ? g1->p0 : aux;
-        if (LaunchPairStorage(v, storage, periodic, src, dst, g1->dim, k, s) != hipSuccess)
-          return PS_HIP_ERR_DEVICE;
+        const hipError_t e = star_pair ? LaunchPairStorage(v, storage, periodic, src, dst, g1->dim, k, s)
+                                       : LaunchStridePairStorage(storage, periodic, src, dst, g1->dim, k, s);
+        if (e != hipSuccess) return PS_HIP_ERR_DEVICE;
         src = dst;
       }
       first = 2 * fused;

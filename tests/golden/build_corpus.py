@@ -11,6 +11,8 @@ main) is linked with its plain C driver test_*.module_base.c.  The
 reference's diffusion benchmark (examples/diffusion-benchmark) is built the
 same way: its Physis program translated, its C++ harness compiled for the
 Physis variant and linked in (tests/_corpus_bin/diffusion_benchmark_physis).
+The DSL programs of this repository under tests/golden/programs are built
+into the same directory.
 """
 import json
 import os
@@ -45,13 +47,34 @@ def build_benchmark(phc):
         return False
 
 
+# DSL programs of this repository (tests/golden/programs), built the same way.
+OWN = os.path.join(HERE, "programs")
+
+
+def build_own(phc):
+    failed = []
+    for name in sorted(os.listdir(OWN)):
+        if not name.endswith(".c"):
+            continue
+        exe = os.path.join(OUT, name[:-2])
+        try:
+            phc.build(os.path.join(OWN, name), exe)
+            os.remove(exe + ".hip")
+        except subprocess.CalledProcessError:
+            failed.append(name)
+    return failed
+
+
 def main(jobs=8):
-    if not os.path.isdir(CASES):
-        print("reference corpus not present; nothing to build", file=sys.stderr)
-        return 0
     sys.path.insert(0, ROOT)
     from physis_amd import phc
     os.makedirs(OUT, exist_ok=True)
+    own_failed = build_own(phc)
+    if own_failed:
+        print("own programs failed: %s" % own_failed, file=sys.stderr)
+    if not os.path.isdir(CASES):
+        print("reference corpus not present; built only this repository's programs", file=sys.stderr)
+        return 1 if own_failed else 0
     stems = []
     for name in sorted(os.listdir(CASES)):
         if not name.startswith("test_") or not name.endswith(".c") or ".manual." in name or \
@@ -80,7 +103,7 @@ def main(jobs=8):
     with open(os.path.join(OUT, "manifest.json"), "w") as f:
         json.dump({"built": built, "failed": failed}, f, indent=1)
     print("built %d, failed %d %s" % (len(built), len(failed), failed), file=sys.stderr)
-    return 1 if failed else 0
+    return 1 if failed or own_failed else 0
 
 
 if __name__ == "__main__":
