@@ -98,9 +98,15 @@ inline hipError_t LaunchRing(const typename P::Args &a, const Box &box, hipStrea
 
 // One step of a star plan: the vectorised skeleton when rows are whole
 // 16-byte vectors, else the LDS ring.
+// (nx, ny, nz: extents of the staged grid.)
 template <class P>
-inline hipError_t LaunchStarStep(const typename P::Args &a, const Box &box, int nx, hipStream_t s) {
+inline hipError_t LaunchStarStep(const typename P::Args &a, const Box &box, int nx, int ny, int nz, hipStream_t s) {
   if (StarSupported(nx)) return LaunchZMarchStar<P, 8>(a, box, 128, true, s);
+  // Whole-grid steps with rows of up to 256 points: the stride skeleton
+  // (several times the LDS ring on small and mid-size grids).
+  if (nx <= 256 && box.lo[0] == 0 && box.hi[0] == nx && box.lo[1] == 0 && box.hi[1] == ny && box.lo[2] == 0 &&
+      box.hi[2] == nz)
+    return LaunchStrideStepGrid<P>(a, nx, ny, nz, s);
   return LaunchRing<P, 3>(a, box, s);
 }
 
@@ -163,8 +169,8 @@ inline void RunPingPong(typename P::Args ab, typename P::Args ba, int iter, int 
   for (int i = fused; i < iter; ++i) {
     typename P::Args a1 = ab;
     P::set_in(a1, src);
-    CheckLaunch(LaunchStarStep<P>(a1, box, nx, s), "step");
-    CheckLaunch(LaunchStarStep<P>(ba, box, nx, s), "step");
+    CheckLaunch(LaunchStarStep<P>(a1, box, nx, ny, nz, s), "step");
+    CheckLaunch(LaunchStarStep<P>(ba, box, nx, ny, nz, s), "step");
     src = A;
   }
 }

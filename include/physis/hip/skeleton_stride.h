@@ -238,6 +238,47 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
   }
 }
 
+// One map over one whole grid: workgroup (blockIdx.x, blockIdx.y) = (y tile
+// of BY rows, z chunk); tiles are renumbered per XCD.
+template <class Plan, int NP, int BY>
+__global__ void __launch_bounds__(64 * BY)
+StrideStepGrid(typename Plan::Args a, int nx, int ny, int nz, int zchunk) {
+  __shared__ StrideLds<NP, BY> L;
+  const int ty = XcdTile(blockIdx.x, gridDim.x);
+  const int zb = (int)blockIdx.y * zchunk;
+  StrideStep<Plan, NP, BY>(a, nx, ny, nz, ty * BY, zb, min(zb + zchunk, nz), L);
+}
+
+// z-chunk of a whole-grid stride launch: the longest of 32, 16 and 8 planes
+// that gives at least 16384 waves, else 4096, else 8.
+inline int StrideZChunk(int tiles_y, int nz, int waves_per_group) {
+  auto waves = [&](int z) { return (int64_t)tiles_y * ((nz + z - 1) / z) * waves_per_group; };
+  for (int want : {16384, 4096})
+    for (int z : {32, 16, 8})
+      if (waves(z) >= want) return z;
+  return 8;
+}
+
+// Host launch of StrideStepGrid for rows of any length up to 512 points (NP
+// = the fewest points per lane that cover a row, 8 rows per workgroup);
+// hipErrorNotSupported for wider rows.
+template <class Plan>
+inline hipError_t LaunchStrideStepGrid(const typename Plan::Args &a, int nx, int ny, int nz, hipStream_t stream) {
+  constexpr int BY = 8;
+  if (nx < 1 || nx > 512 || ny < 1 || nz < 1) return hipErrorNotSupported;
+  const int tiles_y = (ny + BY - 1) / BY;
+  const int zchunk = StrideZChunk(tiles_y, nz, BY);
+  const dim3 grid(tiles_y, (nz + zchunk - 1) / zchunk, 1), block(64, BY, 1);
+  if (nx <= 64) hipLaunchKernelGGL((StrideStepGrid<Plan, 1, BY>), grid, block, 0, stream, a, nx, ny, nz, zchunk);
+  else if (nx <= 128)
+    hipLaunchKernelGGL((StrideStepGrid<Plan, 2, BY>), grid, block, 0, stream, a, nx, ny, nz, zchunk);
+  else if (nx <= 256)
+    hipLaunchKernelGGL((StrideStepGrid<Plan, 4, BY>), grid, block, 0, stream, a, nx, ny, nz, zchunk);
+  else
+    hipLaunchKernelGGL((StrideStepGrid<Plan, 8, BY>), grid, block, 0, stream, a, nx, ny, nz, zchunk);
+  return hipGetLastError();
+}
+
 // Two fused maps over one whole grid: workgroup (blockIdx.x, blockIdx.y) =
 // (y tile of BY - 2 output rows, z chunk); tiles are renumbered per XCD.
 template <class Plan, int NP, int BY>
@@ -251,8 +292,8 @@ StrideTB2Grid(typename Plan::Args a1, typename Plan::Args a2, int nx, int ny, in
 
 // Host launch of StrideTB2Grid for rows of any length up to 512 points:
 // NP = the fewest points per lane that cover a row (1, 2, 4 or 8), 8 rows
-// per workgroup, and the longest z-chunk of 32, 16 and 8 planes that gives
-// at least 16384 waves, else 4096, else 8 (the rule of ChooseTB2Shape).
+// per workgroup, and the z-chunk of StrideZChunk (the rule of
+// ChooseTB2Shape).
 // hipErrorNotSupported for wider rows.
 template <class Plan>
 inline hipError_t LaunchStrideTB2Grid(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx, int ny,
@@ -260,17 +301,7 @@ inline hipError_t LaunchStrideTB2Grid(const typename Plan::Args &a1, const typen
   constexpr int BY = 8;
   if (nx < 1 || nx > 512 || ny < 1 || nz < 1) return hipErrorNotSupported;
   const int tiles_y = (ny + BY - 3) / (BY - 2);
-  auto waves = [&](int z) { return (int64_t)tiles_y * ((nz + z - 1) / z) * BY; };
-  int zchunk = 8;
-  bool found = false;
-  for (int want : {16384, 4096}) {
-    for (int z : {32, 16, 8}) {
-      if (!found && waves(z) >= want) {
-        zchunk = z;
-        found = true;
-      }
-    }
-  }
+  const int zchunk = StrideZChunk(tiles_y, nz, BY);
   const dim3 grid(tiles_y, (nz + zchunk - 1) / zchunk, 1), block(64, BY, 1);
   if (nx <= 64) hipLaunchKernelGGL((StrideTB2Grid<Plan, 1, BY>), grid, block, 0, stream, a1, a2, nx, ny, nz, zchunk);
   else if (nx <= 128)

@@ -1134,7 +1134,8 @@ std::string Translator::EmitMapAndLaunch(const Kernel &k) {
   o << "  psk::Box box = psk::BoxFromDomain(s->dom, " << k.rank << ");\n";
   if (k.star)
     o << "  if (parity < 0) {\n    psk::CheckLaunch(psk::LaunchStarStep<" << P << "<SF>>(a, box, a." << k.staged
-      << ".dim[0], st), \"" << f.name << "\");\n    return;\n  }\n";
+      << ".dim[0], a." << k.staged << ".dim[1], a." << k.staged << ".dim[2], st), \"" << f.name
+      << "\");\n    return;\n  }\n";
   o << "  psk::CheckLaunch(psk::LaunchRing<" << P << "<SF>, " << k.rank << ">(a, box, st, parity), \"" << f.name
     << "\");\n";
   o << "}\n\n";
