@@ -96,19 +96,26 @@ BatchTB2(const typename P::Args *in_args, const typename P::Args *out_args, cons
 
 // A launch configuration of the batch.
 struct Config {
-  int np;      // points per lane (row width 64*np)
+  int np;      // points per lane (row width 64*np); with `mixed`, of one class
   int by;      // rows per workgroup
   int zchunk;  // planes per work item
   int degree;  // 1: one step per launch; 2: two fused steps
+  // 1: every grid runs with the fewest points per lane that cover its rows;
+  // the grids of each such width class run on their own stream (forked from
+  // and joined to the caller's), so the classes overlap.
+  int mixed = 0;
   bool operator<(const Config &o) const {
-    return std::tie(np, by, zchunk, degree) < std::tie(o.np, o.by, o.zchunk, o.degree);
+    return std::tie(np, by, zchunk, degree, mixed) < std::tie(o.np, o.by, o.zchunk, o.degree, o.mixed);
   }
 };
 
 std::string ConfigName(const Config &c) {
-  return "np" + std::to_string(c.np) + "_by" + std::to_string(c.by) + "_z" + std::to_string(c.zchunk) + "_t" +
-         std::to_string(c.degree);
+  return (c.mixed ? std::string("mixed") : "np" + std::to_string(c.np)) + "_by" + std::to_string(c.by) + "_z" +
+         std::to_string(c.zchunk) + "_t" + std::to_string(c.degree);
 }
+
+// Points per lane of a grid's width class: the fewest that cover its rows.
+int ClassNp(int nx) { return nx <= 64 ? 1 : nx <= 128 ? 2 : nx <= 256 ? 4 : 8; }
 
 using KernelFn = const void *;
 
@@ -192,10 +199,23 @@ struct Batch {
   };
   std::map<std::pair<Config, int>, Graph> graphs;
   Config tuned{0, 0, 0, 0};
+  // Mixed mode: width classes present (ascending), and per-class streams and
+  // fork/join events (created on first use; default priority).
+  std::vector<int> classes;
+  hipStream_t class_stream[4] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[4] = {};
 
   ~Batch() {
     for (auto &g : graphs)
       if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+    for (auto &st : class_stream)
+      if (st) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+      }
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    for (auto &e : join_ev)
+      if (e) (void)hipEventDestroy(e);
     for (size_t i = 0; i < aux.size(); ++i)
       if (aux[i] && owns_aux[i]) (void)hipFree(aux[i]);
   }
@@ -232,6 +252,14 @@ struct Batch {
   // A degree-2 run also needs the one-step kernel of the same shape for its
   // unfused steps.
   bool Supports(const Config &c) const {
+    if (c.mixed) {
+      if (c.zchunk <= 0) return false;
+      for (int np : classes)
+        if (StepKernel(storage, false, np, c.by) == nullptr ||
+            (c.degree == 2 && TB2Kernel(storage, false, np, c.by) == nullptr))
+          return false;
+      return true;
+    }
     if (64 * c.np < max_nx || c.zchunk <= 0) return false;
     if (c.degree == 2 && TB2Kernel(storage, false, c.np, c.by) == nullptr) return false;
     return StepKernel(storage, false, c.np, c.by) != nullptr;
@@ -248,6 +276,7 @@ struct Batch {
     std::vector<Item> items;
     const int rows_out = c.degree == 2 ? c.by - 2 : c.by;
     for (int g : order) {
+      if (c.mixed && ClassNp(a[g]->dim[0]) != c.np) continue;  // another width class
       const int ny = a[g]->dim[1], nz = a[g]->dim[2];
       const int zc = c.zchunk <= 0 || c.zchunk > nz ? nz : c.zchunk;
       for (int ty = 0; ty * rows_out < ny; ++ty)
@@ -273,13 +302,52 @@ struct Batch {
     return p;
   }
 
+  // Builds the tables, every work list configuration c uses and, in the
+  // mixed mode, the class streams and events (uploads synchronise, so this
+  // must happen before a graph capture).
+  void Prepare(const Config &c, hipStream_t s) {
+    BuildTables(s);
+    if (!c.mixed) {
+      PlanFor(Config{c.np, c.by, c.zchunk, 1}, s);
+      PlanFor(c, s);
+      return;
+    }
+    for (int np : classes) {
+      PlanFor(Config{np, c.by, c.zchunk, 1, 1}, s);
+      PlanFor(Config{np, c.by, c.zchunk, c.degree, 1}, s);
+    }
+    if (!fork_ev) {
+      CheckHip(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), "hipEventCreate");
+      for (size_t i = 0; i < 4; ++i) {
+        CheckHip(hipStreamCreateWithFlags(&class_stream[i], hipStreamNonBlocking), "hipStreamCreate");
+        CheckHip(hipEventCreateWithFlags(&join_ev[i], hipEventDisableTiming), "hipEventCreate");
+      }
+    }
+  }
+
   // Queues `steps` steps of every grid with configuration c.
   hipError_t Enqueue(const Config &c, int steps, hipStream_t s) {
-    BuildTables(s);
+    Prepare(c, s);
+    if (!c.mixed) return EnqueueOne(c, steps, s);
+    // One stream per width class: fork from s, run each class's steps, join.
+    hipError_t e = hipEventRecord(fork_ev, s);
+    for (size_t i = 0; i < classes.size() && e == hipSuccess; ++i) {
+      hipStream_t cs = class_stream[i];
+      e = hipStreamWaitEvent(cs, fork_ev, 0);
+      if (e == hipSuccess) e = EnqueueOne(Config{classes[i], c.by, c.zchunk, c.degree, 1}, steps, cs);
+      if (e == hipSuccess) e = hipEventRecord(join_ev[i], cs);
+      if (e == hipSuccess) e = hipStreamWaitEvent(s, join_ev[i], 0);
+    }
+    return e;
+  }
+
+  // Queues `steps` steps of the grids configuration c covers (all grids, or
+  // with c.mixed the width class c.np) on stream s.
+  hipError_t EnqueueOne(const Config &c, int steps, hipStream_t s) {
     // Unfused steps always run on the one-step work list (row tiles start at
     // their first row); fused pairs on the two-step list (tiles start one row
     // above their first output row).
-    const Config c1{c.np, c.by, c.zchunk, 1};
+    const Config c1{c.np, c.by, c.zchunk, 1, c.mixed};
     Plan &p1 = PlanFor(c1, s);
     Plan &p2 = PlanFor(c, s);
     if (p1.nitems == 0 || steps <= 0) return hipSuccess;
@@ -325,10 +393,9 @@ struct Batch {
     auto key = std::make_pair(c, steps);
     auto it = graphs.find(key);
     if (it == graphs.end()) {
-      // Everything that allocates or uploads happens before the capture.
-      BuildTables(s);
-      PlanFor(Config{c.np, c.by, c.zchunk, 1}, s);
-      PlanFor(c, s);
+      // Everything that allocates, uploads or creates streams and events
+      // happens before the capture.
+      Prepare(c, s);
       CheckHip(hipStreamSynchronize(s), "sync before capture");
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
@@ -349,6 +416,14 @@ struct Batch {
 
 std::vector<Config> Candidates(const Batch &b) {
   std::vector<Config> out;
+  if (b.classes.size() > 1) {
+    for (int deg : {1, 2})
+      for (int by : {8, 16})
+        for (int zc : {16, 32}) {
+          Config c{0, by, zc, deg, 1};
+          if (b.Supports(c)) out.push_back(c);
+        }
+  }
   for (int deg : {1, 2})
     for (int np : {1, 2, 4, 8})
       for (int by : {4, 8, 16})
@@ -412,6 +487,11 @@ int PSHipBatchCreateEx(void **out, __PSGrid *const *a, __PSGrid *const *b, void 
     bt->aux.push_back(x_aux);
     bt->bytes += Rec(x)->bytes;
   }
+  for (int i = 0; i < bt->Count(); ++i) {
+    const int np = ClassNp(bt->a[i]->dim[0]);
+    if (std::find(bt->classes.begin(), bt->classes.end(), np) == bt->classes.end()) bt->classes.push_back(np);
+  }
+  std::sort(bt->classes.begin(), bt->classes.end());
   bt->tuned = DefaultConfig(*bt);
   *out = bt;
   return PS_HIP_OK;
@@ -446,11 +526,13 @@ int PSHipBatchRun(void *h, int steps, int use_graph) {
   return e == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
 }
 
-// Sets the launch configuration explicitly (np, by, zchunk, degree).
+// Sets the launch configuration explicitly (np, by, zchunk, degree); np = 0
+// selects the mixed mode (each grid with the fewest points per lane that
+// cover its rows, width classes on their own streams).
 int PSHipBatchConfigure(void *h, int np, int by, int zchunk, int degree) {
   if (!h) return PS_HIP_ERR_INVALID;
   Batch *b = static_cast<Batch *>(h);
-  Config c{np, by, zchunk, degree};
+  Config c{np, by, zchunk, degree, np == 0 ? 1 : 0};
   if ((degree != 1 && degree != 2) || !b->Supports(c)) return PS_HIP_ERR_UNSUPPORTED;
   b->tuned = c;
   return PS_HIP_OK;

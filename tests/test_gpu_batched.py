@@ -38,8 +38,9 @@ def check_against_oracle(pairs, fields, coefs, boundary, steps):
 
 
 @pytest.mark.parametrize("boundary", [0, 1])
+# np = 0: the mixed mode (each width class with its own kernel and stream).
 @pytest.mark.parametrize("cfg", [(2, 4, 16, 1), (2, 8, 64, 1), (4, 8, 32, 1), (2, 8, 16, 2), (2, 16, 32, 2),
-                                 (4, 8, 8, 2)])
+                                 (4, 8, 8, 2), (0, 8, 16, 1), (0, 16, 32, 2), (0, 8, 8, 2)])
 @pytest.mark.parametrize("graph", [False, True])
 def test_batch_bitwise_vs_oracle(rt, boundary, cfg, graph):
     steps = 9
@@ -54,10 +55,11 @@ def test_batch_bitwise_vs_oracle(rt, boundary, cfg, graph):
         g2.free()
 
 
-def test_graph_replays_with_eager_work_between(rt):
+@pytest.mark.parametrize("cfg", [(2, 8, 16, 2), (0, 8, 16, 2)])
+def test_graph_replays_with_eager_work_between(rt, cfg):
     steps = 6
     pairs, fields, coefs, batch = make_batch(rt, SHAPES[:6], seed=9, boundary=0)
-    batch.configure(2, 8, 16, 2)
+    batch.configure(*cfg)
     for rep in range(1, 4):
         batch.run(steps, graph=True)
         # Eager work on the same stream between replays.
