@@ -259,9 +259,16 @@ inline int StrideZChunk(int tiles_y, int nz, int waves_per_group) {
   return 8;
 }
 
+// Points per lane of a row of nx points: the fewest of 1, 2, 3, 4, 6 and 8
+// that cover it (a lane computes all NP of its points whether or not they are
+// in the row, so the wave's width follows the row's).
+inline int StrideNp(int nx) {
+  return nx <= 64 ? 1 : nx <= 128 ? 2 : nx <= 192 ? 3 : nx <= 256 ? 4 : nx <= 384 ? 6 : 8;
+}
+
 // Host launch of StrideStepGrid for rows of any length up to 512 points (NP
-// = the fewest points per lane that cover a row, 8 rows per workgroup);
-// hipErrorNotSupported for wider rows.
+// = StrideNp(nx), 8 rows per workgroup); hipErrorNotSupported for wider
+// rows.
 template <class Plan>
 inline hipError_t LaunchStrideStepGrid(const typename Plan::Args &a, int nx, int ny, int nz, hipStream_t stream) {
   constexpr int BY = 8;
@@ -269,13 +276,12 @@ inline hipError_t LaunchStrideStepGrid(const typename Plan::Args &a, int nx, int
   const int tiles_y = (ny + BY - 1) / BY;
   const int zchunk = StrideZChunk(tiles_y, nz, BY);
   const dim3 grid(tiles_y, (nz + zchunk - 1) / zchunk, 1), block(64, BY, 1);
-  if (nx <= 64) hipLaunchKernelGGL((StrideStepGrid<Plan, 1, BY>), grid, block, 0, stream, a, nx, ny, nz, zchunk);
-  else if (nx <= 128)
-    hipLaunchKernelGGL((StrideStepGrid<Plan, 2, BY>), grid, block, 0, stream, a, nx, ny, nz, zchunk);
-  else if (nx <= 256)
-    hipLaunchKernelGGL((StrideStepGrid<Plan, 4, BY>), grid, block, 0, stream, a, nx, ny, nz, zchunk);
-  else
-    hipLaunchKernelGGL((StrideStepGrid<Plan, 8, BY>), grid, block, 0, stream, a, nx, ny, nz, zchunk);
+  switch (StrideNp(nx)) {
+#define PSK_STEP(NP) \
+  case NP: hipLaunchKernelGGL((StrideStepGrid<Plan, NP, BY>), grid, block, 0, stream, a, nx, ny, nz, zchunk); break;
+    PSK_STEP(1) PSK_STEP(2) PSK_STEP(3) PSK_STEP(4) PSK_STEP(6) PSK_STEP(8)
+#undef PSK_STEP
+  }
   return hipGetLastError();
 }
 
@@ -291,9 +297,8 @@ StrideTB2Grid(typename Plan::Args a1, typename Plan::Args a2, int nx, int ny, in
 }
 
 // Host launch of StrideTB2Grid for rows of any length up to 512 points:
-// NP = the fewest points per lane that cover a row (1, 2, 4 or 8), 8 rows
-// per workgroup, and the z-chunk of StrideZChunk (the rule of
-// ChooseTB2Shape).
+// NP = StrideNp(nx), 8 rows per workgroup, and the z-chunk of StrideZChunk
+// (the rule of ChooseTB2Shape).
 // hipErrorNotSupported for wider rows.
 template <class Plan>
 inline hipError_t LaunchStrideTB2Grid(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx, int ny,
@@ -303,13 +308,14 @@ inline hipError_t LaunchStrideTB2Grid(const typename Plan::Args &a1, const typen
   const int tiles_y = (ny + BY - 3) / (BY - 2);
   const int zchunk = StrideZChunk(tiles_y, nz, BY);
   const dim3 grid(tiles_y, (nz + zchunk - 1) / zchunk, 1), block(64, BY, 1);
-  if (nx <= 64) hipLaunchKernelGGL((StrideTB2Grid<Plan, 1, BY>), grid, block, 0, stream, a1, a2, nx, ny, nz, zchunk);
-  else if (nx <= 128)
-    hipLaunchKernelGGL((StrideTB2Grid<Plan, 2, BY>), grid, block, 0, stream, a1, a2, nx, ny, nz, zchunk);
-  else if (nx <= 256)
-    hipLaunchKernelGGL((StrideTB2Grid<Plan, 4, BY>), grid, block, 0, stream, a1, a2, nx, ny, nz, zchunk);
-  else
-    hipLaunchKernelGGL((StrideTB2Grid<Plan, 8, BY>), grid, block, 0, stream, a1, a2, nx, ny, nz, zchunk);
+  switch (StrideNp(nx)) {
+#define PSK_TB2(NP)                                                                                             \
+  case NP:                                                                                                      \
+    hipLaunchKernelGGL((StrideTB2Grid<Plan, NP, BY>), grid, block, 0, stream, a1, a2, nx, ny, nz, zchunk); \
+    break;
+    PSK_TB2(1) PSK_TB2(2) PSK_TB2(3) PSK_TB2(4) PSK_TB2(6) PSK_TB2(8)
+#undef PSK_TB2
+  }
   return hipGetLastError();
 }
 

@@ -114,8 +114,10 @@ std::string ConfigName(const Config &c) {
          std::to_string(c.zchunk) + "_t" + std::to_string(c.degree);
 }
 
-// Points per lane of a grid's width class: the fewest that cover its rows.
-int ClassNp(int nx) { return nx <= 64 ? 1 : nx <= 128 ? 2 : nx <= 256 ? 4 : 8; }
+// Points per lane of a grid's width class: the fewest that cover its rows
+// (1, 2, 3, 4, 6 or 8).
+int ClassNp(int nx) { return psk::StrideNp(nx); }
+constexpr int kMaxClasses = 6;
 
 using KernelFn = const void *;
 
@@ -124,8 +126,8 @@ KernelFn StepKernelT(int np, int by) {
   using P = BatchPlan<S, kPer>;
 #define PSH_K(NP, BY) \
   if (np == NP && by == BY) return reinterpret_cast<KernelFn>(&BatchStep<P, NP, BY>);
-  PSH_K(1, 4) PSH_K(1, 8) PSH_K(1, 16) PSH_K(2, 4) PSH_K(2, 8) PSH_K(2, 16) PSH_K(4, 4) PSH_K(4, 8) PSH_K(4, 16)
-  PSH_K(8, 4) PSH_K(8, 8)
+  PSH_K(1, 4) PSH_K(1, 8) PSH_K(1, 16) PSH_K(2, 4) PSH_K(2, 8) PSH_K(2, 16) PSH_K(3, 8) PSH_K(3, 16) PSH_K(4, 4)
+  PSH_K(4, 8) PSH_K(4, 16) PSH_K(6, 8) PSH_K(8, 4) PSH_K(8, 8)
 #undef PSH_K
   return nullptr;
 }
@@ -135,7 +137,8 @@ KernelFn TB2KernelT(int np, int by) {
   using P = BatchPlan<S, kPer>;
 #define PSH_K(NP, BY) \
   if (np == NP && by == BY) return reinterpret_cast<KernelFn>(&BatchTB2<P, NP, BY>);
-  PSH_K(1, 8) PSH_K(2, 8) PSH_K(4, 8) PSH_K(8, 8) PSH_K(1, 16) PSH_K(2, 16) PSH_K(4, 16)
+  PSH_K(1, 8) PSH_K(2, 8) PSH_K(3, 8) PSH_K(4, 8) PSH_K(6, 8) PSH_K(8, 8) PSH_K(1, 16) PSH_K(2, 16) PSH_K(3, 16)
+  PSH_K(4, 16)
 #undef PSH_K
   return nullptr;
 }
@@ -202,8 +205,8 @@ struct Batch {
   // Mixed mode: width classes present (ascending), and per-class streams and
   // fork/join events (created on first use; default priority).
   std::vector<int> classes;
-  hipStream_t class_stream[4] = {};
-  hipEvent_t fork_ev = nullptr, join_ev[4] = {};
+  hipStream_t class_stream[kMaxClasses] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[kMaxClasses] = {};
 
   ~Batch() {
     for (auto &g : graphs)
@@ -318,7 +321,7 @@ struct Batch {
     }
     if (!fork_ev) {
       CheckHip(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), "hipEventCreate");
-      for (size_t i = 0; i < 4; ++i) {
+      for (int i = 0; i < kMaxClasses; ++i) {
         CheckHip(hipStreamCreateWithFlags(&class_stream[i], hipStreamNonBlocking), "hipStreamCreate");
         CheckHip(hipEventCreateWithFlags(&join_ev[i], hipEventDisableTiming), "hipEventCreate");
       }
@@ -424,22 +427,20 @@ std::vector<Config> Candidates(const Batch &b) {
           if (b.Supports(c)) out.push_back(c);
         }
   }
+  // One width for all grids: the narrowest that fits the widest row only
+  // (wider rows idle lanes).
+  const int np = ClassNp(b.max_nx);
   for (int deg : {1, 2})
-    for (int np : {1, 2, 4, 8})
-      for (int by : {4, 8, 16})
-        for (int zc : {16, 32, 64}) {
-          Config c{np, by, zc, deg};
-          if (!b.Supports(c)) continue;
-          // Narrowest row that fits only (wider rows idle lanes).
-          if (np > 1 && 64 * (np / 2) >= b.max_nx) continue;
-          out.push_back(c);
-        }
+    for (int by : {4, 8, 16})
+      for (int zc : {16, 32, 64}) {
+        Config c{np, by, zc, deg};
+        if (b.Supports(c)) out.push_back(c);
+      }
   return out;
 }
 
 Config DefaultConfig(const Batch &b) {
-  int np = b.max_nx <= 64 ? 1 : b.max_nx <= 128 ? 2 : b.max_nx <= 256 ? 4 : 8;
-  return Config{np, 8, 32, 2};
+  return Config{ClassNp(b.max_nx), 8, 32, 2};
 }
 
 }  // namespace
