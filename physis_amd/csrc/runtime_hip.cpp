@@ -97,6 +97,7 @@ void *Runtime::Aux(size_t bytes) {
 void Runtime::Shutdown() {
   if (!inited_) return;
   (void)hipStreamSynchronize(stream_);
+  ReleaseHostCopyPool();
   if (scratch_) (void)hipFree(scratch_);
   if (pinned_) (void)hipHostFree(pinned_);
   if (aux_) (void)hipFree(aux_);
@@ -190,6 +191,9 @@ int GridNewImpl(__PSGrid **out, int elm_size, int num_dims, const int *dim,
   for (int d = 0; d < PS_MAX_DIM; ++d) r->view.dim[d] = g->dim[d];
   g->dev = &r->view;
   rt.AddGridBytes((ptrdiff_t)r->bytes);
+  // Host arrays of this grid hold elm_size-byte elements (FP32 for 16-bit
+  // storage).
+  PrepareHostCopy((size_t)elm_size * (size_t)g->num_elms);
   *out = g;
   return PS_HIP_OK;
 }
@@ -299,11 +303,7 @@ int PSHipGridCopyin(__PSGrid *g, const void *src) {
   if (!g->p0) return PS_HIP_ERR_INVALID;
   Runtime &rt = Runtime::Get();
   hipStream_t s = rt.stream();
-  if (r->storage == PS_HIP_STORAGE_NATIVE) {
-    if (hipMemcpyAsync(g->p0, src, r->bytes, hipMemcpyHostToDevice, s) != hipSuccess)
-      return PS_HIP_ERR_DEVICE;
-    return hipStreamSynchronize(s) == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
-  }
+  if (r->storage == PS_HIP_STORAGE_NATIVE) return CopyHostToDevice(g->p0, src, r->bytes, s);
   // FP32 host data -> 16-bit device storage, converted on the device in
   // chunks through a staging buffer.
   const int64_t chunk = 8 << 20;  // elements (32 MiB of FP32)
@@ -312,8 +312,8 @@ int PSHipGridCopyin(__PSGrid *g, const void *src) {
   char *d = static_cast<char *>(g->p0);
   for (int64_t off = 0; off < g->num_elms; off += chunk) {
     int64_t n = g->num_elms - off < chunk ? g->num_elms - off : chunk;
-    if (hipMemcpyAsync(stage, h + off, n * sizeof(float), hipMemcpyHostToDevice, s) != hipSuccess)
-      return PS_HIP_ERR_DEVICE;
+    // Blocking, and ordered after the previous chunk's conversion.
+    if (CopyHostToDevice(stage, h + off, n * sizeof(float), s) != PS_HIP_OK) return PS_HIP_ERR_DEVICE;
     ConvertToStorage(d + off * r->stored_elm_size, stage, n, r->storage, s);
   }
   return hipStreamSynchronize(s) == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
@@ -326,11 +326,7 @@ int PSHipGridCopyout(__PSGrid *g, void *dst) {
   if (!g->p0) return PS_HIP_ERR_INVALID;
   Runtime &rt = Runtime::Get();
   hipStream_t s = rt.stream();
-  if (r->storage == PS_HIP_STORAGE_NATIVE) {
-    if (hipMemcpyAsync(dst, g->p0, r->bytes, hipMemcpyDeviceToHost, s) != hipSuccess)
-      return PS_HIP_ERR_DEVICE;
-    return hipStreamSynchronize(s) == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
-  }
+  if (r->storage == PS_HIP_STORAGE_NATIVE) return CopyDeviceToHost(dst, g->p0, r->bytes, s);
   const int64_t chunk = 8 << 20;
   float *stage = static_cast<float *>(rt.Scratch(chunk * sizeof(float)));
   float *h = static_cast<float *>(dst);
@@ -338,10 +334,8 @@ int PSHipGridCopyout(__PSGrid *g, void *dst) {
   for (int64_t off = 0; off < g->num_elms; off += chunk) {
     int64_t n = g->num_elms - off < chunk ? g->num_elms - off : chunk;
     ConvertFromStorage(stage, d + off * r->stored_elm_size, n, r->storage, s);
-    if (hipMemcpyAsync(h + off, stage, n * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess)
-      return PS_HIP_ERR_DEVICE;
-    // The staging buffer is reused by the next chunk.
-    if (hipStreamSynchronize(s) != hipSuccess) return PS_HIP_ERR_DEVICE;
+    // Blocking: the staging buffer is reused by the next chunk.
+    if (CopyDeviceToHost(h + off, stage, n * sizeof(float), s) != PS_HIP_OK) return PS_HIP_ERR_DEVICE;
   }
   return PS_HIP_OK;
 }

@@ -80,6 +80,17 @@ class Runtime {
   std::mutex mu_;
 };
 
+// Blocking transfers between host memory (pageable or not) and device memory,
+// ordered after the work already queued on s; large ones are pipelined
+// through pinned staging buffers by several host threads (hostcopy.cpp).
+int CopyHostToDevice(void *dst, const void *src, size_t bytes, hipStream_t s);
+int CopyDeviceToHost(void *dst, const void *src, size_t bytes, hipStream_t s);
+// Creates the staging buffers a transfer of `bytes` uses (pinning host
+// memory takes milliseconds: grid creation pays it, not the first copy).
+void PrepareHostCopy(size_t bytes);
+// Frees the staging buffers, streams and events of those transfers.
+void ReleaseHostCopyPool();
+
 // Device-side storage conversions for 16-bit grids (buffer_hip.hip).
 void ConvertToStorage(void *dst, const float *src, int64_t n, int storage,
                       hipStream_t s);

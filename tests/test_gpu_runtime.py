@@ -19,6 +19,44 @@ def test_copyin_copyout_roundtrip(rt):
     g.free()
 
 
+# Copies of 16 MiB or more are pipelined through pinned staging buffers by
+# several host threads in 4 MiB chunks: sizes with a partial last chunk, an
+# exact multiple of the chunk, and more chunks than threads.
+LARGE = [(257, 129, 129), (1 << 20, 4, 2), (300, 300, 300)]
+
+
+@pytest.mark.parametrize("shape", LARGE)
+def test_large_copyin_copyout_roundtrip(rt, shape):
+    n = int(np.prod(shape))
+    a = np.random.default_rng(n).random(n, dtype=np.float32)
+    print("shape", shape, "MiB", 4 * n >> 20, flush=True)
+    g = rt.Grid(shape)
+    g.copyin(a)
+    assert np.array_equal(g.copyout().ravel(), a)
+    # Copyout waits for the kernels queued before it (a device-side fill).
+    g.fill_uniform(9)
+    assert np.array_equal(g.copyout(), O.uniform_field(shape, seed=9))
+    # Copyin is complete when it returns: a reduction right after sees it
+    # (max: exact; sum: FP64 in another order than numpy's).
+    b = a[::-1] * np.float32(2)
+    g.copyin(b)
+    assert g.reduce64(rt.PS_MAX) == float(b.max())
+    assert abs(g.reduce64(rt.PS_SUM) - float(np.sum(b, dtype=np.float64))) <= 1e-9 * float(np.sum(b, dtype=np.float64))
+    assert np.array_equal(g.copyout().ravel(), b)
+    g.free()
+
+
+def test_large_bf16_copies_round_to_nearest_even(rt):
+    shape = (300, 300, 100)  # 36 MB of FP32 host data: converted in 32 MiB chunks
+    a = np.random.default_rng(4).random(int(np.prod(shape)), dtype=np.float32)
+    bits = a.view(np.uint32)
+    want = ((bits + np.uint32(0x7FFF) + ((bits >> 16) & 1)) & np.uint32(0xFFFF0000)).view(np.float32)
+    g = rt.Grid(shape, storage=rt.STORAGE_BF16)
+    g.copyin(a)
+    assert np.array_equal(g.copyout().ravel(), want)
+    g.free()
+
+
 def test_new_grid_is_zero_filled_and_double_grids(rt):
     g = rt.Grid((5, 4, 3), dtype=np.float64)
     assert np.all(g.copyout() == 0)
