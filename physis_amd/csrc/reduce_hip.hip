@@ -96,39 +96,47 @@ int BlocksFor(int64_t n) {
 }
 
 template <class S, class Op>
-void Launch(const void *p, int64_t n, double *scratch, hipStream_t s) {
+void Launch(const void *p, int64_t n, double *partials, double *out, hipStream_t s) {
   int nb = BlocksFor(n);
   hipLaunchKernelGGL((reduce_pass1<S, Op>), dim3(nb), dim3(kThreads), 0, s,
-                     static_cast<const typename S::stored_t *>(p), n, scratch + 1);
-  hipLaunchKernelGGL((reduce_pass2<Op>), dim3(1), dim3(kThreads), 0, s, scratch + 1, nb, scratch);
+                     static_cast<const typename S::stored_t *>(p), n, partials);
+  hipLaunchKernelGGL((reduce_pass2<Op>), dim3(1), dim3(kThreads), 0, s, partials, nb, out);
 }
 
 template <class S>
-int Dispatch(const void *p, int64_t n, enum PSReduceOp op, double *scratch, hipStream_t s) {
+int Dispatch(const void *p, int64_t n, enum PSReduceOp op, double *partials, double *out, hipStream_t s) {
   switch (op) {
-    case PS_SUM: Launch<S, psk::OpSum>(p, n, scratch, s); break;
-    case PS_PROD: Launch<S, psk::OpProd>(p, n, scratch, s); break;
-    case PS_MAX: Launch<S, psk::OpMax>(p, n, scratch, s); break;
-    case PS_MIN: Launch<S, psk::OpMin>(p, n, scratch, s); break;
+    case PS_SUM: Launch<S, psk::OpSum>(p, n, partials, out, s); break;
+    case PS_PROD: Launch<S, psk::OpProd>(p, n, partials, out, s); break;
+    case PS_MAX: Launch<S, psk::OpMax>(p, n, partials, out, s); break;
+    case PS_MIN: Launch<S, psk::OpMin>(p, n, partials, out, s); break;
     default: return PS_HIP_ERR_INVALID;
   }
   return hipGetLastError() == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
 }
 
+// Queues the reduction with the partials at `partials` (kMaxBlocks doubles)
+// and the result written to `out` (device memory, or pinned host memory the
+// device can address).
+int ReduceTo(const void *p, int64_t n, int elm_size, int storage, enum PSReduceOp op, double *partials,
+             double *out, hipStream_t s) {
+  if (n <= 0) return PS_HIP_ERR_INVALID;
+  if (storage == PS_HIP_STORAGE_BF16) return Dispatch<psk::BF16>(p, n, op, partials, out, s);
+  if (storage == PS_HIP_STORAGE_FP16) return Dispatch<psk::FP16>(p, n, op, partials, out, s);
+  if (elm_size == 4) return Dispatch<psk::Native<float>>(p, n, op, partials, out, s);
+  if (elm_size == 8) return Dispatch<psk::Native<double>>(p, n, op, partials, out, s);
+  return PS_HIP_ERR_UNSUPPORTED;
+}
+
 }  // namespace
 
 // Queues the reduction of n elements at p (storage kind `storage`, element
-// size elm_size) and leaves the FP64 result in device memory at *result_dev.
+// size elm_size) and leaves the FP64 result in device memory at *result_dev
+// (partials in result_dev[1 .. ReduceScratchDoubles())).
 int ReduceAsync(const void *p, int64_t n, int elm_size, int storage, enum PSReduceOp op,
                 double *result_dev, double *partials_dev, hipStream_t s) {
   (void)partials_dev;
-  if (n <= 0) return PS_HIP_ERR_INVALID;
-  double *scratch = result_dev;
-  if (storage == PS_HIP_STORAGE_BF16) return Dispatch<psk::BF16>(p, n, op, scratch, s);
-  if (storage == PS_HIP_STORAGE_FP16) return Dispatch<psk::FP16>(p, n, op, scratch, s);
-  if (elm_size == 4) return Dispatch<psk::Native<float>>(p, n, op, scratch, s);
-  if (elm_size == 8) return Dispatch<psk::Native<double>>(p, n, op, scratch, s);
-  return PS_HIP_ERR_UNSUPPORTED;
+  return ReduceTo(p, n, elm_size, storage, op, result_dev + 1, result_dev, s);
 }
 
 // Scratch doubles a reduction of any size needs: result + partials.
@@ -146,14 +154,17 @@ int PSHipReduceGrid(double *out, enum PSReduceOp op, __PSGrid *g) {
   GridRec *r = Rec(g);
   Runtime &rt = Runtime::Get();
   hipStream_t s = rt.stream();
-  double *scratch = static_cast<double *>(rt.Scratch(ReduceScratchDoubles() * sizeof(double)));
-  int st = ReduceAsync(g->p0, g->num_elms, g->elm_size, r->storage, op, scratch, nullptr, s);
-  if (st != PS_HIP_OK) return st;
+  double *partials = static_cast<double *>(rt.Scratch(ReduceScratchDoubles() * sizeof(double)));
+  // The folding kernel stores the result straight into pinned host memory
+  // (no copy to queue); it is visible once the stream has synchronised.
   double *host = static_cast<double *>(rt.PinnedScratch(sizeof(double)));
-  if (hipMemcpyAsync(host, scratch, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
+  double *host_dev = nullptr;
+  if (hipHostGetDevicePointer(reinterpret_cast<void **>(&host_dev), host, 0) != hipSuccess)
     return PS_HIP_ERR_DEVICE;
+  int st = ReduceTo(g->p0, g->num_elms, g->elm_size, r->storage, op, partials, host_dev, s);
+  if (st != PS_HIP_OK) return st;
   if (hipStreamSynchronize(s) != hipSuccess) return PS_HIP_ERR_DEVICE;
-  *out = *host;
+  *out = *static_cast<volatile double *>(host);
   return PS_HIP_OK;
 }
 
