@@ -33,6 +33,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "physis/hip/skeleton.h"
 #include "physis/hip/skeleton_star.h"
 
@@ -44,8 +46,15 @@ struct TBGeometry {
   int remap;
 };
 
-template <class Plan, int NWX, int BY>
-__global__ void __launch_bounds__(64 * NWX * BY)
+// kRenamed selects the loop schedule.  false: the next u0 plane is loaded
+// before the barrier into a fourth register set and the queues are shifted
+// by register moves at the end of each step.  true: the queues hold three
+// planes each and rotate their roles over an unrolled period of three steps
+// (no moves); the next u0 plane is loaded, after u1 is computed, into the
+// set of the plane u1 no longer needs, and halo rows load straight into
+// their own registers once published.
+template <class Plan, int NWX, int BY, bool kRenamed = false>
+__global__ void __launch_bounds__(64 * NWX * BY, kRenamed ? 8 : 1)
 ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
   using S = typename Plan::Storage;
   using st = typename S::stored_t;
@@ -55,7 +64,12 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
   __shared__ float L0[2][BY + 2][ROW];
   __shared__ float L1[2][BY][ROW];
 
-  const int lane = threadIdx.x & 63, wx = threadIdx.x >> 6, wy = threadIdx.y;
+  // wx and wy are the same for every lane of a wave (blockDim.x is a
+  // multiple of 64); broadcasting them lets the compiler keep row offsets in
+  // scalar registers and branch on row roles without exec-mask updates.
+  const int lane = threadIdx.x & 63;
+  const int wx = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int tile = geo.remap ? XcdTile(blockIdx.x, geo.tiles_y) : (int)blockIdx.x;
   int nx, ny, nz;
   Plan::stage_dims(a1, nx, ny, nz);
@@ -105,6 +119,109 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
     v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
   };
 
+  // One u1 point row at plane k (this wave's row) from u0 planes b, c, t.
+  auto level1 = [&](int buf, int kf, const float (&b)[4], const float (&c)[4], const float (&t)[4],
+                    float (&u)[4]) {
+    float nv[4], sv[4];
+    get4(&L0[buf][wy][0], nv);
+    get4(&L0[buf][wy + 2][0], sv);
+    const float wl = L0[buf][wy + 1][xs + 3], er = L0[buf][wy + 1][xs + 8];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      StarCtx q;
+      q.c = c[j];
+      q.w = j > 0 ? c[j - 1] : wl;
+      q.e = j < 3 ? c[j + 1] : er;
+      q.n = nv[j];
+      q.s = sv[j];
+      q.b = b[j];
+      q.t = t[j];
+      u[j] = RoundTrip<S>(ComputeAt<Plan, true, true, true>(q, a1, xs + j, yf, kf));
+    }
+  };
+  // One u2 output row at plane k - 1 from u1 planes b, c, t.  Clamped plans:
+  // at a y face the row's own u1 values stand in for the missing neighbour
+  // row (a wave-uniform choice of LDS row), so only the z faces need selects.
+  auto level2 = [&](int buf, int k, const float (&b)[4], const float (&c)[4], const float (&t)[4]) {
+    float n1[4], s1[4], v[4];
+    const int rn = !W && y == 0 ? wy : wy - 1, rs = !W && y == ny - 1 ? wy : wy + 1;
+    get4(&L1[buf][rn][0], n1);
+    get4(&L1[buf][rs][0], s1);
+    const float w1 = L1[buf][wy][xs + 3], e1 = L1[buf][wy][xs + 8];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      StarCtx q;
+      q.c = c[j];
+      q.w = j > 0 ? c[j - 1] : w1;
+      q.e = j < 3 ? c[j + 1] : e1;
+      q.n = n1[j];
+      q.s = s1[j];
+      q.b = b[j];
+      q.t = t[j];
+      v[j] = ComputeAt<Plan, true, true, false>(q, a2, xs + j, y, k - 1);
+    }
+    if (has) Store4<S>(out + (int64_t)(k - 1) * plane + (int64_t)y * nx + xs, v);
+  };
+
+  if constexpr (kRenamed) {
+    // A wave owns at most one of the two halo rows (BY >= 3): one register
+    // set and one row offset serve both.
+    const bool halo = halo_lo || halo_hi;
+    const int hslot = halo_lo ? 0 : BY + 1;
+    // Every lane loads, without a branch: a lane past the row end reads the
+    // row's last four points (never stored or published).  Loads outside
+    // divergent branches let the compiler wait for just the load it needs
+    // rather than for all of them.
+    const int xl = has ? xs : nx - 4;
+    const int64_t lrow = (int64_t)yf * nx + xl;
+    const int64_t hrow = (int64_t)FixIndex<W>(halo_lo ? ya - 1 : ya + BY, ny) * nx + xl;
+    auto ldu = [&](int64_t off, float (&v)[4]) { Load4<S>(in + off, v); };
+    float Q0[3][4], Q1[3][4] = {}, hv[4] = {};
+    ldu(zoff(zb - 2) + lrow, Q0[0]);
+    ldu(zoff(zb - 1) + lrow, Q0[1]);
+    ldu(zoff(zb) + lrow, Q0[2]);
+    if (halo) ldu(zoff(zb - 1) + hrow, hv);
+    // Plane indices of k (z0), k + 1 (z1) and k + 2 (z2), as below.
+    int z0 = FixIndex<W>(zb - 1, nz), z1 = FixIndex<W>(zb, nz), z2 = FixIndex<W>(zb + 1, nz);
+    auto advance = [&](int z) { return W ? (z + 1 == nz ? 0 : z + 1) : min(z + 1, nz - 1); };
+    // Step k with role rotation r = (k - zb + 1) % 3: u0 planes k-1, k, k+1
+    // in Q0[r], Q0[r+1], Q0[r+2]; u1 planes k-2, k-1 in Q1[r], Q1[r+1], u1
+    // plane k into Q1[r+2] (indices mod 3).
+    auto step = [&](int k, auto rc) {
+      constexpr int r = decltype(rc)::value;
+      constexpr int i0 = r, i1 = (r + 1) % 3, i2 = (r + 2) % 3;
+      const int buf = (k + 2) & 1;
+      put(&L0[buf][wy + 1][0], Q0[i1]);
+      if (halo) put(&L0[buf][hslot][0], hv);
+      put(&L1[buf][wy][0], Q1[i1]);
+      // At k == ze the planes loaded below are not used (z1, z2 are in-grid
+      // indices whatever k is, so the loads stay in bounds).
+      if (halo) ldu((int64_t)z1 * plane + hrow, hv);
+      __syncthreads();
+      // Lanes past the row end compute on in-bounds LDS garbage that is
+      // never published or stored.
+      level1(buf, z0, Q0[i0], Q0[i1], Q0[i2], Q1[i2]);
+      // Q0[i0] (plane k-1) is dead: it receives plane k+2.
+      ldu((int64_t)z2 * plane + lrow, Q0[i0]);
+      if (out_row && k - 1 >= zb) level2(buf, k, Q1[i0], Q1[i1], Q1[i2]);
+      z0 = z1;
+      z1 = z2;
+      z2 = advance(z2);
+    };
+    using R0 = std::integral_constant<int, 0>;
+    using R1 = std::integral_constant<int, 1>;
+    using R2 = std::integral_constant<int, 2>;
+    int k = zb - 1;
+    for (; k + 2 <= ze; k += 3) {
+      step(k, R0{});
+      step(k + 1, R1{});
+      step(k + 2, R2{});
+    }
+    if (k <= ze) step(k, R0{});
+    if (k + 1 <= ze) step(k + 1, R1{});
+    return;
+  }
+
   float b0[4], c0[4], t0[4], p0[4] = {}, hlo[4] = {}, hhi[4] = {}, nlo[4] = {}, nhi[4] = {};
   float b1[4] = {}, c1[4] = {}, t1[4] = {};
   ld(zoff(zb - 2) + row, b0);
@@ -132,48 +249,8 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
     }
     __syncthreads();
     if (has) {
-      // u1 at plane k, this wave's row.
-      float nv[4], sv[4];
-      get4(&L0[buf][wy][0], nv);
-      get4(&L0[buf][wy + 2][0], sv);
-      const float wl = L0[buf][wy + 1][xs + 3], er = L0[buf][wy + 1][xs + 8];
-      const int kf = z0;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        StarCtx c;
-        c.c = c0[j];
-        c.w = j > 0 ? c0[j - 1] : wl;
-        c.e = j < 3 ? c0[j + 1] : er;
-        c.n = nv[j];
-        c.s = sv[j];
-        c.b = b0[j];
-        c.t = t0[j];
-        t1[j] = RoundTrip<S>(ComputeAt<Plan, true, true, true>(c, a1, xs + j, yf, kf));
-      }
-      // u2 at plane k-1 on the output rows.
-      if (out_row && k - 1 >= zb) {
-        float n1[4], s1[4], v[4];
-        // Clamped plans: at a y face the row's own u1 values stand in for
-        // the missing neighbour row (a wave-uniform choice of LDS row), so
-        // only the z faces need selects in u2.
-        const int rn = !W && y == 0 ? wy : wy - 1, rs = !W && y == ny - 1 ? wy : wy + 1;
-        get4(&L1[buf][rn][0], n1);
-        get4(&L1[buf][rs][0], s1);
-        const float w1 = L1[buf][wy][xs + 3], e1 = L1[buf][wy][xs + 8];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          StarCtx c;
-          c.c = c1[j];
-          c.w = j > 0 ? c1[j - 1] : w1;
-          c.e = j < 3 ? c1[j + 1] : e1;
-          c.n = n1[j];
-          c.s = s1[j];
-          c.b = b1[j];
-          c.t = t1[j];
-          v[j] = ComputeAt<Plan, true, true, false>(c, a2, xs + j, y, k - 1);
-        }
-        Store4<S>(out + (int64_t)(k - 1) * plane + (int64_t)y * nx + xs, v);
-      }
+      level1(buf, z0, b0, c0, t0, t1);  // u1 at plane k, this wave's row
+      if (out_row && k - 1 >= zb) level2(buf, k, b1, c1, t1);  // u2 at plane k-1
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -229,7 +306,7 @@ inline TB2Shape ChooseTB2Shape(int nx, int ny, int nz) {
   return sh;
 }
 
-template <class Plan, int NWX, int BY>
+template <class Plan, int NWX, int BY, bool kRenamed = false>
 inline hipError_t LaunchZMarchStarTB2(const typename Plan::Args &a1, const typename Plan::Args &a2, int ny,
                                       int nz, int zchunk, bool remap, hipStream_t stream) {
   TBGeometry geo;
@@ -238,7 +315,8 @@ inline hipError_t LaunchZMarchStarTB2(const typename Plan::Args &a1, const typen
   geo.zchunk = zchunk;
   geo.remap = remap ? 1 : 0;
   dim3 grid(geo.tiles_y, (nz + zchunk - 1) / zchunk, 1);
-  hipLaunchKernelGGL((ZMarchStar4TB2<Plan, NWX, BY>), grid, dim3(64 * NWX, BY, 1), 0, stream, a1, a2, geo);
+  hipLaunchKernelGGL((ZMarchStar4TB2<Plan, NWX, BY, kRenamed>), grid, dim3(64 * NWX, BY, 1), 0, stream, a1, a2,
+                     geo);
   return hipGetLastError();
 }
 
@@ -249,11 +327,11 @@ inline hipError_t LaunchZMarchStarTB2Auto(const typename Plan::Args &a1, const t
                                           int ny, int nz, hipStream_t stream) {
   const TB2Shape sh = ChooseTB2Shape(nx, ny, nz);
   if (sh.nwx == 1 && StarTB2Supported<1>(nx, ny, nz))
-    return LaunchZMarchStarTB2<Plan, 1, 8>(a1, a2, ny, nz, sh.zchunk, true, stream);
+    return LaunchZMarchStarTB2<Plan, 1, 8, true>(a1, a2, ny, nz, sh.zchunk, true, stream);
   if (sh.nwx == 2 && StarTB2Supported<2>(nx, ny, nz))
-    return LaunchZMarchStarTB2<Plan, 2, 8>(a1, a2, ny, nz, sh.zchunk, true, stream);
+    return LaunchZMarchStarTB2<Plan, 2, 8, true>(a1, a2, ny, nz, sh.zchunk, true, stream);
   if (sh.nwx == 4 && StarTB2Supported<4>(nx, ny, nz))
-    return LaunchZMarchStarTB2<Plan, 4, 4>(a1, a2, ny, nz, sh.zchunk, true, stream);
+    return LaunchZMarchStarTB2<Plan, 4, 4, true>(a1, a2, ny, nz, sh.zchunk, true, stream);
   return hipErrorNotSupported;
 }
 

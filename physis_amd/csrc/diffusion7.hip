@@ -108,6 +108,7 @@ struct Variant {
   int zchunk;  // 0 = whole column
   bool remap;
   int rows = 1;  // family 1: tile rows per thread
+  bool renamed = false;  // family 4: the loop schedule without queue moves
 };
 
 const Variant kVariants[] = {
@@ -146,6 +147,10 @@ const Variant kVariants[] = {
     {"tb2_w1_by16_z8", 4, 1, 16, 8, true},
     {"tb2_w2_by8_z8", 4, 2, 8, 8, true},
     {"tb2_w2_by6_z16", 4, 2, 6, 16, true},
+    {"tb2r_w2_by8_z32", 4, 2, 8, 32, true, 1, true},
+    {"tb2r_w2_by8_z16", 4, 2, 8, 16, true, 1, true},
+    {"tb2r_w1_by8_z32", 4, 1, 8, 32, true, 1, true},
+    {"tb2r_w1_by16_z32", 4, 1, 16, 32, true, 1, true},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 // Measured on MI355X at 512^3 FP32 (bench.py --sweep): star4_by8_z128 ~710
@@ -168,6 +173,17 @@ hipError_t LaunchPair(const Variant &v, const void *src, void *dst, const int *d
   a2.g2 = static_cast<st *>(dst);
   for (int d = 0; d < 3; ++d) a1.dim[d] = a2.dim[d] = dim[d];
   a1.k = a2.k = k;
+#define PSH_TBR(NWX, BY)                                                                          \
+  if (v.renamed && v.bx == NWX && v.by == BY) {                                                   \
+    if (!psk::StarTB2Supported<NWX>(dim[0], dim[1], dim[2])) return hipErrorNotSupported;       \
+    return psk::LaunchZMarchStarTB2<P, NWX, BY, true>(a1, a2, dim[1], dim[2], v.zchunk, v.remap, s); \
+  }
+  PSH_TBR(2, 8)
+  PSH_TBR(1, 8)
+  PSH_TBR(1, 16)
+  PSH_TBR(4, 4)
+#undef PSH_TBR
+  if (v.renamed) return hipErrorInvalidValue;
 #define PSH_TB(NWX, BY)                                                                     \
   if (v.bx == NWX && v.by == BY) {                                                          \
     if (!psk::StarTB2Supported<NWX>(dim[0], dim[1], dim[2])) return hipErrorNotSupported; \
@@ -348,7 +364,7 @@ int PSHipDiffusion7Run(__PSGrid *g1, __PSGrid *g2, const __PSDomain *dom, const 
   // Variant 0: two fused steps per pass with the shape psk::ChooseTB2Shape
   // picks for this grid.
   const psk::TB2Shape sh = psk::ChooseTB2Shape(g1->dim[0], g1->dim[1], g1->dim[2]);
-  const Variant autov{"tb2_auto", 4, sh.nwx, sh.by, sh.zchunk, true};
+  const Variant autov{"tb2_auto", 4, sh.nwx, sh.by, sh.zchunk, true, 1, true};
   const Variant &v = variant == 0 ? autov : kVariants[variant - 1];
   Coef7 k{coef[0], coef[1], coef[2], coef[3], coef[4], coef[5], coef[6]};
   psk::Box box = psk::BoxFromDomain(*dom, 3);
