@@ -46,6 +46,21 @@ struct TBGeometry {
   int remap;
 };
 
+// Shared memory of one fused tile: u0 rows (with the two halo rows) and u1
+// rows of a plane, double-buffered by step parity; four pad floats at each
+// row end.
+template <int NWX, int BY>
+struct StarTB2Lds {
+  static constexpr int kRow = 256 * NWX + 8;
+  float u0[2][BY + 2][kRow];
+  float u1[2][BY][kRow];
+};
+
+// One fused tile: u1 on rows [ya, ya+BY) and u2 on rows [ya+1, ya+BY-1)
+// (those inside the grid), planes [zb, ze), of a grid of whole 16-byte rows
+// (nx <= 256*NWX); called by the single-grid kernel below and by batched
+// launches over many grids.  Block: 64*NWX x BY threads.
+//
 // kRenamed selects the loop schedule.  false: the next u0 plane is loaded
 // before the barrier into a fourth register set and the queues are shifted
 // by register moves at the end of each step.  true: the queues hold three
@@ -53,16 +68,15 @@ struct TBGeometry {
 // (no moves); the next u0 plane is loaded, after u1 is computed, into the
 // set of the plane u1 no longer needs, and halo rows load straight into
 // their own registers once published.
-template <class Plan, int NWX, int BY, bool kRenamed = false>
-__global__ void __launch_bounds__(64 * NWX * BY, kRenamed ? 8 : 1)
-ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
+template <class Plan, int NWX, int BY, bool kRenamed>
+__device__ __forceinline__ void StarTB2Tile(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx,
+                                            int ny, int nz, int ya, int zb, int ze, StarTB2Lds<NWX, BY> &L) {
   using S = typename Plan::Storage;
   using st = typename S::stored_t;
   constexpr bool W = Plan::kWrap;
-  constexpr int WX = 256 * NWX, ROW = WX + 8;
   static_assert(BY >= 3, "need at least one output row");
-  __shared__ float L0[2][BY + 2][ROW];
-  __shared__ float L1[2][BY][ROW];
+  auto &L0 = L.u0;
+  auto &L1 = L.u1;
 
   // wx and wy are the same for every lane of a wave (blockDim.x is a
   // multiple of 64); broadcasting them lets the compiler keep row offsets in
@@ -70,13 +84,7 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
   const int lane = threadIdx.x & 63;
   const int wx = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int tile = geo.remap ? XcdTile(blockIdx.x, geo.tiles_y) : (int)blockIdx.x;
-  int nx, ny, nz;
-  Plan::stage_dims(a1, nx, ny, nz);
-  const int ya = tile * (BY - 2) - 1;
   const int y = ya + wy;
-  const int zb = (int)blockIdx.y * geo.zchunk;
-  const int ze = min(zb + geo.zchunk, nz);
   const int xs = wx * 256 + lane * 4;
   const bool has = xs < nx;
   const int64_t plane = (int64_t)nx * ny;
@@ -266,6 +274,18 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
     z1 = z2;
     z2 = advance(z2);
   }
+}
+
+template <class Plan, int NWX, int BY, bool kRenamed = false>
+__global__ void __launch_bounds__(64 * NWX * BY, kRenamed ? 8 : 1)
+ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
+  __shared__ StarTB2Lds<NWX, BY> L;
+  const int tile = geo.remap ? XcdTile(blockIdx.x, geo.tiles_y) : (int)blockIdx.x;
+  int nx, ny, nz;
+  Plan::stage_dims(a1, nx, ny, nz);
+  const int zb = (int)blockIdx.y * geo.zchunk;
+  const int ze = min(zb + geo.zchunk, nz);
+  StarTB2Tile<Plan, NWX, BY, kRenamed>(a1, a2, nx, ny, nz, tile * (BY - 2) - 1, zb, ze, L);
 }
 
 // Whether a grid can take the fused two-step path with NWX waves per row.

@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "physis/hip/skeleton.h"
 #include "physis/hip/skeleton_star.h"
 
@@ -151,7 +153,10 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
   using S = typename Plan::Storage;
   using st = typename S::stored_t;
   constexpr bool W = Plan::kWrap;
-  const int lane = threadIdx.x, wy = threadIdx.y;
+  // wy is the same for every lane of a wave (blockDim.x == 64): broadcast,
+  // it keeps row offsets scalar and row-role branches free of exec masks.
+  const int lane = threadIdx.x;
+  const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int y = ya + wy;
   const int yf = FixIndex<W>(y, ny);
   const int64_t plane = (int64_t)nx * ny;
@@ -159,83 +164,102 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
   st *out = Plan::out_ptr(a2);
   const int64_t row = (int64_t)yf * nx;
   const bool halo_lo = wy == 0, halo_hi = wy == BY - 1;
-  const int64_t hlo_row = (int64_t)FixIndex<W>(ya - 1, ny) * nx;
-  const int64_t hhi_row = (int64_t)FixIndex<W>(ya + BY, ny) * nx;
+  // A wave owns at most one of the two halo rows (BY >= 3).
+  const bool halo = halo_lo || halo_hi;
+  const int hslot = halo_lo ? 0 : BY + 1;
+  const int64_t hrow = (int64_t)FixIndex<W>(halo_lo ? ya - 1 : ya + BY, ny) * nx;
   const bool out_row = wy >= 1 && wy <= BY - 2 && y >= 0 && y < ny;
   auto zo = [&](int z) { return (int64_t)FixIndex<W>(z, nz) * plane; };
-
-  float b0[NP], c0[NP], t0[NP], p0[NP] = {}, hlo[NP] = {}, hhi[NP] = {}, nlo[NP] = {}, nhi[NP] = {};
-  float b1[NP] = {}, c1[NP] = {}, t1[NP] = {};
-  StrideLoad<S, NP>(in + zo(zb - 2) + row, b0, lane, nx);
-  StrideLoad<S, NP>(in + zo(zb - 1) + row, c0, lane, nx);
-  StrideLoad<S, NP>(in + zo(zb) + row, t0, lane, nx);
-  if (halo_lo) StrideLoad<S, NP>(in + zo(zb - 1) + hlo_row, hlo, lane, nx);
-  if (halo_hi) StrideLoad<S, NP>(in + zo(zb - 1) + hhi_row, hhi, lane, nx);
-  // Wrapped/clamped indices of planes k, k+1, k+2.
-  int z0 = FixIndex<W>(zb - 1, nz), z1 = FixIndex<W>(zb, nz), z2 = FixIndex<W>(zb + 1, nz);
-  for (int k = zb - 1; k <= ze; ++k) {
-    const int buf = (k + 2) & 1;
-    auto &R0 = L.u0[buf];
-    auto &R1 = L.u1[buf];
-    StridePut<W, NP>(R0[wy + 1], c0, lane, nx);
-    if (halo_lo) StridePut<W, NP>(R0[0], hlo, lane, nx);
-    if (halo_hi) StridePut<W, NP>(R0[BY + 1], hhi, lane, nx);
-    StridePut<W, NP>(R1[wy], c1, lane, nx);
-    if (k < ze) {
-      StrideLoad<S, NP>(in + (int64_t)z2 * plane + row, p0, lane, nx);
-      if (halo_lo) StrideLoad<S, NP>(in + (int64_t)z1 * plane + hlo_row, nlo, lane, nx);
-      if (halo_hi) StrideLoad<S, NP>(in + (int64_t)z1 * plane + hhi_row, nhi, lane, nx);
-    }
-    __syncthreads();
-    const int kf = z0;
-    z0 = z1;
-    z1 = z2;
-    z2 = NextPlane<W>(z2, nz);
+  // Every lane loads without a branch; points past the row end read the
+  // row's last point and are never published or stored.  (Loads outside
+  // divergent branches let the compiler wait for just the load it needs.)
+  int xc[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) xc[q] = min(lane + 64 * q, nx - 1);
+  auto ld = [&](const st *r, float (&v)[NP]) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q) v[q] = S::load(r + xc[q]);
+  };
+  // u1 at plane k (this wave's row) from u0 planes b, c, t.
+  auto level1 = [&](float (*R0)[StrideTB2Lds<NP, BY>::kRow], int kf, const float (&b)[NP], const float (&c)[NP],
+                    const float (&t)[NP], float (&u)[NP]) {
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
       const int x = lane + 64 * q;
-      if (x < nx) {
-        StarCtx cx;
-        cx.c = c0[q];
-        cx.w = R0[wy + 1][x + 3];
-        cx.e = R0[wy + 1][x + 5];
-        cx.n = R0[wy][x + 4];
-        cx.s = R0[wy + 2][x + 4];
-        cx.b = b0[q];
-        cx.t = t0[q];
-        t1[q] = RoundTrip<S>(ComputeAt<Plan, false, true, true>(cx, a1, x, yf, kf));
-      }
+      StarCtx cx;
+      cx.c = c[q];
+      cx.w = R0[wy + 1][x + 3];
+      cx.e = R0[wy + 1][x + 5];
+      cx.n = R0[wy][x + 4];
+      cx.s = R0[wy + 2][x + 4];
+      cx.b = b[q];
+      cx.t = t[q];
+      u[q] = RoundTrip<S>(ComputeAt<Plan, false, true, true>(cx, a1, x, yf, kf));
     }
-    if (out_row && k - 1 >= zb) {
-      st *o = out + (int64_t)(k - 1) * plane + (int64_t)y * nx;
-      const int rn = !W && y == 0 ? wy : wy - 1, rs = !W && y == ny - 1 ? wy : wy + 1;
-#pragma unroll
-      for (int q = 0; q < NP; ++q) {
-        const int x = lane + 64 * q;
-        if (x < nx) {
-          StarCtx cx;
-          cx.c = c1[q];
-          cx.w = R1[wy][x + 3];
-          cx.e = R1[wy][x + 5];
-          cx.n = R1[rn][x + 4];
-          cx.s = R1[rs][x + 4];
-          cx.b = b1[q];
-          cx.t = t1[q];
-          S::store(o + x, ComputeAt<Plan, false, true, false>(cx, a2, x, y, k - 1));
-        }
-      }
-    }
+  };
+  // u2 output row at plane k - 1 from u1 planes b, c, t.
+  auto level2 = [&](float (*R1)[StrideTB2Lds<NP, BY>::kRow], int k, const float (&b)[NP], const float (&c)[NP],
+                    const float (&t)[NP]) {
+    st *o = out + (int64_t)(k - 1) * plane + (int64_t)y * nx;
+    const int rn = !W && y == 0 ? wy : wy - 1, rs = !W && y == ny - 1 ? wy : wy + 1;
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
-      b0[q] = c0[q];
-      c0[q] = t0[q];
-      t0[q] = p0[q];
-      hlo[q] = nlo[q];
-      hhi[q] = nhi[q];
-      b1[q] = c1[q];
-      c1[q] = t1[q];
+      const int x = lane + 64 * q;
+      StarCtx cx;
+      cx.c = c[q];
+      cx.w = R1[wy][x + 3];
+      cx.e = R1[wy][x + 5];
+      cx.n = R1[rn][x + 4];
+      cx.s = R1[rs][x + 4];
+      cx.b = b[q];
+      cx.t = t[q];
+      const float v = ComputeAt<Plan, false, true, false>(cx, a2, x, y, k - 1);
+      if (x < nx) S::store(o + x, v);
     }
+  };
+
+  // The queues hold three planes each and rotate their roles over an
+  // unrolled period of three steps (no register moves): at step k with
+  // r = (k - zb + 1) % 3, u0 planes k-1, k, k+1 are Q0[r], Q0[r+1], Q0[r+2]
+  // and u1 planes k-2, k-1 are Q1[r], Q1[r+1]; u1 plane k goes to Q1[r+2]
+  // and u0 plane k+2 is loaded into Q0[r] once u1 no longer needs it.
+  float Q0[3][NP], Q1[3][NP] = {}, hv[NP] = {};
+  ld(in + zo(zb - 2) + row, Q0[0]);
+  ld(in + zo(zb - 1) + row, Q0[1]);
+  ld(in + zo(zb) + row, Q0[2]);
+  if (halo) ld(in + zo(zb - 1) + hrow, hv);
+  // Wrapped/clamped indices of planes k, k+1, k+2.
+  int z0 = FixIndex<W>(zb - 1, nz), z1 = FixIndex<W>(zb, nz), z2 = FixIndex<W>(zb + 1, nz);
+  auto step = [&](int k, auto rc) {
+    constexpr int r = decltype(rc)::value;
+    constexpr int i0 = r, i1 = (r + 1) % 3, i2 = (r + 2) % 3;
+    const int buf = (k + 2) & 1;
+    auto &R0 = L.u0[buf];
+    auto &R1 = L.u1[buf];
+    StridePut<W, NP>(R0[wy + 1], Q0[i1], lane, nx);
+    if (halo) StridePut<W, NP>(R0[hslot], hv, lane, nx);
+    StridePut<W, NP>(R1[wy], Q1[i1], lane, nx);
+    // z1 is an in-grid index for every k: at k == ze the load is unused.
+    if (halo) ld(in + (int64_t)z1 * plane + hrow, hv);
+    __syncthreads();
+    level1(R0, z0, Q0[i0], Q0[i1], Q0[i2], Q1[i2]);
+    ld(in + (int64_t)z2 * plane + row, Q0[i0]);
+    if (out_row && k - 1 >= zb) level2(R1, k, Q1[i0], Q1[i1], Q1[i2]);
+    z0 = z1;
+    z1 = z2;
+    z2 = NextPlane<W>(z2, nz);
+  };
+  using C0 = std::integral_constant<int, 0>;
+  using C1 = std::integral_constant<int, 1>;
+  using C2 = std::integral_constant<int, 2>;
+  int k = zb - 1;
+  for (; k + 2 <= ze; k += 3) {
+    step(k, C0{});
+    step(k + 1, C1{});
+    step(k + 2, C2{});
   }
+  if (k <= ze) step(k, C0{});
+  if (k + 1 <= ze) step(k + 1, C1{});
 }
 
 // One map over one whole grid: workgroup (blockIdx.x, blockIdx.y) = (y tile

@@ -163,9 +163,11 @@ extern int PSHipBatchRun(void *batch, int steps, int use_graph);
 extern int PSHipBatchRunOn(void *batch, int steps, void *stream);
 /* Launch shape: points per lane np (row width 64*np), rows per workgroup by,
  * planes per work item zchunk, temporal degree 1 or 2.  np = 0: mixed mode,
- * every grid with the fewest points per lane that cover its rows and each
- * such width class on its own stream (forked from and joined to the
- * caller's). */
+ * every grid in its class (the fewest points per lane that cover its rows;
+ * grids of whole 16-byte rows longer than 128 points: the vectorised star
+ * tile) and each class on its own stream (forked from and joined to the
+ * caller's).  np = -1: the mixed mode with 16 rows per workgroup for the
+ * classes of 1 and 2 points per lane. */
 extern int PSHipBatchConfigure(void *batch, int np, int by, int zchunk, int degree);
 extern int PSHipBatchConfigName(void *batch, char *buf, int len);
 /* Times every candidate shape for `steps` steps on the batch (contents

@@ -20,6 +20,7 @@
 #include "diffusion7_body.h"
 #include "physis/hip/skeleton.h"
 #include "physis/hip/skeleton_star.h"
+#include "physis/hip/skeleton_star_tb.h"
 #include "physis/hip/skeleton_stride.h"
 #include "physis/hip/storage.h"
 
@@ -94,15 +95,44 @@ BatchTB2(const typename P::Args *in_args, const typename P::Args *out_args, cons
   }
 }
 
+// Classes of the mixed mode: the stride widths (ClassNp below), and two classes for
+// grids of whole 16-byte rows longer than 128 points, whose fused steps run on the vectorised star
+// tile (psk::StarTB2Tile, 8 rows per workgroup, one or two waves per row of
+// up to 256 or 512 points) and whose unfused steps on the 4- or 8-point
+// stride kernels.
+constexpr int kStar1 = 101, kStar2 = 102;
+constexpr int kMaxClasses = 8;
+constexpr int kStarBY = 8;
+inline bool IsStar(int cls) { return cls == kStar1 || cls == kStar2; }
+inline int StarNwx(int cls) { return cls == kStar1 ? 1 : 2; }
+// Stride width of a class's one-step kernel.
+inline int StepNp(int cls) { return IsStar(cls) ? (cls == kStar1 ? 4 : 8) : cls; }
+
+template <class P, int NWX>
+__global__ void __launch_bounds__(64 * NWX * kStarBY, 8)
+BatchStarTB2(const typename P::Args *in_args, const typename P::Args *out_args, const Item *items, int nitems) {
+  __shared__ psk::StarTB2Lds<NWX, kStarBY> L;
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const Item it = items[w];
+    const typename P::Args a1 = in_args[it.grid];
+    const typename P::Args a2 = out_args[it.grid];
+    psk::StarTB2Tile<P, NWX, kStarBY, true>(a1, a2, a1.dim[0], a1.dim[1], a1.dim[2], it.y0, it.zb, it.ze, L);
+    __syncthreads();
+  }
+}
+
 // A launch configuration of the batch.
 struct Config {
   int np;      // points per lane (row width 64*np); with `mixed`, of one class
   int by;      // rows per workgroup
   int zchunk;  // planes per work item
   int degree;  // 1: one step per launch; 2: two fused steps
-  // 1: every grid runs with the fewest points per lane that cover its rows;
-  // the grids of each such width class run on their own stream (forked from
-  // and joined to the caller's), so the classes overlap.
+  // 1: every grid runs in its class (the fewest points per lane that cover
+  // its rows, or a star class); the grids of each class run on their own
+  // stream (forked from and joined to the caller's), so the classes overlap.
+  // 2: the same, with 16 rows per workgroup for the 1- and 2-point classes
+  // (their kernels keep full occupancy at 16 rows, which halves the share
+  // of halo rows).
   int mixed = 0;
   bool operator<(const Config &o) const {
     return std::tie(np, by, zchunk, degree, mixed) < std::tie(o.np, o.by, o.zchunk, o.degree, o.mixed);
@@ -110,14 +140,16 @@ struct Config {
 };
 
 std::string ConfigName(const Config &c) {
-  return (c.mixed ? std::string("mixed") : "np" + std::to_string(c.np)) + "_by" + std::to_string(c.by) + "_z" +
+  return (c.mixed ? std::string(c.mixed == 2 ? "mixedn16" : "mixed") : "np" + std::to_string(c.np)) + "_by" +
+         std::to_string(c.by) + "_z" +
          std::to_string(c.zchunk) + "_t" + std::to_string(c.degree);
 }
 
 // Points per lane of a grid's width class: the fewest that cover its rows
 // (1, 2, 3, 4, 6 or 8).
 int ClassNp(int nx) { return psk::StrideNp(nx); }
-constexpr int kMaxClasses = 6;
+
+
 
 using KernelFn = const void *;
 
@@ -127,7 +159,7 @@ KernelFn StepKernelT(int np, int by) {
 #define PSH_K(NP, BY) \
   if (np == NP && by == BY) return reinterpret_cast<KernelFn>(&BatchStep<P, NP, BY>);
   PSH_K(1, 4) PSH_K(1, 8) PSH_K(1, 16) PSH_K(2, 4) PSH_K(2, 8) PSH_K(2, 16) PSH_K(3, 8) PSH_K(3, 16) PSH_K(4, 4)
-  PSH_K(4, 8) PSH_K(4, 16) PSH_K(6, 8) PSH_K(8, 4) PSH_K(8, 8)
+  PSH_K(4, 8) PSH_K(4, 16) PSH_K(6, 8) PSH_K(8, 4) PSH_K(8, 8) PSH_K(8, 16)
 #undef PSH_K
   return nullptr;
 }
@@ -143,18 +175,41 @@ KernelFn TB2KernelT(int np, int by) {
   return nullptr;
 }
 
-// Kernel for a storage kind (PSHipStorage) and boundary form.
+template <class S, bool kPer>
+KernelFn StarTB2KernelT(int cls) {
+  using P = BatchPlan<S, kPer>;
+  return cls == kStar1 ? reinterpret_cast<KernelFn>(&BatchStarTB2<P, 1>)
+                       : reinterpret_cast<KernelFn>(&BatchStarTB2<P, 2>);
+}
+
+// Kernel for a storage kind (PSHipStorage), boundary form and class (a
+// stride width, or a star class: one-step kernels of star classes are the
+// stride ones of StepNp, fused ones the star tile whatever `by` is).
 KernelFn StepKernel(int storage, bool per, int np, int by) {
+  np = StepNp(np);
   if (storage == PS_HIP_STORAGE_BF16)
     return per ? StepKernelT<psk::BF16, true>(np, by) : StepKernelT<psk::BF16, false>(np, by);
   return per ? StepKernelT<psk::Native<float>, true>(np, by) : StepKernelT<psk::Native<float>, false>(np, by);
 }
 
 KernelFn TB2Kernel(int storage, bool per, int np, int by) {
+  if (IsStar(np)) {
+    if (storage == PS_HIP_STORAGE_BF16)
+      return per ? StarTB2KernelT<psk::BF16, true>(np) : StarTB2KernelT<psk::BF16, false>(np);
+    return per ? StarTB2KernelT<psk::Native<float>, true>(np) : StarTB2KernelT<psk::Native<float>, false>(np);
+  }
   if (storage == PS_HIP_STORAGE_BF16)
     return per ? TB2KernelT<psk::BF16, true>(np, by) : TB2KernelT<psk::BF16, false>(np, by);
   return per ? TB2KernelT<psk::Native<float>, true>(np, by) : TB2KernelT<psk::Native<float>, false>(np, by);
 }
+
+// Rows per workgroup of class `cls` in mixed configuration c.
+int ClassBy(const Config &c, int cls) { return c.mixed == 2 && !IsStar(cls) && cls <= 2 ? 16 : c.by; }
+
+// Launch block of a configuration's fused (degree 2) or one-step kernel.
+dim3 PairBlock(int np, int by) { return IsStar(np) ? dim3(64 * StarNwx(np), kStarBY) : dim3(64, by); }
+// Output rows of a fused tile.
+int PairRows(int np, int by) { return (IsStar(np) ? kStarBY : by) - 2; }
 
 // A device array of T, reallocated on growth.
 template <class T>
@@ -205,6 +260,7 @@ struct Batch {
   // Mixed mode: width classes present (ascending), and per-class streams and
   // fork/join events (created on first use; default priority).
   std::vector<int> classes;
+  std::vector<int> cls;  // class of each grid
   hipStream_t class_stream[kMaxClasses] = {};
   hipEvent_t fork_ev = nullptr, join_ev[kMaxClasses] = {};
 
@@ -258,8 +314,8 @@ struct Batch {
     if (c.mixed) {
       if (c.zchunk <= 0) return false;
       for (int np : classes)
-        if (StepKernel(storage, false, np, c.by) == nullptr ||
-            (c.degree == 2 && TB2Kernel(storage, false, np, c.by) == nullptr))
+        if (StepKernel(storage, false, np, ClassBy(c, np)) == nullptr ||
+            (c.degree == 2 && TB2Kernel(storage, false, np, ClassBy(c, np)) == nullptr))
           return false;
       return true;
     }
@@ -277,9 +333,9 @@ struct Batch {
     for (int i = 0; i < Count(); ++i) order[i] = i;
     std::sort(order.begin(), order.end(), [&](int x, int y) { return a[x]->num_elms > a[y]->num_elms; });
     std::vector<Item> items;
-    const int rows_out = c.degree == 2 ? c.by - 2 : c.by;
+    const int rows_out = c.degree == 2 ? PairRows(c.np, c.by) : c.by;
     for (int g : order) {
-      if (c.mixed && ClassNp(a[g]->dim[0]) != c.np) continue;  // another width class
+      if (c.mixed && cls[g] != c.np) continue;  // another class
       const int ny = a[g]->dim[1], nz = a[g]->dim[2];
       const int zc = c.zchunk <= 0 || c.zchunk > nz ? nz : c.zchunk;
       for (int ty = 0; ty * rows_out < ny; ++ty)
@@ -297,7 +353,8 @@ struct Batch {
     // Persistent grid: as many workgroups as fit on the device at once.
     KernelFn k = c.degree == 2 ? TB2Kernel(storage, false, c.np, c.by) : StepKernel(storage, false, c.np, c.by);
     int per_cu = 1, cus = 1;
-    CheckHip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64 * c.by, 0), "occupancy");
+    const dim3 blk = c.degree == 2 ? PairBlock(c.np, c.by) : dim3(64, c.by);
+    CheckHip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, (int)(blk.x * blk.y), 0), "occupancy");
     hipDeviceProp_t prop;
     CheckHip(hipGetDeviceProperties(&prop, Runtime::Get().device()), "props");
     cus = prop.multiProcessorCount;
@@ -316,8 +373,8 @@ struct Batch {
       return;
     }
     for (int np : classes) {
-      PlanFor(Config{np, c.by, c.zchunk, 1, 1}, s);
-      PlanFor(Config{np, c.by, c.zchunk, c.degree, 1}, s);
+      PlanFor(Config{np, ClassBy(c, np), c.zchunk, 1, 1}, s);
+      PlanFor(Config{np, ClassBy(c, np), c.zchunk, c.degree, 1}, s);
     }
     if (!fork_ev) {
       CheckHip(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), "hipEventCreate");
@@ -337,7 +394,7 @@ struct Batch {
     for (size_t i = 0; i < classes.size() && e == hipSuccess; ++i) {
       hipStream_t cs = class_stream[i];
       e = hipStreamWaitEvent(cs, fork_ev, 0);
-      if (e == hipSuccess) e = EnqueueOne(Config{classes[i], c.by, c.zchunk, c.degree, 1}, steps, cs);
+      if (e == hipSuccess) e = EnqueueOne(Config{classes[i], ClassBy(c, classes[i]), c.zchunk, c.degree, 1}, steps, cs);
       if (e == hipSuccess) e = hipEventRecord(join_ev[i], cs);
       if (e == hipSuccess) e = hipStreamWaitEvent(s, join_ev[i], 0);
     }
@@ -354,7 +411,7 @@ struct Batch {
     Plan &p1 = PlanFor(c1, s);
     Plan &p2 = PlanFor(c, s);
     if (p1.nitems == 0 || steps <= 0) return hipSuccess;
-    dim3 block(64, c.by);
+    const dim3 block(64, c.by), pair_block = PairBlock(c.np, c.by);
     auto step = [&](const BatchArgs *t) {
       KernelFn k = StepKernel(storage, periodic, c.np, c.by);
       const void *tv = t;
@@ -369,7 +426,7 @@ struct Batch {
       const Item *iv = p2.items.p;
       int n = p2.nitems;
       void *args[] = {&ti, &to, &iv, &n};
-      return hipLaunchKernel(k, dim3(p2.blocks), block, args, 0, s);
+      return hipLaunchKernel(k, dim3(p2.blocks), pair_block, args, 0, s);
     };
     hipError_t e = hipSuccess;
     int first = 0;
@@ -419,13 +476,17 @@ struct Batch {
 
 std::vector<Config> Candidates(const Batch &b) {
   std::vector<Config> out;
-  if (b.classes.size() > 1) {
+  if (b.classes.size() > 1 || (b.classes.size() == 1 && IsStar(b.classes[0]))) {
     for (int deg : {1, 2})
       for (int by : {8, 16})
-        for (int zc : {16, 32}) {
+        for (int zc : {16, 32, 64}) {
           Config c{0, by, zc, deg, 1};
           if (b.Supports(c)) out.push_back(c);
         }
+    for (int zc : {32, 64}) {
+      Config c{0, 8, zc, 2, 2};
+      if (b.Supports(c)) out.push_back(c);
+    }
   }
   // One width for all grids: the narrowest that fits the widest row only
   // (wider rows idle lanes).
@@ -489,8 +550,16 @@ int PSHipBatchCreateEx(void **out, __PSGrid *const *a, __PSGrid *const *b, void 
     bt->bytes += Rec(x)->bytes;
   }
   for (int i = 0; i < bt->Count(); ++i) {
-    const int np = ClassNp(bt->a[i]->dim[0]);
-    if (std::find(bt->classes.begin(), bt->classes.end(), np) == bt->classes.end()) bt->classes.push_back(np);
+    // Star classes need whole 16-byte rows in 16-byte aligned buffers, and
+    // pay only for rows of more than 128 points: a star wave spans 256
+    // points, so narrower rows would leave most of its lanes idle.
+    const int nx = bt->a[i]->dim[0];
+    auto aligned = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
+    const bool star = nx % 4 == 0 && nx > 128 && aligned(bt->a[i]->p0) && aligned(bt->b[i]->p0) &&
+                      aligned(bt->aux[i]);
+    const int c = star ? (nx <= 256 ? kStar1 : kStar2) : ClassNp(nx);
+    bt->cls.push_back(c);
+    if (std::find(bt->classes.begin(), bt->classes.end(), c) == bt->classes.end()) bt->classes.push_back(c);
   }
   std::sort(bt->classes.begin(), bt->classes.end());
   bt->tuned = DefaultConfig(*bt);
@@ -528,12 +597,14 @@ int PSHipBatchRun(void *h, int steps, int use_graph) {
 }
 
 // Sets the launch configuration explicitly (np, by, zchunk, degree); np = 0
-// selects the mixed mode (each grid with the fewest points per lane that
-// cover its rows, width classes on their own streams).
+// selects the mixed mode (each grid in its class, classes on their own
+// streams), np = -1 the mixed mode with 16-row tiles for the 1- and 2-point
+// classes.
 int PSHipBatchConfigure(void *h, int np, int by, int zchunk, int degree) {
   if (!h) return PS_HIP_ERR_INVALID;
   Batch *b = static_cast<Batch *>(h);
-  Config c{np, by, zchunk, degree, np == 0 ? 1 : 0};
+  Config c{np, by, zchunk, degree, np == 0 ? 1 : np == -1 ? 2 : 0};
+  if (np < 0) c.np = 0;
   if ((degree != 1 && degree != 2) || !b->Supports(c)) return PS_HIP_ERR_UNSUPPORTED;
   b->tuned = c;
   return PS_HIP_OK;

@@ -151,6 +151,9 @@ const Variant kVariants[] = {
     {"tb2r_w2_by8_z16", 4, 2, 8, 16, true, 1, true},
     {"tb2r_w1_by8_z32", 4, 1, 8, 32, true, 1, true},
     {"tb2r_w1_by16_z32", 4, 1, 16, 32, true, 1, true},
+    {"tb2r_w2_by8_z8", 4, 2, 8, 8, true, 1, true},
+    {"tb2r_w1_by8_z16", 4, 1, 8, 16, true, 1, true},
+    {"tb2r_w1_by8_z8", 4, 1, 8, 8, true, 1, true},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 // Measured on MI355X at 512^3 FP32 (bench.py --sweep): star4_by8_z128 ~710

@@ -26,8 +26,12 @@ def make_batch(rt, shapes, seed, boundary):
     return pairs, fields, coefs, rt.Batch(pairs, coefs, boundary)
 
 
-SHAPES = [(5, 7, 3), (70, 9, 11), (33, 40, 6), (64, 64, 64), (1, 1, 1), (17, 3, 29), (128, 5, 4), (2, 31, 8),
+# Rows of up to 128 points (every configuration below covers them), then
+# whole 16-byte rows longer than 128 points (132, 256, 300, 200), which the
+# mixed mode runs in its star classes (one or two waves per row).
+NARROW = [(5, 7, 3), (70, 9, 11), (33, 40, 6), (64, 64, 64), (1, 1, 1), (17, 3, 29), (128, 5, 4), (2, 31, 8),
           (101, 12, 9), (8, 8, 8), (66, 2, 2), (127, 6, 5)]
+SHAPES = NARROW + [(132, 9, 7), (256, 6, 5), (300, 4, 6), (200, 17, 33)]
 
 
 def check_against_oracle(pairs, fields, coefs, boundary, steps):
@@ -38,13 +42,16 @@ def check_against_oracle(pairs, fields, coefs, boundary, steps):
 
 
 @pytest.mark.parametrize("boundary", [0, 1])
-# np = 0: the mixed mode (each width class with its own kernel and stream).
+# np = 0: the mixed mode (each class with its own kernel and stream); -1: the
+# same with 16-row tiles for the 1- and 2-point classes.
 @pytest.mark.parametrize("cfg", [(2, 4, 16, 1), (2, 8, 64, 1), (4, 8, 32, 1), (2, 8, 16, 2), (2, 16, 32, 2),
-                                 (4, 8, 8, 2), (0, 8, 16, 1), (0, 16, 32, 2), (0, 8, 8, 2)])
+                                 (4, 8, 8, 2), (0, 8, 16, 1), (0, 16, 32, 2), (0, 8, 8, 2), (-1, 8, 32, 2),
+                                 (-1, 8, 16, 1)])
 @pytest.mark.parametrize("graph", [False, True])
 def test_batch_bitwise_vs_oracle(rt, boundary, cfg, graph):
     steps = 9
-    pairs, fields, coefs, batch = make_batch(rt, SHAPES, seed=5, boundary=boundary)
+    shapes = SHAPES if cfg[0] <= 0 else NARROW
+    pairs, fields, coefs, batch = make_batch(rt, shapes, seed=5, boundary=boundary)
     batch.configure(*cfg)
     print("config", batch.config_name(), "graph", graph, flush=True)
     batch.run(steps, graph=graph)
@@ -58,7 +65,7 @@ def test_batch_bitwise_vs_oracle(rt, boundary, cfg, graph):
 @pytest.mark.parametrize("cfg", [(2, 8, 16, 2), (0, 8, 16, 2)])
 def test_graph_replays_with_eager_work_between(rt, cfg):
     steps = 6
-    pairs, fields, coefs, batch = make_batch(rt, SHAPES[:6], seed=9, boundary=0)
+    pairs, fields, coefs, batch = make_batch(rt, NARROW[:6] + (SHAPES[-2:] if cfg[0] == 0 else []), seed=9, boundary=0)
     batch.configure(*cfg)
     for rep in range(1, 4):
         batch.run(steps, graph=True)
