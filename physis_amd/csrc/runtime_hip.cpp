@@ -191,14 +191,22 @@ int GridNewImpl(__PSGrid **out, int elm_size, int num_dims, const int *dim,
   for (int d = 0; d < PS_MAX_DIM; ++d) r->view.dim[d] = g->dim[d];
   g->dev = &r->view;
   rt.AddGridBytes((ptrdiff_t)r->bytes);
-  // Host arrays of this grid hold elm_size-byte elements (FP32 for 16-bit
-  // storage).
-  PrepareHostCopy((size_t)elm_size * (size_t)g->num_elms);
   *out = g;
   return PS_HIP_OK;
 }
 
+// The staging buffers PSGridCopyin/Copyout of grid g will use; host arrays
+// hold elm_size-byte elements (FP32 for 16-bit storage).
+void PrepareCopies(const __PSGrid *g) {
+  if (g && g != INVALID_GRID && g->p0) PrepareHostCopy((size_t)g->elm_size * (size_t)g->num_elms);
+}
+
 }  // namespace
+
+int GridNewDevice(__PSGrid **out, int elm_size, int num_dims, const int *dim, int storage) {
+  return GridNewImpl(out, elm_size, num_dims, dim, storage, nullptr);
+}
+
 }  // namespace physis_hip
 
 using namespace physis_hip;
@@ -241,6 +249,7 @@ __PSGrid *__PSGridNew(int elm_size, int num_dims, PSVectorInt dim,
   int st = GridNewImpl(&g, elm_size, num_dims, dim, PS_HIP_STORAGE_NATIVE, func);
   if (st == PS_HIP_ERR_NOMEM) return INVALID_GRID;
   if (st != PS_HIP_OK) Fatal("__PSGridNew", "invalid grid shape");
+  PrepareCopies(g);
   return g;
 }
 
@@ -253,6 +262,7 @@ __PSGrid *__PSGridNewAttr(int elm_size, int num_dims, PSVectorInt dim,
   int st = GridNewImpl(&g, elm_size, num_dims, dim, storage, nullptr);
   if (st == PS_HIP_ERR_NOMEM) return INVALID_GRID;
   if (st != PS_HIP_OK) Fatal("__PSGridNewAttr", "invalid grid shape or storage");
+  PrepareCopies(g);
   return g;
 }
 
@@ -261,7 +271,9 @@ int PSHipGridNew(__PSGrid **out, int elm_size, int num_dims, const int *dim,
   if (!out || !dim) return PS_HIP_ERR_INVALID;
   if (storage < PS_HIP_STORAGE_NATIVE || storage > PS_HIP_STORAGE_FP16)
     return PS_HIP_ERR_INVALID;
-  return GridNewImpl(out, elm_size, num_dims, dim, storage, nullptr);
+  const int st = GridNewImpl(out, elm_size, num_dims, dim, storage, nullptr);
+  if (st == PS_HIP_OK) PrepareCopies(*out);
+  return st;
 }
 
 int PSHipGridFree(__PSGrid *g) {

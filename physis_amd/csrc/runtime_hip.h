@@ -100,6 +100,9 @@ void ConvertFromStorage(float *dst, const void *src, int64_t n, int storage,
 size_t StoredElmSize(int elm_size, int storage);
 
 // Stream-explicit building blocks used by the solve server (serve.hip).
+// PSHipGridNew without preparing host-copy staging (grids that never leave
+// the device).
+int GridNewDevice(__PSGrid **out, int elm_size, int num_dims, const int *dim, int storage);
 int FillUniformOn(__PSGrid *g, uint64_t seed, float lo, float hi, hipStream_t s);
 int FillPatternOn(__PSGrid *g, int profile, float value, hipStream_t s);
 // FP64 reduction of n elements; result at result_dev[0], partials in

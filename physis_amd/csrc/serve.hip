@@ -279,7 +279,7 @@ int Server::Admit(int idx, double now, hipStream_t s, bool batched) {
   lv.rq = &r;
   lv.admitted = now;
   const int dim[3] = {r.n, r.n, r.n};
-  auto grid = [&](__PSGrid **g) { return PSHipGridNew(g, 4, 3, dim, r.storage); };
+  auto grid = [&](__PSGrid **g) { return GridNewDevice(g, 4, 3, dim, r.storage); };
   int st = grid(&lv.A);
   if (st == PS_HIP_OK) st = grid(&lv.B);
   if (st == PS_HIP_OK && hipMalloc(&lv.aux, (size_t)r.n * r.n * r.n * (r.storage ? 2 : 4)) != hipSuccess) {
