@@ -376,9 +376,11 @@ struct Batch {
       PlanFor(Config{np, ClassBy(c, np), c.zchunk, 1, 1}, s);
       PlanFor(Config{np, ClassBy(c, np), c.zchunk, c.degree, 1}, s);
     }
+    // One stream and join event per class present (every stream a process
+    // keeps takes a share of the device's few hardware queues).
     if (!fork_ev) {
       CheckHip(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), "hipEventCreate");
-      for (int i = 0; i < kMaxClasses; ++i) {
+      for (size_t i = 0; i < classes.size(); ++i) {
         CheckHip(hipStreamCreateWithFlags(&class_stream[i], hipStreamNonBlocking), "hipStreamCreate");
         CheckHip(hipEventCreateWithFlags(&join_ev[i], hipEventDisableTiming), "hipEventCreate");
       }

@@ -3,15 +3,18 @@
 //
 // One hipMemcpy from or to pageable memory is staged by the HIP runtime
 // through pinned memory with a single host thread: ~10 GB/s device to host,
-// ~30 GB/s host to device on MI355X.  Here W host threads each own a stream
-// and two pinned staging buffers and move chunks w, w + W, w + 2W, ...: the
-// DMA of a thread's next chunk overlaps the memcpy of its current one, and
-// the memcpys (and the first-touch page faults of a freshly allocated
-// destination array) run on W cores while the DMA engines serve the other
-// threads.  Every transfer is ordered after the work already queued on the
-// caller's stream (an event the worker streams wait on), and the call
-// returns when all bytes have arrived, as the blocking reference calls do
-// (reference runtime/libphysis_rt_cuda.cc:116-139).
+// ~30 GB/s host to device on MI355X.  Here W host threads each own two pinned
+// staging buffers and move chunks w, w + W, w + 2W, ...: the DMA of a
+// thread's next chunk overlaps the memcpy of its current one, and the
+// memcpys (and the first-touch page faults of a freshly allocated
+// destination array) run on W cores while the link serves the other
+// threads' chunks.  The DMAs share two streams, threads alternating (one
+// stream reached ~28 GB/s device to host, two fill the link, and every
+// stream a process keeps takes a share of the device's few hardware
+// queues); each thread waits on its own events.  Every transfer is ordered
+// after the work already queued on the caller's stream (an event the copy
+// stream waits on), and the call returns when all bytes have arrived, as the
+// blocking reference calls do (reference runtime/libphysis_rt_cuda.cc:116-139).
 #include <algorithm>
 #include <atomic>
 #include <cstring>
@@ -28,25 +31,28 @@ namespace {
 constexpr size_t kChunk = 4u << 20;         // bytes per staged transfer
 constexpr size_t kPipelineMin = 16u << 20;  // smaller copies: one hipMemcpy
 constexpr int kMaxWorkers = 8;
+constexpr int kDmaStreams = 2;
 
 struct Worker {
   void *buf[2] = {};  // pinned, kChunk bytes each
   hipEvent_t ev[2] = {};
-  hipStream_t stream = nullptr;  // default priority
 };
 
 struct Pool {
   std::mutex mu;  // one pipelined copy at a time
   Worker w[kMaxWorkers];
   int n = 0;
+  hipStream_t stream[kDmaStreams] = {};  // DMA streams (default priority)
   hipEvent_t start = nullptr;
 
   // Creates workers up to `want` (kept until ReleaseHostCopyPool).
   void Ensure(int want) {
-    if (!start) CheckHip(hipEventCreateWithFlags(&start, hipEventDisableTiming), "hipEventCreate");
+    if (!start) {
+      CheckHip(hipEventCreateWithFlags(&start, hipEventDisableTiming), "hipEventCreate");
+      for (auto &st : stream) CheckHip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+    }
     for (; n < want; ++n) {
       Worker &k = w[n];
-      CheckHip(hipStreamCreateWithFlags(&k.stream, hipStreamNonBlocking), "hipStreamCreate");
       for (int b = 0; b < 2; ++b) {
         CheckHip(hipHostMalloc(&k.buf[b], kChunk, hipHostMallocDefault), "hipHostMalloc(copy stage)");
         CheckHip(hipEventCreateWithFlags(&k.ev[b], hipEventDisableTiming), "hipEventCreate");
@@ -75,6 +81,8 @@ int Pipeline(void *dev, void *host, size_t bytes, bool to_device, hipStream_t s)
   const int W = WorkersFor(bytes);
   P.Ensure(W);
   if (hipEventRecord(P.start, s) != hipSuccess) return PS_HIP_ERR_DEVICE;
+  for (auto &st : P.stream)
+    if (hipStreamWaitEvent(st, P.start, 0) != hipSuccess) return PS_HIP_ERR_DEVICE;
   const size_t chunks = (bytes + kChunk - 1) / kChunk;
   std::atomic<bool> ok{true};
   const int device = Runtime::Get().device();
@@ -85,7 +93,8 @@ int Pipeline(void *dev, void *host, size_t bytes, bool to_device, hipStream_t s)
     Worker &k = P.w[wi];
     auto len = [&](size_t i) { return std::min(kChunk, bytes - i * kChunk); };
     // A new host thread starts on device 0: select the runtime's device.
-    bool good = hipSetDevice(device) == hipSuccess && hipStreamWaitEvent(k.stream, P.start, 0) == hipSuccess;
+    bool good = hipSetDevice(device) == hipSuccess;
+    hipStream_t cs = P.stream[wi % kDmaStreams];
     if (to_device) {
       bool used[2] = {false, false};
       int b = 0;
@@ -96,14 +105,14 @@ int Pipeline(void *dev, void *host, size_t bytes, bool to_device, hipStream_t s)
           break;
         }
         std::memcpy(k.buf[b], h + i * kChunk, len(i));
-        good = hipMemcpyAsync(d + i * kChunk, k.buf[b], len(i), hipMemcpyHostToDevice, k.stream) == hipSuccess &&
-               hipEventRecord(k.ev[b], k.stream) == hipSuccess;
+        good = hipMemcpyAsync(d + i * kChunk, k.buf[b], len(i), hipMemcpyHostToDevice, cs) == hipSuccess &&
+               hipEventRecord(k.ev[b], cs) == hipSuccess;
         used[b] = true;
       }
     } else {
       auto issue = [&](size_t i, int b) {
-        return hipMemcpyAsync(k.buf[b], d + i * kChunk, len(i), hipMemcpyDeviceToHost, k.stream) == hipSuccess &&
-               hipEventRecord(k.ev[b], k.stream) == hipSuccess;
+        return hipMemcpyAsync(k.buf[b], d + i * kChunk, len(i), hipMemcpyDeviceToHost, cs) == hipSuccess &&
+               hipEventRecord(k.ev[b], cs) == hipSuccess;
       };
       int b = 0;
       if (good && (size_t)wi < chunks) good = issue(wi, 0);
@@ -122,7 +131,8 @@ int Pipeline(void *dev, void *host, size_t bytes, bool to_device, hipStream_t s)
     }
     // Every exit waits for this worker's transfers: no DMA into or out of a
     // staging buffer outlives the call.
-    if (hipStreamSynchronize(k.stream) != hipSuccess) good = false;
+    for (int b = 0; b < 2; ++b)
+      if (hipEventSynchronize(k.ev[b]) != hipSuccess) good = false;
     if (!good) ok.store(false, std::memory_order_relaxed);
   };
 
@@ -166,17 +176,21 @@ int CopyDeviceToHost(void *dst, const void *src, size_t bytes, hipStream_t s) {
 void ReleaseHostCopyPool() {
   Pool &P = ThePool();
   std::lock_guard<std::mutex> lk(P.mu);
+  for (auto &st : P.stream)
+    if (st) (void)hipStreamSynchronize(st);
   for (int i = 0; i < P.n; ++i) {
     Worker &k = P.w[i];
-    (void)hipStreamSynchronize(k.stream);
     for (int b = 0; b < 2; ++b) {
       (void)hipHostFree(k.buf[b]);
       (void)hipEventDestroy(k.ev[b]);
     }
-    (void)hipStreamDestroy(k.stream);
     k = Worker{};
   }
   P.n = 0;
+  for (auto &st : P.stream) {
+    if (st) (void)hipStreamDestroy(st);
+    st = nullptr;
+  }
   if (P.start) (void)hipEventDestroy(P.start);
   P.start = nullptr;
 }
