@@ -98,3 +98,36 @@ def test_batch_rejects_wide_or_mismatched_grids(rt):
     c, d = rt.Grid((8, 4, 4)), rt.Grid((8, 4, 5))
     with pytest.raises(rt.PhysisError):
         rt.Batch([(c, d)], [[0.1] * 6 + [0.4]])
+
+
+@pytest.mark.parametrize("cfg", [(0, 8, 16, 2), (0, 8, 64, 1)])
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_bf16_mixed_batch_equals_single_grid_runs(rt, cfg, boundary):
+    # BF16 storage, every class of the mixed mode (the star classes too):
+    # each grid must end bitwise equal to the same grid advanced alone by
+    # the plain single-grid run (both round to BF16 at every step).
+    steps = 9
+    rng = np.random.default_rng(11)
+    batch_pairs, solo_pairs, coefs = [], [], []
+    for i, shp in enumerate(SHAPES):
+        f = O.uniform_field(shp, seed=700 + i)
+        nb = rng.uniform(0.02, 0.15, 6).astype(np.float32)
+        coefs.append([float(v) for v in nb] + [float(np.float32(1.0) - np.float32(nb.sum()))])
+        for pairs in (batch_pairs, solo_pairs):
+            g1, g2 = rt.Grid(shp, storage=rt.STORAGE_BF16), rt.Grid(shp, storage=rt.STORAGE_BF16)
+            g1.copyin(f)
+            g2.copyin(np.zeros_like(f))
+            pairs.append((g1, g2))
+    batch = rt.Batch(batch_pairs, coefs, boundary)
+    batch.configure(*cfg)
+    print("config", batch.config_name(), flush=True)
+    batch.run(steps, graph=True)
+    for (s1, s2), c in zip(solo_pairs, coefs):
+        rt.diffusion7_run(s1, s2, c, boundary, steps)
+    for (b1, b2), (s1, s2) in zip(batch_pairs, solo_pairs):
+        assert np.array_equal(b1.copyout(), s1.copyout()), "grid %s" % (b1.shape,)
+        assert np.array_equal(b2.copyout(), s2.copyout()), "grid %s" % (b2.shape,)
+    batch.destroy()
+    for g1, g2 in batch_pairs + solo_pairs:
+        g1.free()
+        g2.free()
