@@ -48,8 +48,11 @@ def test_64cubed_100_steps_bitwise_all_variants(rt, boundary):
         assert np.array_equal(got2, want2), "variant %d" % v
 
 
+# (516, 5, 9) and (1024, 3, 4): whole 16-byte rows wider than 512 points,
+# which the default variant fuses with four waves per row.
 @pytest.mark.parametrize("shape", [(37, 23, 19), (65, 5, 3), (1, 1, 1), (2, 3, 2), (130, 9, 70),
-                                   (36, 23, 19), (4, 5, 3), (260, 7, 9), (512, 3, 5), (8, 1, 1)])
+                                   (36, 23, 19), (4, 5, 3), (260, 7, 9), (512, 3, 5), (8, 1, 1),
+                                   (516, 5, 9), (1024, 3, 4)])
 @pytest.mark.parametrize("boundary", [0, 1])
 def test_ragged_shapes_bitwise(rt, shape, boundary):
     field = O.uniform_field(shape, seed=11)
