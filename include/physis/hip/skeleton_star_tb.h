@@ -300,9 +300,9 @@ inline bool StarTB2Supported(int nx, int ny, int nz) {
 // z-chunk of 32, 16 and 8 planes that still gives kTB2WavesFull waves, else
 // the largest that gives kTB2Waves, else 8 (a chunk re-reads two halo
 // planes, so longer chunks move fewer bytes once the device is full).
-// Timed on MI355X (GLUPS, clamped): 128^3 258, 192^3 629, 256^3 905, 384^3
-// 966, 512^3 1186-1196, each within 1% of the best of 14 shapes (a fixed
-// 2 x 8 x 32-plane shape: 586 at 256^3, 946 at 384^3).
+// Timed on MI355X with the rotating-queue loop (GLUPS, clamped): 128^3 298,
+// 192^3 673, 256^3 1011, 320^3 837, 384^3 1033, 448^3 1104, 512^3 1255-1314,
+// each within run-to-run noise (2-4%) of the best 8-, 16- or 32-plane chunk.
 struct TB2Shape {
   int nwx, by, zchunk;
 };
