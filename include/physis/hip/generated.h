@@ -14,6 +14,7 @@
 
 #include "physis/physis_hip.h"
 #include "physis/physis_hip_ext.h"
+#include "physis/runtime.h"
 #include "physis/hip/skeleton.h"
 #include "physis/hip/skeleton_star.h"
 #include "physis/hip/skeleton_star_tb.h"
@@ -118,23 +119,32 @@ inline bool FusedPairSupported(int nx, int ny, int nz) {
          (nx >= 1 && nx <= 512 && ny >= 1 && nz >= 1);
 }
 
-// Two fused steps src -> dst for a star plan; false if the grid does not fit
-// a supported row width.
+// Two fused steps src -> dst for a star plan.  mid: also store the
+// intermediate step through out_ptr(a1).  Returns the tile that ran: 1 the
+// vectorised tile (rows of whole 16-byte vectors), 2 the stride tile, 0 none
+// (the row width is not supported).
 template <class P>
-inline bool LaunchFusedPair(const typename P::Args &a1, const typename P::Args &a2, int nx, int ny, int nz,
-                            hipStream_t s) {
-  hipError_t e = LaunchZMarchStarTB2Auto<P>(a1, a2, nx, ny, nz, s);
-  if (e == hipErrorNotSupported) e = LaunchStrideTB2Grid<P>(a1, a2, nx, ny, nz, s);
-  if (e == hipErrorNotSupported) return false;
+inline int LaunchFusedPair(const typename P::Args &a1, const typename P::Args &a2, int nx, int ny, int nz,
+                           hipStream_t s, bool mid) {
+  hipError_t e = LaunchZMarchStarTB2Auto<P>(a1, a2, nx, ny, nz, s, mid);
+  int tile = 1;
+  if (e == hipErrorNotSupported) {
+    e = LaunchStrideTB2Grid<P>(a1, a2, nx, ny, nz, s, mid);
+    tile = 2;
+  }
+  if (e == hipErrorNotSupported) return 0;
   CheckLaunch(e, "fused pair");
-  return true;
+  return tile;
 }
 
 // PSStencilRun(map(k, A -> B), map(k, B -> A), iter) for a star plan whose
-// two maps differ only in the swapped grids, over the whole grid: iter-1
-// fused pairs ping-pong between A and a third buffer, then the last pair runs
-// unfused (src -> B, B -> A) so both grids end exactly as with 2*iter
-// separate steps.  ab: args of map 1 (A -> B); ba: args of map 2 (B -> A).
+// two maps differ only in the swapped grids, over the whole grid.  With
+// iter >= 2 every pair of steps is one fused pass: the passes run along the
+// buffer path of FusedPathDst (A, a third buffer and B), and the last pass
+// writes u_{2 iter} to A and its intermediate u_{2 iter - 1} to B, so both
+// grids end exactly as with 2*iter separate steps.  iter == 1 runs the two
+// steps unfused (a single fused pass would have to write A while reading
+// it).  ab: args of map 1 (A -> B); ba: args of map 2 (B -> A).
 template <class P>
 inline void RunPingPong(typename P::Args ab, typename P::Args ba, int iter, int nx, int ny, int nz,
                         size_t grid_bytes, hipStream_t s) {
@@ -147,31 +157,34 @@ inline void RunPingPong(typename P::Args ab, typename P::Args ba, int iter, int 
   }
   st *A = P::out_ptr(ba);
   st *B = P::out_ptr(ab);
-  const st *src = A;
-  int fused = iter - 1;
-  if (fused > 0 && FusedPairSupported(nx, ny, nz)) {
+  if (iter >= 2 && FusedPairSupported(nx, ny, nz)) {
     st *aux = static_cast<st *>(__PSHipAuxBuffer(grid_bytes));
     if (!aux) {
       fprintf(stderr, "physis-hip: no memory for the fused-step buffer\n");
       PSAbort(1);
     }
-    for (int i = 0; i < fused; ++i) {
-      st *dst = (i & 1) ? A : aux;
+    const st *src = A;
+    int tile = 0;
+    for (int j = 0; j < iter; ++j) {
+      st *dst = FusedPathDst(j, iter, A, B, aux);
       typename P::Args a1 = ab, a2 = ba;
       P::set_in(a1, src);
       P::set_out(a2, dst);
-      LaunchFusedPair<P>(a1, a2, nx, ny, nz, s);
+      tile = LaunchFusedPair<P>(a1, a2, nx, ny, nz, s, j == iter - 1);
+      if (!tile) {
+        fprintf(stderr, "physis-hip: fused pair does not fit the grid\n");
+        PSAbort(1);
+      }
       src = dst;
     }
-  } else {
-    fused = 0;
+    if (__ps_trace)
+      fprintf(__ps_trace, "Physis: fused pairs: %d passes on the %s tile\n", iter,
+              tile == 1 ? "vectorised" : "stride");
+    return;
   }
-  for (int i = fused; i < iter; ++i) {
-    typename P::Args a1 = ab;
-    P::set_in(a1, src);
-    CheckLaunch(LaunchStarStep<P>(a1, box, nx, ny, nz, s), "step");
+  for (int i = 0; i < iter; ++i) {
+    CheckLaunch(LaunchStarStep<P>(ab, box, nx, ny, nz, s), "step");
     CheckLaunch(LaunchStarStep<P>(ba, box, nx, ny, nz, s), "step");
-    src = A;
   }
 }
 

@@ -61,6 +61,11 @@ struct StarTB2Lds {
 // (nx <= 256*NWX); called by the single-grid kernel below and by batched
 // launches over many grids.  Block: 64*NWX x BY threads.
 //
+// kMid: the tile also stores u1 on its output rows and planes through
+// out_ptr(a1) (the first map's output grid), so a run's last fused pass
+// leaves both grids of the ping-pong as two separate steps leave them (one
+// more write per point instead of an unfused pair of steps).
+//
 // kRenamed selects the loop schedule.  false: the next u0 plane is loaded
 // before the barrier into a fourth register set and the queues are shifted
 // by register moves at the end of each step.  true: the queues hold three
@@ -68,13 +73,14 @@ struct StarTB2Lds {
 // (no moves); the next u0 plane is loaded, after u1 is computed, into the
 // set of the plane u1 no longer needs, and halo rows load straight into
 // their own registers once published.
-template <class Plan, int NWX, int BY, bool kRenamed>
+template <class Plan, int NWX, int BY, bool kRenamed, bool kMid = false>
 __device__ __forceinline__ void StarTB2Tile(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx,
                                             int ny, int nz, int ya, int zb, int ze, StarTB2Lds<NWX, BY> &L) {
   using S = typename Plan::Storage;
   using st = typename S::stored_t;
   constexpr bool W = Plan::kWrap;
   static_assert(BY >= 3, "need at least one output row");
+  static_assert(!kMid || kRenamed, "the intermediate store is part of the rotating-queue schedule");
   auto &L0 = L.u0;
   auto &L1 = L.u1;
 
@@ -209,6 +215,12 @@ __device__ __forceinline__ void StarTB2Tile(const typename Plan::Args &a1, const
       // Lanes past the row end compute on in-bounds LDS garbage that is
       // never published or stored.
       level1(buf, z0, Q0[i0], Q0[i1], Q0[i2], Q1[i2]);
+      if constexpr (kMid) {
+        // u1 of plane k on an output row: rounded to storage already, so
+        // the stored bits are those a separate step stores.
+        if (has && out_row && k >= zb && k < ze)
+          Store4<S>(Plan::out_ptr(a1) + (int64_t)k * plane + (int64_t)y * nx + xs, Q1[i2]);
+      }
       // Q0[i0] (plane k-1) is dead: it receives plane k+2.
       ldu((int64_t)z2 * plane + lrow, Q0[i0]);
       if (out_row && k - 1 >= zb) level2(buf, k, Q1[i0], Q1[i1], Q1[i2]);
@@ -276,7 +288,7 @@ __device__ __forceinline__ void StarTB2Tile(const typename Plan::Args &a1, const
   }
 }
 
-template <class Plan, int NWX, int BY, bool kRenamed = false>
+template <class Plan, int NWX, int BY, bool kRenamed = false, bool kMid = false>
 __global__ void __launch_bounds__(64 * NWX * BY, kRenamed ? 8 : 1)
 ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
   __shared__ StarTB2Lds<NWX, BY> L;
@@ -285,7 +297,7 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
   Plan::stage_dims(a1, nx, ny, nz);
   const int zb = (int)blockIdx.y * geo.zchunk;
   const int ze = min(zb + geo.zchunk, nz);
-  StarTB2Tile<Plan, NWX, BY, kRenamed>(a1, a2, nx, ny, nz, tile * (BY - 2) - 1, zb, ze, L);
+  StarTB2Tile<Plan, NWX, BY, kRenamed, kMid>(a1, a2, nx, ny, nz, tile * (BY - 2) - 1, zb, ze, L);
 }
 
 // Whether a grid can take the fused two-step path with NWX waves per row.
@@ -326,7 +338,7 @@ inline TB2Shape ChooseTB2Shape(int nx, int ny, int nz) {
   return sh;
 }
 
-template <class Plan, int NWX, int BY, bool kRenamed = false>
+template <class Plan, int NWX, int BY, bool kRenamed = false, bool kMid = false>
 inline hipError_t LaunchZMarchStarTB2(const typename Plan::Args &a1, const typename Plan::Args &a2, int ny,
                                       int nz, int zchunk, bool remap, hipStream_t stream) {
   TBGeometry geo;
@@ -335,24 +347,49 @@ inline hipError_t LaunchZMarchStarTB2(const typename Plan::Args &a1, const typen
   geo.zchunk = zchunk;
   geo.remap = remap ? 1 : 0;
   dim3 grid(geo.tiles_y, (nz + zchunk - 1) / zchunk, 1);
-  hipLaunchKernelGGL((ZMarchStar4TB2<Plan, NWX, BY, kRenamed>), grid, dim3(64 * NWX, BY, 1), 0, stream, a1, a2,
-                     geo);
+  hipLaunchKernelGGL((ZMarchStar4TB2<Plan, NWX, BY, kRenamed, kMid>), grid, dim3(64 * NWX, BY, 1), 0, stream, a1,
+                     a2, geo);
   return hipGetLastError();
 }
 
 // Two fused steps with the shape ChooseTB2Shape picks; hipErrorNotSupported
 // when the rows are not whole 16-byte vectors or wider than 1024 points.
+// mid: also store the intermediate level through out_ptr(a1) (kMid above).
 template <class Plan>
 inline hipError_t LaunchZMarchStarTB2Auto(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx,
-                                          int ny, int nz, hipStream_t stream) {
+                                          int ny, int nz, hipStream_t stream, bool mid = false) {
   const TB2Shape sh = ChooseTB2Shape(nx, ny, nz);
-  if (sh.nwx == 1 && StarTB2Supported<1>(nx, ny, nz))
-    return LaunchZMarchStarTB2<Plan, 1, 8, true>(a1, a2, ny, nz, sh.zchunk, true, stream);
-  if (sh.nwx == 2 && StarTB2Supported<2>(nx, ny, nz))
-    return LaunchZMarchStarTB2<Plan, 2, 8, true>(a1, a2, ny, nz, sh.zchunk, true, stream);
-  if (sh.nwx == 4 && StarTB2Supported<4>(nx, ny, nz))
-    return LaunchZMarchStarTB2<Plan, 4, 4, true>(a1, a2, ny, nz, sh.zchunk, true, stream);
+#define PSK_TB2_AUTO(NWX, BY)                                                                          \
+  if (sh.nwx == NWX && StarTB2Supported<NWX>(nx, ny, nz))                                               \
+    return mid ? LaunchZMarchStarTB2<Plan, NWX, BY, true, true>(a1, a2, ny, nz, sh.zchunk, true, stream) \
+               : LaunchZMarchStarTB2<Plan, NWX, BY, true, false>(a1, a2, ny, nz, sh.zchunk, true, stream);
+  PSK_TB2_AUTO(1, 8)
+  PSK_TB2_AUTO(2, 8)
+  PSK_TB2_AUTO(4, 4)
+#undef PSK_TB2_AUTO
   return hipErrorNotSupported;
+}
+
+// Buffer path of a run of `pairs` fused passes (pairs >= 2) that starts from
+// and ends in buffer A, the last pass reading a third buffer X (so that it may
+// also store its intermediate level into B): pass j reads path[j] and writes
+// path[j + 1], where path[0] = path[pairs] = A and path[pairs - 1] = X; the
+// passes before alternate between A and X, with B standing in for A as the
+// first destination when that would make a pass write its own input.  No
+// pass writes the buffer it reads (workgroups march at different paces), and
+// the last pass reads neither A nor B.  FusedPathBuffer returns path[i] as
+// 0 (A), 1 (B) or 2 (X).
+__host__ __device__ inline int FusedPathBuffer(int i, int pairs) {
+  if (i == 0 || i == pairs) return 0;
+  if (((pairs - 1 - i) & 1) == 0) return 2;
+  return i == 1 ? 1 : 0;
+}
+
+// The buffer pass j of such a run writes.
+template <class T>
+__host__ __device__ inline T *FusedPathDst(int j, int pairs, T *A, T *B, T *X) {
+  const int b = FusedPathBuffer(j + 1, pairs);
+  return b == 0 ? A : (b == 1 ? B : X);
 }
 
 }  // namespace psk

@@ -146,8 +146,9 @@ __device__ __forceinline__ void StrideStep(const typename Plan::Args &a, int nx,
 
 // Two fused maps: u1 = body(u0 from stage_ptr(a1)) on rows [y0-1+1 ...]:
 // the tile computes u1 on rows [ya, ya+BY) and writes u2 = body(u1) through
-// out_ptr(a2) on rows [ya+1, ya+BY-1), planes [zb, ze).
-template <class Plan, int NP, int BY>
+// out_ptr(a2) on rows [ya+1, ya+BY-1), planes [zb, ze).  kMid: u1 of those
+// rows and planes is also stored through out_ptr(a1).
+template <class Plan, int NP, int BY, bool kMid = false>
 __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx,
                                           int ny, int nz, int ya, int zb, int ze, StrideTB2Lds<NP, BY> &L) {
   using S = typename Plan::Storage;
@@ -243,6 +244,14 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
     if (halo) ld(in + (int64_t)z1 * plane + hrow, hv);
     __syncthreads();
     level1(R0, z0, Q0[i0], Q0[i1], Q0[i2], Q1[i2]);
+    if constexpr (kMid) {
+      if (out_row && k >= zb && k < ze) {
+        st *m = Plan::out_ptr(a1) + (int64_t)k * plane + (int64_t)y * nx;
+#pragma unroll
+        for (int q = 0; q < NP; ++q)
+          if (lane + 64 * q < nx) S::store(m + lane + 64 * q, Q1[i2][q]);
+      }
+    }
     ld(in + (int64_t)z2 * plane + row, Q0[i0]);
     if (out_row && k - 1 >= zb) level2(R1, k, Q1[i0], Q1[i1], Q1[i2]);
     z0 = z1;
@@ -311,31 +320,35 @@ inline hipError_t LaunchStrideStepGrid(const typename Plan::Args &a, int nx, int
 
 // Two fused maps over one whole grid: workgroup (blockIdx.x, blockIdx.y) =
 // (y tile of BY - 2 output rows, z chunk); tiles are renumbered per XCD.
-template <class Plan, int NP, int BY>
+template <class Plan, int NP, int BY, bool kMid = false>
 __global__ void __launch_bounds__(64 * BY)
 StrideTB2Grid(typename Plan::Args a1, typename Plan::Args a2, int nx, int ny, int nz, int zchunk) {
   __shared__ StrideTB2Lds<NP, BY> L;
   const int ty = XcdTile(blockIdx.x, gridDim.x);
   const int zb = (int)blockIdx.y * zchunk;
-  StrideTB2<Plan, NP, BY>(a1, a2, nx, ny, nz, ty * (BY - 2) - 1, zb, min(zb + zchunk, nz), L);
+  StrideTB2<Plan, NP, BY, kMid>(a1, a2, nx, ny, nz, ty * (BY - 2) - 1, zb, min(zb + zchunk, nz), L);
 }
 
 // Host launch of StrideTB2Grid for rows of any length up to 512 points:
 // NP = StrideNp(nx), 8 rows per workgroup, and the z-chunk of StrideZChunk
 // (the rule of ChooseTB2Shape).
-// hipErrorNotSupported for wider rows.
+// hipErrorNotSupported for wider rows.  mid: also store the intermediate
+// level through out_ptr(a1) (kMid above).
 template <class Plan>
 inline hipError_t LaunchStrideTB2Grid(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx, int ny,
-                                      int nz, hipStream_t stream) {
+                                      int nz, hipStream_t stream, bool mid = false) {
   constexpr int BY = 8;
   if (nx < 1 || nx > 512 || ny < 1 || nz < 1) return hipErrorNotSupported;
   const int tiles_y = (ny + BY - 3) / (BY - 2);
   const int zchunk = StrideZChunk(tiles_y, nz, BY);
   const dim3 grid(tiles_y, (nz + zchunk - 1) / zchunk, 1), block(64, BY, 1);
   switch (StrideNp(nx)) {
-#define PSK_TB2(NP)                                                                                             \
-  case NP:                                                                                                      \
-    hipLaunchKernelGGL((StrideTB2Grid<Plan, NP, BY>), grid, block, 0, stream, a1, a2, nx, ny, nz, zchunk); \
+#define PSK_TB2(NP)                                                                                                 \
+  case NP:                                                                                                          \
+    if (mid)                                                                                                        \
+      hipLaunchKernelGGL((StrideTB2Grid<Plan, NP, BY, true>), grid, block, 0, stream, a1, a2, nx, ny, nz, zchunk);  \
+    else                                                                                                            \
+      hipLaunchKernelGGL((StrideTB2Grid<Plan, NP, BY, false>), grid, block, 0, stream, a1, a2, nx, ny, nz, zchunk); \
     break;
     PSK_TB2(1) PSK_TB2(2) PSK_TB2(3) PSK_TB2(4) PSK_TB2(6) PSK_TB2(8)
 #undef PSK_TB2

@@ -6,7 +6,8 @@
 // (grid, row tile, z chunk) items sorted largest grid first.  With fused
 // steps on (temporal degree 2) each launch advances every grid two steps
 // through a per-grid third buffer, as the single-grid path does, and the last
-// pair runs unfused so every B_i ends as the reference leaves it.  All the
+// fused pass also stores its intermediate step, so every B_i ends as the
+// reference leaves it.  All the
 // launches of a run are captured once into a hipGraph and replayed as one
 // graph launch.
 //
@@ -82,7 +83,9 @@ BatchStep(const typename P::Args *args, const Item *items, int nitems) {
   }
 }
 
-template <class P, int NP, int BY>
+// kMid: the input table's `out` receives the intermediate step (the last
+// fused pass of a run).
+template <class P, int NP, int BY, bool kMid>
 __global__ void __launch_bounds__(64 * BY)
 BatchTB2(const typename P::Args *in_args, const typename P::Args *out_args, const Item *items, int nitems) {
   __shared__ psk::StrideTB2Lds<NP, BY> L;
@@ -90,7 +93,7 @@ BatchTB2(const typename P::Args *in_args, const typename P::Args *out_args, cons
     const Item it = items[w];
     const typename P::Args a1 = in_args[it.grid];
     const typename P::Args a2 = out_args[it.grid];
-    psk::StrideTB2<P, NP, BY>(a1, a2, a1.dim[0], a1.dim[1], a1.dim[2], it.y0, it.zb, it.ze, L);
+    psk::StrideTB2<P, NP, BY, kMid>(a1, a2, a1.dim[0], a1.dim[1], a1.dim[2], it.y0, it.zb, it.ze, L);
     __syncthreads();
   }
 }
@@ -108,7 +111,7 @@ inline int StarNwx(int cls) { return cls == kStar1 ? 1 : 2; }
 // Stride width of a class's one-step kernel.
 inline int StepNp(int cls) { return IsStar(cls) ? (cls == kStar1 ? 4 : 8) : cls; }
 
-template <class P, int NWX>
+template <class P, int NWX, bool kMid>
 __global__ void __launch_bounds__(64 * NWX * kStarBY, 8)
 BatchStarTB2(const typename P::Args *in_args, const typename P::Args *out_args, const Item *items, int nitems) {
   __shared__ psk::StarTB2Lds<NWX, kStarBY> L;
@@ -116,7 +119,8 @@ BatchStarTB2(const typename P::Args *in_args, const typename P::Args *out_args, 
     const Item it = items[w];
     const typename P::Args a1 = in_args[it.grid];
     const typename P::Args a2 = out_args[it.grid];
-    psk::StarTB2Tile<P, NWX, kStarBY, true>(a1, a2, a1.dim[0], a1.dim[1], a1.dim[2], it.y0, it.zb, it.ze, L);
+    psk::StarTB2Tile<P, NWX, kStarBY, true, kMid>(a1, a2, a1.dim[0], a1.dim[1], a1.dim[2], it.y0, it.zb, it.ze,
+                                                  L);
     __syncthreads();
   }
 }
@@ -164,22 +168,28 @@ KernelFn StepKernelT(int np, int by) {
   return nullptr;
 }
 
-template <class S, bool kPer>
+template <class S, bool kPer, bool kMid>
 KernelFn TB2KernelT(int np, int by) {
   using P = BatchPlan<S, kPer>;
 #define PSH_K(NP, BY) \
-  if (np == NP && by == BY) return reinterpret_cast<KernelFn>(&BatchTB2<P, NP, BY>);
+  if (np == NP && by == BY) return reinterpret_cast<KernelFn>(&BatchTB2<P, NP, BY, kMid>);
   PSH_K(1, 8) PSH_K(2, 8) PSH_K(3, 8) PSH_K(4, 8) PSH_K(6, 8) PSH_K(8, 8) PSH_K(1, 16) PSH_K(2, 16) PSH_K(3, 16)
   PSH_K(4, 16)
 #undef PSH_K
   return nullptr;
 }
 
-template <class S, bool kPer>
+template <class S, bool kPer, bool kMid>
 KernelFn StarTB2KernelT(int cls) {
   using P = BatchPlan<S, kPer>;
-  return cls == kStar1 ? reinterpret_cast<KernelFn>(&BatchStarTB2<P, 1>)
-                       : reinterpret_cast<KernelFn>(&BatchStarTB2<P, 2>);
+  return cls == kStar1 ? reinterpret_cast<KernelFn>(&BatchStarTB2<P, 1, kMid>)
+                       : reinterpret_cast<KernelFn>(&BatchStarTB2<P, 2, kMid>);
+}
+
+template <class S, bool kPer>
+KernelFn TB2KernelFor(int np, int by, bool mid) {
+  if (IsStar(np)) return mid ? StarTB2KernelT<S, kPer, true>(np) : StarTB2KernelT<S, kPer, false>(np);
+  return mid ? TB2KernelT<S, kPer, true>(np, by) : TB2KernelT<S, kPer, false>(np, by);
 }
 
 // Kernel for a storage kind (PSHipStorage), boundary form and class (a
@@ -192,15 +202,12 @@ KernelFn StepKernel(int storage, bool per, int np, int by) {
   return per ? StepKernelT<psk::Native<float>, true>(np, by) : StepKernelT<psk::Native<float>, false>(np, by);
 }
 
-KernelFn TB2Kernel(int storage, bool per, int np, int by) {
-  if (IsStar(np)) {
-    if (storage == PS_HIP_STORAGE_BF16)
-      return per ? StarTB2KernelT<psk::BF16, true>(np) : StarTB2KernelT<psk::BF16, false>(np);
-    return per ? StarTB2KernelT<psk::Native<float>, true>(np) : StarTB2KernelT<psk::Native<float>, false>(np);
-  }
+// mid: the kernel that also stores the intermediate step (last fused pass).
+KernelFn TB2Kernel(int storage, bool per, int np, int by, bool mid = false) {
   if (storage == PS_HIP_STORAGE_BF16)
-    return per ? TB2KernelT<psk::BF16, true>(np, by) : TB2KernelT<psk::BF16, false>(np, by);
-  return per ? TB2KernelT<psk::Native<float>, true>(np, by) : TB2KernelT<psk::Native<float>, false>(np, by);
+    return per ? TB2KernelFor<psk::BF16, true>(np, by, mid) : TB2KernelFor<psk::BF16, false>(np, by, mid);
+  return per ? TB2KernelFor<psk::Native<float>, true>(np, by, mid)
+             : TB2KernelFor<psk::Native<float>, false>(np, by, mid);
 }
 
 // Rows per workgroup of class `cls` in mixed configuration c.
@@ -250,7 +257,10 @@ struct Batch {
     int blocks = 0;
   };
   std::map<Config, Plan> plans;
-  DevArray<BatchArgs> t_ab, t_ba, t_a_aux_in, t_aux_a_out, t_a_aux_out, t_aux_in, t_aux_b;
+  // Single steps: t_ab (A -> B), t_ba (B -> A).  Fused passes: input tables
+  // t_in[b] and output tables t_out[b] for the buffers b = A, B, X of
+  // psk::FusedPathBuffer, and t_in_mid (input X, intermediate into B).
+  DevArray<BatchArgs> t_ab, t_ba, t_in[3], t_out[3], t_in_mid;
   bool tables_ready = false;
   struct Graph {
     hipGraphExec_t exec = nullptr;
@@ -284,27 +294,28 @@ struct Batch {
   void BuildTables(hipStream_t s) {
     if (tables_ready) return;
     using Args = BatchArgs;
-    std::vector<Args> ab, ba, a_in, aux_out_a, aux_out, aux_in, aux_b;
+    std::vector<Args> ab, ba, in[3], out[3], in_mid;
     for (int i = 0; i < Count(); ++i) {
       Args x;
       for (int d = 0; d < 3; ++d) x.dim[d] = a[i]->dim[d];
       x.k = coef[i];
       void *A = a[i]->p0, *B = b[i]->p0, *X = aux[i];
-      x.in = A; x.out = B; ab.push_back(x);          // map 1: A -> B
-      x.in = B; x.out = A; ba.push_back(x);          // map 2: B -> A
-      x.in = A; x.out = nullptr; a_in.push_back(x);  // fused input A
-      x.in = nullptr; x.out = A; aux_out_a.push_back(x);  // fused output A
-      x.in = nullptr; x.out = X; aux_out.push_back(x);    // fused output aux
-      x.in = X; x.out = nullptr; aux_in.push_back(x);     // fused input aux
-      x.in = X; x.out = B; aux_b.push_back(x);            // last pair from aux: aux -> B
+      void *buf[3] = {A, B, X};
+      x.in = A; x.out = B; ab.push_back(x);  // map 1: A -> B
+      x.in = B; x.out = A; ba.push_back(x);  // map 2: B -> A
+      for (int j = 0; j < 3; ++j) {
+        x.in = buf[j]; x.out = nullptr; in[j].push_back(x);
+        x.in = nullptr; x.out = buf[j]; out[j].push_back(x);
+      }
+      x.in = X; x.out = B; in_mid.push_back(x);
     }
     t_ab.Assign(ab, s);
     t_ba.Assign(ba, s);
-    t_a_aux_in.Assign(a_in, s);
-    t_aux_a_out.Assign(aux_out_a, s);
-    t_a_aux_out.Assign(aux_out, s);
-    t_aux_in.Assign(aux_in, s);
-    t_aux_b.Assign(aux_b, s);
+    for (int j = 0; j < 3; ++j) {
+      t_in[j].Assign(in[j], s);
+      t_out[j].Assign(out[j], s);
+    }
+    t_in_mid.Assign(in_mid, s);
     tables_ready = true;
   }
 
@@ -422,8 +433,8 @@ struct Batch {
       void *args[] = {&tv, &iv, &n};
       return hipLaunchKernel(k, dim3(p1.blocks), block, args, 0, s);
     };
-    auto pair = [&](const BatchArgs *tin, const BatchArgs *tout) {
-      KernelFn k = TB2Kernel(storage, periodic, c.np, c.by);
+    auto pair = [&](const BatchArgs *tin, const BatchArgs *tout, bool mid) {
+      KernelFn k = TB2Kernel(storage, periodic, c.np, c.by, mid);
       const void *ti = tin, *to = tout;
       const Item *iv = p2.items.p;
       int n = p2.nitems;
@@ -431,22 +442,21 @@ struct Batch {
       return hipLaunchKernel(k, dim3(p2.blocks), pair_block, args, 0, s);
     };
     hipError_t e = hipSuccess;
-    int first = 0;
-    bool from_aux = false;
-    if (c.degree == 2 && steps >= 4) {
-      const int fused = steps / 2 - 1;
-      for (int i = 0; i < fused && e == hipSuccess; ++i) {
-        // A -> aux, aux -> A, ...
-        e = (i & 1) ? pair(t_aux_in.p, t_aux_a_out.p) : pair(t_a_aux_in.p, t_a_aux_out.p);
+    const int pairs = steps / 2;
+    if (c.degree == 2 && pairs >= 2) {
+      // Every pair fused along psk::FusedPathBuffer's path from A back to A.
+      // An even count: the last pass also stores u_(steps-1) into B.  An odd
+      // count: the pairs leave u_(steps-1) in A, and map 1 (A -> B) follows.
+      const bool odd = steps & 1;
+      for (int j = 0; j < pairs && e == hipSuccess; ++j) {
+        const int src = psk::FusedPathBuffer(j, pairs), dst = psk::FusedPathBuffer(j + 1, pairs);
+        const bool mid = j == pairs - 1 && !odd;
+        e = pair(mid ? t_in_mid.p : t_in[src].p, t_out[dst].p, mid);
       }
-      from_aux = fused & 1;
-      first = 2 * fused;
+      if (odd && e == hipSuccess) e = step(t_ab.p);
+      return e;
     }
-    for (int i = first; i < steps && e == hipSuccess; ++i) {
-      const bool even = ((i - first) & 1) == 0;
-      if (even) e = step((i == first && from_aux) ? t_aux_b.p : t_ab.p);
-      else e = step(t_ba.p);
-    }
+    for (int i = 0; i < steps && e == hipSuccess; ++i) e = step((i & 1) ? t_ba.p : t_ab.p);
     return e;
   }
 

@@ -163,23 +163,27 @@ constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 constexpr int kStarVariant = 14;
 
 // Launches one fused pair of steps src -> (on chip) -> dst with variant v
-// (family 4).  Returns hipErrorNotSupported if the grid does not qualify.
+// (family 4); a non-null mid also receives the intermediate step (rotating-
+// queue variants only).  Returns hipErrorNotSupported if the grid does not
+// qualify.
 template <class S, bool kPeriodic>
-hipError_t LaunchPair(const Variant &v, const void *src, void *dst, const int *dim, const Coef7 &k,
+hipError_t LaunchPair(const Variant &v, const void *src, void *dst, void *mid, const int *dim, const Coef7 &k,
                       hipStream_t s) {
   using P = DiffusionPlan<S, kPeriodic>;
   using st = typename S::stored_t;
   typename P::Args a1, a2;
   a1.g1 = static_cast<const st *>(src);
-  a1.g2 = nullptr;
+  a1.g2 = static_cast<st *>(mid);
   a2.g1 = nullptr;
   a2.g2 = static_cast<st *>(dst);
   for (int d = 0; d < 3; ++d) a1.dim[d] = a2.dim[d] = dim[d];
   a1.k = a2.k = k;
-#define PSH_TBR(NWX, BY)                                                                          \
-  if (v.renamed && v.bx == NWX && v.by == BY) {                                                   \
-    if (!psk::StarTB2Supported<NWX>(dim[0], dim[1], dim[2])) return hipErrorNotSupported;       \
-    return psk::LaunchZMarchStarTB2<P, NWX, BY, true>(a1, a2, dim[1], dim[2], v.zchunk, v.remap, s); \
+  if (mid && !v.renamed) return hipErrorInvalidValue;
+#define PSH_TBR(NWX, BY)                                                                                   \
+  if (v.renamed && v.bx == NWX && v.by == BY) {                                                            \
+    if (!psk::StarTB2Supported<NWX>(dim[0], dim[1], dim[2])) return hipErrorNotSupported;                \
+    return mid ? psk::LaunchZMarchStarTB2<P, NWX, BY, true, true>(a1, a2, dim[1], dim[2], v.zchunk, v.remap, s) \
+               : psk::LaunchZMarchStarTB2<P, NWX, BY, true, false>(a1, a2, dim[1], dim[2], v.zchunk, v.remap, s); \
   }
   PSH_TBR(2, 8)
   PSH_TBR(1, 8)
@@ -205,47 +209,47 @@ hipError_t LaunchPair(const Variant &v, const void *src, void *dst, const int *d
 // A fused pair on the stride skeleton (rows of any length up to 512 points),
 // for whole-grid runs whose rows are not whole 16-byte vectors.
 template <class S, bool kPeriodic>
-hipError_t LaunchStridePair(const void *src, void *dst, const int *dim, const Coef7 &k, hipStream_t s) {
+hipError_t LaunchStridePair(const void *src, void *dst, void *mid, const int *dim, const Coef7 &k, hipStream_t s) {
   using P = DiffusionPlan<S, kPeriodic>;
   using st = typename S::stored_t;
   typename P::Args a1, a2;
   a1.g1 = static_cast<const st *>(src);
-  a1.g2 = nullptr;
+  a1.g2 = static_cast<st *>(mid);
   a2.g1 = nullptr;
   a2.g2 = static_cast<st *>(dst);
   for (int d = 0; d < 3; ++d) a1.dim[d] = a2.dim[d] = dim[d];
   a1.k = a2.k = k;
-  return psk::LaunchStrideTB2Grid<P>(a1, a2, dim[0], dim[1], dim[2], s);
+  return psk::LaunchStrideTB2Grid<P>(a1, a2, dim[0], dim[1], dim[2], s, mid != nullptr);
 }
 
-hipError_t LaunchStridePairStorage(int storage, bool periodic, const void *src, void *dst, const int *dim,
-                                   const Coef7 &k, hipStream_t s) {
+hipError_t LaunchStridePairStorage(int storage, bool periodic, const void *src, void *dst, void *mid,
+                                   const int *dim, const Coef7 &k, hipStream_t s) {
   switch (storage) {
     case PS_HIP_STORAGE_NATIVE:
-      return periodic ? LaunchStridePair<psk::Native<float>, true>(src, dst, dim, k, s)
-                      : LaunchStridePair<psk::Native<float>, false>(src, dst, dim, k, s);
+      return periodic ? LaunchStridePair<psk::Native<float>, true>(src, dst, mid, dim, k, s)
+                      : LaunchStridePair<psk::Native<float>, false>(src, dst, mid, dim, k, s);
     case PS_HIP_STORAGE_BF16:
-      return periodic ? LaunchStridePair<psk::BF16, true>(src, dst, dim, k, s)
-                      : LaunchStridePair<psk::BF16, false>(src, dst, dim, k, s);
+      return periodic ? LaunchStridePair<psk::BF16, true>(src, dst, mid, dim, k, s)
+                      : LaunchStridePair<psk::BF16, false>(src, dst, mid, dim, k, s);
     case PS_HIP_STORAGE_FP16:
-      return periodic ? LaunchStridePair<psk::FP16, true>(src, dst, dim, k, s)
-                      : LaunchStridePair<psk::FP16, false>(src, dst, dim, k, s);
+      return periodic ? LaunchStridePair<psk::FP16, true>(src, dst, mid, dim, k, s)
+                      : LaunchStridePair<psk::FP16, false>(src, dst, mid, dim, k, s);
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t LaunchPairStorage(const Variant &v, int storage, bool periodic, const void *src, void *dst,
+hipError_t LaunchPairStorage(const Variant &v, int storage, bool periodic, const void *src, void *dst, void *mid,
                              const int *dim, const Coef7 &k, hipStream_t s) {
   switch (storage) {
     case PS_HIP_STORAGE_NATIVE:
-      return periodic ? LaunchPair<psk::Native<float>, true>(v, src, dst, dim, k, s)
-                      : LaunchPair<psk::Native<float>, false>(v, src, dst, dim, k, s);
+      return periodic ? LaunchPair<psk::Native<float>, true>(v, src, dst, mid, dim, k, s)
+                      : LaunchPair<psk::Native<float>, false>(v, src, dst, mid, dim, k, s);
     case PS_HIP_STORAGE_BF16:
-      return periodic ? LaunchPair<psk::BF16, true>(v, src, dst, dim, k, s)
-                      : LaunchPair<psk::BF16, false>(v, src, dst, dim, k, s);
+      return periodic ? LaunchPair<psk::BF16, true>(v, src, dst, mid, dim, k, s)
+                      : LaunchPair<psk::BF16, false>(v, src, dst, mid, dim, k, s);
     case PS_HIP_STORAGE_FP16:
-      return periodic ? LaunchPair<psk::FP16, true>(v, src, dst, dim, k, s)
-                      : LaunchPair<psk::FP16, false>(v, src, dst, dim, k, s);
+      return periodic ? LaunchPair<psk::FP16, true>(v, src, dst, mid, dim, k, s)
+                      : LaunchPair<psk::FP16, false>(v, src, dst, mid, dim, k, s);
   }
   return hipErrorInvalidValue;
 }
@@ -374,36 +378,57 @@ int PSHipDiffusion7Run(__PSGrid *g1, __PSGrid *g2, const __PSDomain *dom, const 
   hipStream_t s = Runtime::Get().stream();
   const int storage = Rec(g1)->storage;
   const bool periodic = boundary == PS_HIP_PERIODIC;
-  int first = 0;        // first map index left to run one step at a time
-  const void *src = g1->p0;
   if (v.family == 4) {
     const Variant &one = kVariants[kStarVariant - 1];
-    const int fused = steps / 2 - 1;  // the last pair stays unfused so g2 ends as in the ref target
     // The default variant fuses rows that are not whole 16-byte vectors on
     // the stride skeleton (whole-grid runs, rows up to 512 points).
     const bool star_pair = PairApplies(v, g1->dim, box);
     bool whole = true;
     for (int d = 0; d < 3; ++d) whole = whole && box.lo[d] == 0 && box.hi[d] == g1->dim[d];
     const bool stride_pair = !star_pair && variant == 0 && whole && g1->dim[0] <= 512;
-    if (fused > 0 && (star_pair || stride_pair)) {
-      // Fused pairs ping-pong between g1 and a third buffer; the last pair then
-      // reads whichever holds the field: map(src -> g2), map(g2 -> g1).
+    auto pair = [&](const void *src, void *dst, void *mid) {
+      return star_pair ? LaunchPairStorage(v, storage, periodic, src, dst, mid, g1->dim, k, s)
+                       : LaunchStridePairStorage(storage, periodic, src, dst, mid, g1->dim, k, s);
+    };
+    auto single = [&](const void *in, void *out) { return LaunchStep(one, storage, periodic, in, out, g1->dim, box, k, s); };
+    const int pairs = steps / 2;
+    if (pairs >= 2 && (star_pair || stride_pair)) {
       void *aux = Runtime::Get().Aux(Rec(g1)->bytes);
       if (!aux) return PS_HIP_ERR_NOMEM;
-      for (int i = 0; i < fused; ++i) {
-        void *dst = (i & 1) ? g1->p0 : aux;
-        const hipError_t e = star_pair ? LaunchPairStorage(v, storage, periodic, src, dst, g1->dim, k, s)
-                                       : LaunchStridePairStorage(storage, periodic, src, dst, g1->dim, k, s);
-        if (e != hipSuccess) return PS_HIP_ERR_DEVICE;
+      // Map i reads g1 and writes g2 for even i, the reverse for odd i, so
+      // after `steps` maps the grid the last map wrote holds u_steps and the
+      // other u_(steps-1).  An odd count first runs map 0 (g1 -> g2) alone;
+      // the pairs then start from A and leave u_steps in A, u_(steps-1) in B.
+      void *A = g1->p0, *B = g2->p0;
+      if (steps & 1) {
+        if (single(g1->p0, g2->p0) != hipSuccess) return PS_HIP_ERR_DEVICE;
+        A = g2->p0;
+        B = g1->p0;
+      }
+      if (v.renamed) {
+        // Every pair fused; the last pass also stores its intermediate step
+        // into B (psk::FusedPathDst).
+        const void *src = A;
+        for (int j = 0; j < pairs; ++j) {
+          void *dst = psk::FusedPathDst(j, pairs, A, B, aux);
+          if (pair(src, dst, j == pairs - 1 ? B : nullptr) != hipSuccess) return PS_HIP_ERR_DEVICE;
+          src = dst;
+        }
+        return PS_HIP_OK;
+      }
+      // Moving-queue variants: pairs - 1 fused passes between A and the third
+      // buffer, then the last pair unfused (src -> B -> A).
+      const void *src = A;
+      for (int j = 0; j < pairs - 1; ++j) {
+        void *dst = (j & 1) ? A : aux;
+        if (pair(src, dst, nullptr) != hipSuccess) return PS_HIP_ERR_DEVICE;
         src = dst;
       }
-      first = 2 * fused;
+      if (single(src, B) != hipSuccess || single(B, A) != hipSuccess) return PS_HIP_ERR_DEVICE;
+      return PS_HIP_OK;
     }
-    for (int i = first; i < steps; ++i) {
-      const void *in = (i == first) ? src : ((i - first) & 1 ? g2->p0 : g1->p0);
-      void *out = ((i - first) & 1) ? g1->p0 : g2->p0;
-      if (LaunchStep(one, storage, periodic, in, out, g1->dim, box, k, s) != hipSuccess)
-        return PS_HIP_ERR_DEVICE;
+    for (int i = 0; i < steps; ++i) {
+      if (single((i & 1) ? g2->p0 : g1->p0, (i & 1) ? g1->p0 : g2->p0) != hipSuccess) return PS_HIP_ERR_DEVICE;
     }
     return PS_HIP_OK;
   }

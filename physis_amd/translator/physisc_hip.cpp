@@ -558,6 +558,11 @@ struct Kernel {
   bool wrap = false;
   bool star = false;    // vectorised star plan (compute() available)
   std::string out_grid; // the single emitted grid when star
+  // Two ping-pong maps of a star kernel may fuse into one pass only when the
+  // body never reads the grid it emits: in a fused pass the second map's
+  // reads of its output grid (the first map's input) would see neither the
+  // value the separate steps see nor a finished buffer.
+  bool fusable = false;
   // Locals assigned once from PSGridDim(staged grid, d): name -> d.
   std::map<std::string, int> dimvar;
   int clamp_ifs = 0;    // clamped-face reads of the staged grid (ClampIf)
@@ -790,6 +795,9 @@ void Translator::AnalyzeKernel(Kernel &k) {
     if (in_star && tail_ok && (k.rx | k.ry | k.rz) <= 1 && og.elem == "float" && og.rank == 3) {
       k.star = true;
       k.out_grid = *k.emitted.begin();
+      k.fusable = true;
+      for (const Get &g : k.gets)
+        if (g.grid == k.out_grid) k.fusable = false;
     }
   }
   if (k.star && !k.wrap) {
@@ -1020,7 +1028,7 @@ std::string Translator::EmitPlan(const Kernel &k) {
   std::string sv = k.staged.empty() ? "float" : k.gtype.at(k.staged).elem;
   o << "// Plan for kernel " << f.name << " (rank " << k.rank << ", staged grid: "
     << (k.staged.empty() ? "none" : k.staged) << ", radius " << k.rx << "/" << k.ry << "/" << k.rz
-    << (k.star ? ", star form" : "") << ")\n";
+    << (k.star ? ", star form" : "") << (k.fusable ? ", fusable pairs" : "") << ")\n";
   o << "template <class SF>\nstruct " << P << " {\n";
   o << "  using SV = " << sv << ";\n";
   o << "  static constexpr int NS = " << (k.staged.empty() ? 0 : 1) << ", RX = " << k.rx << ", RY = " << k.ry
@@ -1161,23 +1169,30 @@ std::string Translator::EmitMapAndLaunch(const Kernel &k) {
   o << "  }\n";
   if (cfg_.error_check) o << "  __PSCheckDeviceError(\"" << f.name << "\");\n";
   o << "}\n\n";
-  if (k.star) {
+  if (k.fusable) {
     // Ping-pong fusion: two maps of this kernel that differ only by swapping
-    // the staged and the emitted grid, over the whole grid.
+    // the staged and the emitted grid, over the whole grid.  Other grids the
+    // body reads must be neither of the two (a fused pass reads them while
+    // the swapped grids hold other time levels than separate steps see).
     o << "static int __PSFusable_" << f.name << "(const struct " << S << " *a, const struct " << S << " *b) {\n";
     o << "  if (a->" << k.staged << " != b->" << k.out_grid << " || a->" << k.out_grid << " != b->" << k.staged
       << " || a->" << k.staged << " == a->" << k.out_grid << ") return 0;\n";
     for (const Param &p : k.all) {
       if (p.name == k.staged || p.name == k.out_grid) continue;
-      if (p.is_grid) o << "  if (a->" << p.name << " != b->" << p.name << ") return 0;\n";
-      else o << "  if (memcmp(&a->" << p.name << ", &b->" << p.name << ", sizeof(a->" << p.name
-             << ")) != 0) return 0;\n";
+      if (p.is_grid) {
+        o << "  if (a->" << p.name << " != b->" << p.name << ") return 0;\n";
+        o << "  if (a->" << p.name << " == a->" << k.staged << " || a->" << p.name << " == a->" << k.out_grid
+          << ") return 0;\n";
+      } else {
+        o << "  if (memcmp(&a->" << p.name << ", &b->" << p.name << ", sizeof(a->" << p.name
+          << ")) != 0) return 0;\n";
+      }
     }
     o << "  const __PSGrid *g = a->" << k.staged << ";\n";
     o << "  for (int d = 0; d < 3; ++d) {\n"
       << "    if (a->dom.local_min[d] != 0 || a->dom.local_max[d] != g->dim[d]) return 0;\n"
       << "    if (b->dom.local_min[d] != 0 || b->dom.local_max[d] != g->dim[d]) return 0;\n  }\n";
-    o << "  return g->dim[0] % 4 == 0 && g->dim[0] <= 1024;\n}\n\n";
+    o << "  return psk::FusedPairSupported(g->dim[0], g->dim[1], g->dim[2]);\n}\n\n";
     o << "template <class SF>\nstatic void __PSRunPingPongT_" << f.name << "(const struct " << S
       << " *a, const struct " << S << " *b, int iter, hipStream_t st) {\n";
     o << "  const __PSGrid *g = a->" << k.staged << ";\n";
@@ -1211,7 +1226,7 @@ std::string Translator::EmitRun(const std::vector<std::pair<std::string, std::st
     o << "\");\n  __PSStopwatch __st;\n  __PSStopwatchStart(&__st);\n";
   }
   bool fusable = cfg_.fuse && maps.size() == 2 && maps[0].first == maps[1].first &&
-                 kernels_.at(maps[0].first).star && modes[0] == 0 && modes[1] == 0;
+                 kernels_.at(maps[0].first).fusable && modes[0] == 0 && modes[1] == 0;
   if (fusable) {
     const std::string &kn = maps[0].first;
     o << "  if (iter > 1 && __PSFusable_" << kn << "(&s0, &s1)) {\n";

@@ -131,3 +131,65 @@ def test_bf16_mixed_batch_equals_single_grid_runs(rt, cfg, boundary):
     for g1, g2 in batch_pairs + solo_pairs:
         g1.free()
         g2.free()
+
+
+@pytest.mark.parametrize("steps", [2, 4, 5, 8])
+@pytest.mark.parametrize("cfg", [(2, 8, 16, 2), (0, 8, 16, 2), (0, 8, 64, 2)])
+def test_batch_step_counts_bitwise_vs_oracle(rt, cfg, steps):
+    """Fused runs of every pair-count parity: the passes follow a three-buffer
+    path back to A, the last storing its intermediate step into B (even
+    counts) or followed by one step A -> B (odd counts)."""
+    shapes = SHAPES if cfg[0] <= 0 else NARROW
+    pairs, fields, coefs, batch = make_batch(rt, shapes, seed=13, boundary=0)
+    batch.configure(*cfg)
+    print("config", batch.config_name(), "steps", steps, flush=True)
+    batch.run(steps, graph=True)
+    check_against_oracle(pairs, fields, coefs, 0, steps)
+    batch.destroy()
+    for g1, g2 in pairs:
+        g1.free()
+        g2.free()
+
+
+def test_batched_workload_256_grids_200_steps_equal_single_grid_runs(rt):
+    """The batched workload at its size: 256 FP32 grids with edges drawn
+    log-uniform in [32, 256] per axis (seed 2024), fields uniform [0, 1),
+    clamped, 200 steps with the autotuned launch shape and one hipGraph.
+    Every grid (both buffers) ends bitwise equal to the same grid advanced
+    alone by the plain single-grid run, and the three smallest grids equal
+    the oracle."""
+    coef = [0.1] * 6 + [0.4]
+    steps = 200
+    rng = np.random.default_rng(2024)
+    shapes = [tuple(int(round(v)) for v in np.exp(rng.uniform(np.log(32), np.log(256), 3))) for _ in range(256)]
+    pairs = [(rt.Grid(s), rt.Grid(s)) for s in shapes]
+    for i, (a, b) in enumerate(pairs):
+        a.fill_uniform(1000 + i)
+        b.fill_uniform(5000 + i)
+    batch = rt.Batch(pairs, [coef] * len(pairs), rt.CLAMPED)
+    batch.tune(8)
+    print("tuned", batch.config_name(), flush=True)
+    for i, (a, b) in enumerate(pairs):
+        a.fill_uniform(1000 + i)
+        b.fill_uniform(5000 + i)
+    batch.run(steps)
+    rt.synchronize()
+    print("batched run done", flush=True)
+    smallest = sorted(range(len(shapes)), key=lambda i: np.prod(shapes[i]))[:3]
+    for i, (a, b) in enumerate(pairs):
+        s1, s2 = rt.Grid(shapes[i]), rt.Grid(shapes[i])
+        s1.fill_uniform(1000 + i)
+        s2.fill_uniform(5000 + i)
+        rt.diffusion7_run(s1, s2, coef, rt.CLAMPED, steps)
+        ga, gb, w1, w2 = a.copyout(), b.copyout(), s1.copyout(), s2.copyout()
+        assert np.array_equal(ga, w1), "grid %d %s" % (i, shapes[i])
+        assert np.array_equal(gb, w2), "grid %d %s" % (i, shapes[i])
+        if i in smallest:
+            o1, o2 = O.diffusion7(O.uniform_field(shapes[i], seed=1000 + i), coef, periodic=False, steps=steps)
+            assert np.array_equal(ga, o1) and np.array_equal(gb, o2), "grid %d vs oracle" % i
+        s1.free()
+        s2.free()
+    batch.destroy()
+    for a, b in pairs:
+        a.free()
+        b.free()

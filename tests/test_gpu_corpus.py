@@ -118,10 +118,56 @@ def test_star_program_ragged_rows_bitwise_vs_oracle(tmp_path, shape, periodic):
     field = O.uniform_field((nx, ny, nz), seed=40 + nx)
     src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
     field.astype(np.float32).tofile(src)
-    out = subprocess.run([exe("star_ragged"), str(nx), str(ny), str(nz), str(it), str(periodic), str(src), str(dst)],
-                         capture_output=True, timeout=120)
+    out = subprocess.run([exe("star_ragged"), str(nx), str(ny), str(nz), str(it), str(periodic), str(src), str(dst),
+                          "--physis-trace"], capture_output=True, timeout=120)
     assert out.returncode == 0, out.stderr.decode()[-2000:]
     got = np.fromfile(dst, np.float32).reshape(2, nz, ny, nx)
     want1, want2 = O.diffusion7(field, [0.11, 0.07, 0.05, 0.13, 0.02, 0.17, 0.45], periodic=bool(periodic),
                                 steps=2 * it)
     assert np.array_equal(got[0], want1) and np.array_equal(got[1], want2)
+    # Every pair ran fused, on the tile the row width calls for.
+    tile = "vectorised" if nx % 4 == 0 else "stride"
+    assert ("Physis: fused pairs: %d passes on the %s tile" % (it, tile)) in out.stderr.decode()
+
+
+def wave_oracle(u, v, k, iters):
+    """The leapfrog program of tests/golden/programs/wave_pingpong.c evaluated
+    on the CPU in FP32, operation for operation as its kernel body is written
+    (w + e + n + s + d + t - 6c, then 2c - b + k*lap), with separate steps:
+    map(u -> v) then map(v -> u), iters times."""
+    import numpy as np
+    k = np.float32(k)
+
+    def step(a, b):
+        c = a
+        lap = (np.roll(a, 1, 2) + np.roll(a, -1, 2) + np.roll(a, 1, 1) + np.roll(a, -1, 1) + np.roll(a, 1, 0)
+               + np.roll(a, -1, 0) - np.float32(6.0) * c)
+        return np.float32(2.0) * c - b + k * lap
+
+    for _ in range(iters):
+        v = step(u, v)
+        u = step(v, u)
+    return u, v
+
+
+@pytest.mark.parametrize("shape", [(64, 24, 20, 5), (37, 11, 9, 4)])
+def test_kernel_reading_its_output_grid_runs_unfused_and_matches(tmp_path, shape):
+    """A ping-pong pair whose kernel reads the grid it emits (leapfrog wave,
+    tests/golden/programs/wave_pingpong.c) is not fused: both grids equal a
+    step-by-step FP32 evaluation of the program bitwise, and no fused pass
+    is traced."""
+    import numpy as np
+    from oracle import oracle as O
+    nx, ny, nz, it = shape
+    u = O.uniform_field((nx, ny, nz), seed=7)
+    v = (O.uniform_field((nx, ny, nz), seed=8) * np.float32(0.5)).astype(np.float32)
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    np.concatenate([u.ravel(), v.ravel()]).astype(np.float32).tofile(src)
+    print("wave", shape, flush=True)
+    out = subprocess.run([exe("wave_pingpong"), str(nx), str(ny), str(nz), str(it), str(src), str(dst),
+                          "--physis-trace"], capture_output=True, timeout=120)
+    assert out.returncode == 0, out.stderr.decode()[-2000:]
+    assert "fused pairs" not in out.stderr.decode()
+    got = np.fromfile(dst, np.float32).reshape(2, nz, ny, nx)
+    want_u, want_v = wave_oracle(u, v, 0.1, it)
+    assert np.array_equal(got[0], want_u) and np.array_equal(got[1], want_v)

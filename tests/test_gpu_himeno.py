@@ -76,6 +76,49 @@ def test_himeno_config3_shape_400_sweeps_vs_fp64(rt):
     np.testing.assert_allclose(gosa, wgosa, rtol=1e-4)
 
 
+# Unit roundoff of 16-bit storage (BF16: 8 significant bits, FP16: 11).  The
+# workload's iteration diverges (|p| ~ 3e3 after 40 sweeps), and a rounding
+# error grows with the field, so the relative error accumulates about one
+# rounding per sweep: bound 2 * sweeps * u for the field (max error / peak)
+# and twice that for the residual sums of squares.  Measured at 40 sweeps:
+# BF16 7.1e-2 (bound 0.156).
+UNIT_ROUNDOFF16 = {1: 2.0 ** -9, 2: 2.0 ** -12}
+
+
+@pytest.mark.parametrize("storage", [1, 2])
+def test_himeno_config3_shape_16bit_storage_vs_fp64(rt, storage):
+    """The 257x129x129 workload with 16-bit grid storage (FP32 arithmetic):
+    error of the one-sweep-per-launch path against the FP64 oracle within
+    2 * sweeps * u after 40 sweeps (400 sweeps diverge in FP64 too:
+    |p| ~ 1e63), and the fused-sweep path (two and three sweeps per pass)
+    equal to it bitwise."""
+    shape = (257, 129, 129)
+    p, coefs = himeno_inputs(shape, seed=200)
+    sweeps = 40
+    got0, got1, gosa = run_hip(rt, p, coefs, 0.8, sweeps, storage=storage)
+    want0, _, wgosa = O.himeno(p.astype(np.float64), [c.astype(np.float64) for c in coefs], 0.8, sweeps,
+                               dtype=np.float64)
+    rel = np.max(np.abs(got0 - want0)) / np.max(np.abs(want0))
+    print("storage", storage, "257x129x129, %d sweeps: max error / peak vs FP64" % sweeps, rel, flush=True)
+    assert rel < 2 * sweeps * UNIT_ROUNDOFF16[storage]
+    np.testing.assert_allclose(gosa, wgosa, rtol=4 * sweeps * UNIT_ROUNDOFF16[storage])
+    for degree, by in ((2, 8), (3, 16)):
+        print("fused degree", degree, flush=True)
+        g0, g1 = rt.Grid(shape, storage=storage), rt.Grid(shape, storage=storage)
+        g0.copyin(p)
+        g1.copyin(p)
+        grids = []
+        for c in coefs:
+            g = rt.Grid(shape, storage=storage)
+            g.copyin(c)
+            grids.append(g)
+        fgosa = rt.himeno_run_tb(g0, g1, grids, 0.8, sweeps, degree, by, 1, 32)
+        assert np.array_equal(g0.copyout(), got0) and np.array_equal(g1.copyout(), got1)
+        np.testing.assert_allclose(fgosa, gosa, rtol=1e-12)
+        for g in grids + [g0, g1]:
+            g.free()
+
+
 def test_himeno_convergent_form_400_sweeps_vs_fp64(rt):
     """With the workload's coefficient ranges (b-terms ~1) the relaxation
     diverges (|p| ~ 3e3 after 40 sweeps, ~1e63 after 400 in FP64), so 400

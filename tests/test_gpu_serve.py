@@ -81,21 +81,28 @@ def test_fp32_results_match_oracle(rt, mixed):
             g1, _ = O.diffusion7(field, [0.1] * 6 + [0.4], periodic=False, steps=steps)
             want = float(np.sum(g1.astype(np.float64)))
         else:
-            coefs = []
-            for i in range(12):
-                if i == 3:
-                    coefs.append(np.full((n, n, n), np.float32(1.0) / np.float32(6.0), np.float32))
-                elif 4 <= i <= 6 or i == 11:
-                    coefs.append(np.zeros((n, n, n), np.float32))
-                elif i == 10:
-                    coefs.append(np.ones((n, n, n), np.float32))
-                else:
-                    coefs.append(O.uniform_field((n, n, n), seed=q["seed"] + i, lo=0.9, hi=1.0))
-            z = np.arange(n, dtype=np.float32)
-            p = np.broadcast_to((z * z / np.float32((n - 1) * (n - 1)))[:, None, None], (n, n, n)).astype(np.float32)
-            p0, _, _ = O.himeno(p, coefs, 0.8, steps)
-            want = float(np.sum(p0.astype(np.float64)))
+            want = jacobi_checksum(q["seed"], n, steps)
         assert abs(r["checksum"] - want) <= 1e-12 * max(1.0, abs(want)), (q, r["checksum"], want)
+
+
+def jacobi_checksum(seed, n, steps):
+    """FP64 sum of the oracle's p0 after `steps` sweeps of a served Jacobi
+    request: a0..a2 and c0..c2 uniform in [0.9, 1.0], a3 = 1/6, b = 0,
+    bnd = 1, wrk1 = 0, p = (z/(n-1))^2 on an n^3 grid."""
+    coefs = []
+    for i in range(12):
+        if i == 3:
+            coefs.append(np.full((n, n, n), np.float32(1.0) / np.float32(6.0), np.float32))
+        elif 4 <= i <= 6 or i == 11:
+            coefs.append(np.zeros((n, n, n), np.float32))
+        elif i == 10:
+            coefs.append(np.ones((n, n, n), np.float32))
+        else:
+            coefs.append(O.uniform_field((n, n, n), seed=seed + i, lo=0.9, hi=1.0))
+    z = np.arange(n, dtype=np.float32)
+    p = np.broadcast_to((z * z / np.float32((n - 1) * (n - 1)))[:, None, None], (n, n, n)).astype(np.float32)
+    p0, _, _ = O.himeno(p, coefs, 0.8, steps)
+    return float(np.sum(p0.astype(np.float64)))
 
 
 def test_large_jacobi_requests_run_in_arrival_order_and_small_work_overlaps(rt):
@@ -119,3 +126,74 @@ def test_large_jacobi_requests_run_in_arrival_order_and_small_work_overlaps(rt):
     for q, r in zip(reqs, res):
         alone = rt.serve_trace([dict(q, arrival=0.0)], BIG_BUDGET, TICK, rt.SERVE_BATCHED)[0]
         assert key(alone) == key(r), q
+
+
+def workload_trace():
+    """The served workload: 24 requests with Poisson arrivals at 40/s (seed
+    7), 70% 7-point diffusion and 30% 19-point Jacobi, cubic edges uniform in
+    [32, 384], FP32 or BF16 storage per request, each to residual 1e-5 or
+    2000 steps; plus two requests of 350^3 or more so that both kinds come in
+    both storage kinds at that size (the seeded trace has a 377^3 FP32
+    diffusion and a 383^3 BF16 Jacobi request)."""
+    rng = np.random.default_rng(7)
+    t, reqs = 0.0, []
+    for i in range(24):
+        t += rng.exponential(1.0 / 40.0)
+        reqs.append(dict(kind=0 if rng.random() < 0.7 else 1, n=int(rng.integers(32, 385)),
+                         storage=int(rng.random() < 0.5), seed=1000 + i, arrival=t, max_steps=2000, tol=1e-5))
+    reqs.append(dict(kind=0, n=360, storage=1, seed=2000, arrival=0.30, max_steps=2000, tol=1e-5))
+    reqs.append(dict(kind=1, n=352, storage=0, seed=2001, arrival=0.31, max_steps=2000, tol=1e-5))
+    reqs.sort(key=lambda r: r["arrival"])
+    return reqs
+
+
+WORKLOAD_BUDGET = 48 << 30
+WORKLOAD_TICK = 50
+
+
+@pytest.fixture(scope="module")
+def workload(rt):
+    reqs = workload_trace()
+    big = [(q["kind"], q["storage"]) for q in reqs if q["n"] >= 350]
+    assert set(big) >= {(0, 0), (0, 1), (1, 0), (1, 1)}
+    print("serving the workload trace", flush=True)
+    return reqs, rt.serve_trace(reqs, WORKLOAD_BUDGET, WORKLOAD_TICK, rt.SERVE_BATCHED)
+
+
+def test_workload_mix_same_alone_in_the_mix_and_sequential(rt, workload):
+    """Every request of the served workload completes, and its result (status,
+    step count, FP64 checksum of the final field) is bitwise the same in the
+    mix, served alone, and in the sequential plain-run loop."""
+    reqs, mixed = workload
+    for q, r in zip(reqs, mixed):
+        assert r["status"] == 0, q
+        assert r["steps"] <= q["max_steps"] + WORKLOAD_TICK - 1
+    seq = rt.serve_trace(reqs, WORKLOAD_BUDGET, WORKLOAD_TICK, rt.SERVE_SEQUENTIAL)
+    for q, a, b in zip(reqs, mixed, seq):
+        assert key(a) == key(b), q
+    for q, r in zip(reqs, mixed):
+        print("alone", q["kind"], q["n"], q["storage"], flush=True)
+        alone = rt.serve_trace([dict(q, arrival=0.0)], WORKLOAD_BUDGET, WORKLOAD_TICK, rt.SERVE_BATCHED)[0]
+        assert key(alone) == key(r), q
+
+
+def test_workload_fp32_checksums_match_oracle(rt, workload):
+    """FP32 requests of the workload whose CPU oracle run is short (up to
+    3e9 point updates for diffusion, 2e8 for Jacobi) have the oracle's FP64
+    checksum of the final field, to FP64 summation order."""
+    reqs, mixed = workload
+    checked = 0
+    for q, r in zip(reqs, mixed):
+        n, steps = q["n"], r["steps"]
+        if q["storage"] != 0 or n ** 3 * steps > (3e9 if q["kind"] == 0 else 2e8):
+            continue
+        print("oracle", q["kind"], n, steps, flush=True)
+        if q["kind"] == 0:
+            field = O.uniform_field((n, n, n), seed=q["seed"])
+            g1, _ = O.diffusion7(field, [0.1] * 6 + [0.4], periodic=False, steps=steps)
+            want = float(np.sum(g1.astype(np.float64)))
+        else:
+            want = jacobi_checksum(q["seed"], n, steps)
+        assert abs(r["checksum"] - want) <= 1e-12 * max(1.0, abs(want)), (q, r["checksum"], want)
+        checked += 1
+    assert checked >= 2

@@ -113,3 +113,67 @@ def test_periodic_kernels_have_no_exact_form(tmp_path):
     body = ("if (x == 0) w = PSGridGet(g1, x, y, z); else w = PSGridGet(g1, x-1, y, z);\n"
             "  e = PSGridGetPeriodic(g1, x+1, y, z);\n  n = s = b = t = c;")
     assert exact_body(translate(tmp_path, body)) is None
+
+
+WAVE = """
+#include "physis/physis.h"
+static void wave(const int x, const int y, const int z, PSGrid3DFloat a, PSGrid3DFloat b, float k) {
+  float c, w;
+  c = PSGridGet(a, x, y, z);
+  w = PSGridGetPeriodic(a, x - 1, y, z);
+  PSGridEmit(b, 2.0f * c - %(read)s + k * w);
+  return;
+}
+int main(int argc, char *argv[]) {
+  PSInit(&argc, &argv, 3, 16, 16, 16);
+  PSGrid3DFloat u = PSGrid3DFloatNew(16, 16, 16);
+  PSGrid3DFloat v = PSGrid3DFloatNew(16, 16, 16);
+  PSDomain3D d = PSDomain3DNew(0, 16, 0, 16, 0, 16);
+  PSStencilRun(PSStencilMap(wave, d, u, v, 0.1f), PSStencilMap(wave, d, v, u, 0.1f), 4);
+  PSFinalize();
+  return 0;
+}
+"""
+
+
+def translate_text(tmp_path, text):
+    if not os.path.exists(PHYSISC):
+        pytest.skip("physisc-hip is not built (make -C physis_amd/translator)")
+    src = tmp_path / "w.c"
+    src.write_text(text)
+    out = tmp_path / "w.hip"
+    subprocess.run([PHYSISC, "-o", str(out), str(src)], check=True, capture_output=True)
+    return out.read_text()
+
+
+@pytest.mark.parametrize("read", ["PSGridGet(b, x, y, z)", "PSGridGetPeriodic(b, x + 1, y, z)"])
+def test_kernel_reading_its_output_grid_is_not_fused(tmp_path, read):
+    """A ping-pong pair of a kernel that reads the grid it emits (a leapfrog
+    wave: b = 2a - b + ...) must run as separate steps: in a fused pass the
+    second map would read its output grid while it holds another time level
+    than two separate steps see."""
+    hip = translate_text(tmp_path, WAVE % {"read": read})
+    assert "__PSRunPingPong_wave" not in hip and "__PSFusable_wave" not in hip
+    assert "fusable pairs" not in hip
+
+
+def test_star_kernel_not_reading_its_output_is_fused(tmp_path):
+    hip = translate_text(tmp_path, WAVE % {"read": "c"})
+    assert "__PSFusable_wave" in hip and "fusable pairs" in hip
+    # Rows of any supported width qualify (the stride tile serves ragged
+    # rows), not just rows of whole 16-byte vectors.
+    fus = hip[hip.find("static int __PSFusable_wave"):]
+    fus = fus[:fus.find("\n}\n")]
+    assert "psk::FusedPairSupported" in fus and "% 4" not in fus
+
+
+def test_fusable_check_rejects_grids_aliasing_the_swapped_pair(tmp_path):
+    """Another grid the body reads may not be one of the two swapped grids:
+    __PSFusable compares it with both at run time."""
+    text = (WAVE % {"read": "PSGridGet(g, x, y, z)"}).replace(
+        "PSGrid3DFloat b, float k", "PSGrid3DFloat b, PSGrid3DFloat g, float k").replace(
+        "u, v, 0.1f), PSStencilMap(wave, d, v, u, 0.1f)", "u, v, u, 0.1f), PSStencilMap(wave, d, v, u, u, 0.1f)")
+    hip = translate_text(tmp_path, text)
+    fus = hip[hip.find("static int __PSFusable_wave"):]
+    fus = fus[:fus.find("\n}\n")]
+    assert "a->g == a->a || a->g == a->b" in fus
