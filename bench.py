@@ -10,9 +10,17 @@ Grids are device resident before the timed region; the timed region runs
 exactly --steps steps.  The periodic form is timed too and reported beside.
 
 value     = GLUPS = 512^3 * steps / seconds / 1e9 (clamped form)
-roofline  = algorithmic HBM bytes (8 B per lattice update: one FP32 read,
-            one FP32 write; diffusion3d.h:97-100) / average kernel duration
-            from HIP events on the runtime stream, against 8 TB/s
+roofline  = the bytes the launches of the timed run must move (a fused pass
+            of two steps reads the field once and writes it once: 8 B per
+            point; the run's last pass also writes the intermediate step: 12 B
+            per point; a single step 8 B per point) / their total duration
+            from HIP events on the runtime stream, against 8 TB/s.  The
+            reference's own figure (8 B per lattice update for every step,
+            diffusion3d.h:97-100) is reported beside it as
+            single_step_equiv_gbs: fusing steps lets it exceed the peak.
+fp64_error = max |u - u_exact| / peak after the same number of steps from a
+            separable cosine mode, whose exact FP64 solution is the mode times
+            lambda^steps.
 cpu_baseline = the reference's own single-threaded CPU baseline
             (examples/diffusion-benchmark/{main,baseline,diffusion3d}.cc built
             -O3 into oracle/_ref) on 20 steps of the same 512^3 update.
@@ -46,6 +54,34 @@ def time_steps(rt, g1, g2, boundary, steps, variant):
     return t1 - t0, timer.elapsed_ms() / 1e3
 
 
+def must_move_bytes(n, steps, elem=4):
+    """HBM bytes the launches of one default run of `steps` steps must move
+    on an n^3 grid: with at least two pairs every pair is one fused pass
+    (read + write of the field), the last pass also writes the intermediate
+    step, and an odd count runs one single step first; otherwise every step
+    is a single step (read + write)."""
+    pts = n ** 3
+    pairs = steps // 2
+    if pairs >= 2:
+        return pts * elem * (2 * pairs + 1 + (2 if steps % 2 else 0))
+    return pts * elem * 2 * steps
+
+
+def host_info():
+    """Model name and logical core count of the host CPU."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"host_cpu": model, "host_cores_available": cores, "host_cores_total": os.cpu_count()}
+
+
 def cpu_baseline(n):
     """The reference's CPU baseline on a bounded sample (20 steps at n^3)."""
     exe = os.path.join(ROOT, "oracle", "_ref", "diffusion_baseline.exe")
@@ -57,9 +93,10 @@ def cpu_baseline(n):
             m = re.search(r"Elapsed time\s*:\s*([0-9.]+)", out)
             if m:
                 sec = float(m.group(1))
-                return {"value": n ** 3 * count / sec / 1e9, "unit": "GLUPS", "cores": 1, "kind": "reference",
-                        "sample": "%d steps of %d^3 FP32 7-pt clamped diffusion, reference "
-                                  "examples/diffusion-benchmark baseline built -O3, 1 thread" % (count, n)}
+                return dict({"value": n ** 3 * count / sec / 1e9, "unit": "GLUPS", "cores": 1, "kind": "reference",
+                             "sample": "%d steps of %d^3 FP32 7-pt clamped diffusion, reference "
+                                       "examples/diffusion-benchmark baseline built -O3, 1 thread" % (count, n)},
+                            **host_info())
         except (subprocess.SubprocessError, OSError) as e:
             print("cpu baseline (reference) failed: %s" % e, file=sys.stderr)
     # Fallback: the oracle's restatement of the ref target.
@@ -71,8 +108,63 @@ def cpu_baseline(n):
     O.diffusion7(field, COEF, periodic=False, steps=steps)
     sec = time.perf_counter() - t0
     del field, np
-    return {"value": n ** 3 * steps / sec / 1e9, "unit": "GLUPS", "cores": 1, "kind": "port",
-            "sample": "%d steps of %d^3 FP32 7-pt clamped diffusion, ref-target restatement, 1 thread" % (steps, n)}
+    return dict({"value": n ** 3 * steps / sec / 1e9, "unit": "GLUPS", "cores": 1, "kind": "port",
+                 "sample": "%d steps of %d^3 FP32 7-pt clamped diffusion, ref-target restatement, 1 thread"
+                           % (steps, n)}, **host_info())
+
+
+def fp64_error(rt, n, steps):
+    """Max error / peak of the default clamped run of `steps` steps on n^3,
+    against the exact FP64 solution from the separable cosine mode
+    cos(pi k (j + 1/2) / n), k = 2 (an eigenvector of the clamped operator
+    with eigenvalue lambda = cc + 2 c_nb sum_d cos(pi k / n) per step)."""
+    import numpy as np
+    j = np.arange(n, dtype=np.float64)
+    k = 2
+    c = np.cos(np.pi * k * (j + 0.5) / n)
+    lam = 0.4 + 0.6 * np.cos(np.pi * k / n)
+    cyx = c[:, None] * c[None, :]
+    field = (c[:, None, None] * cyx[None]).astype(np.float32)
+    g1, g2 = rt.Grid((n, n, n)), rt.Grid((n, n, n))
+    g1.copyin(field)
+    del field
+    rt.diffusion7_run(g1, g2, COEF, 0, steps)
+    got = g1.copyout()
+    g1.free()
+    g2.free()
+    decay = lam ** steps
+    err = 0.0
+    for z in range(0, n, 64):
+        want = c[z:z + 64, None, None] * cyx[None] * decay
+        err = max(err, float(np.max(np.abs(got[z:z + 64].astype(np.float64) - want))))
+    return {"workload": "%d^3 clamped, %d steps from the cosine mode k=2" % (n, steps),
+            "max_rel_err_vs_fp64": err / decay,
+            "note": "the reference's FP32 evaluation order (no FMA), which the kernel reproduces bitwise, "
+                    "gives 1.1e-5 to 1.2e-5 at 1000 steps"}
+
+
+def dropin_bench(n, steps):
+    """The drop-in path timed without host copies: the reference's diffusion
+    benchmark program translated by physisc-hip with TRACE_KERNEL, driven by
+    tests/golden/dropin_timing.cc (a warm run, then `steps` steps); the time
+    is the one its traced PSStencilRun reports (launches to the end of the
+    last kernel)."""
+    exe = os.path.join(ROOT, "tests", "_corpus_bin", "diffusion_dropin_timing")
+    if not os.path.exists(exe):
+        return None
+    try:
+        out = subprocess.run([exe, str(n), str(steps), "--physis-trace"], capture_output=True, text=True,
+                             timeout=120, check=True).stderr
+    except (subprocess.SubprocessError, OSError) as e:
+        print("drop-in timing failed: %s" % e, file=sys.stderr)
+        return None
+    times = re.findall(r"Stencil finished \(time: ([0-9.]+)\)", out)
+    if len(times) < 2:
+        return None
+    ms = float(times[-1])
+    return {"workload": "reference examples/diffusion-benchmark/diffusion3d_physis.c translated by physisc-hip "
+                        "(TRACE_KERNEL), %d^3, %d steps, clamped" % (n, steps),
+            "ms": ms, "glups": round(n ** 3 * steps / (ms / 1e3) / 1e9, 1)}
 
 
 def batched_bench(rt, n_grids=256, steps=200, seed=2024):
@@ -263,6 +355,10 @@ def himeno_bench(rt, sweeps=400, seed=300):
         wall, dev = timed(lambda n: rt.himeno_run(p0, p1, grids, 0.8, n))
         out[name] = {"ms_per_sweep": round(dev / sweeps * 1e3, 4), "glups": round(interior * sweeps / dev / 1e9, 1),
                      "gbs": round(bytes_sweep * sweeps / dev / 1e9, 1), "wall_s": round(wall, 3)}
+        out["note"] = ("with these coefficient ranges the iteration diverges (|p| ~ 3e3 after 40 sweeps, ~1e63 "
+                       "after 400 in FP64), so the FP32 and BF16 fields overflow to inf before 400 sweeps; the "
+                       "kernels' time does not depend on the values.  Error against FP64 is checked at 40 sweeps "
+                       "(tests/test_gpu_himeno.py)")
         # Several sweeps per HBM pass (same results bitwise): the fastest of
         # the fused shapes the solve server's tuner chooses from.
         best = None
@@ -382,6 +478,10 @@ def main():
     print("reduce and host copies: %s" % json.dumps(aux), file=sys.stderr, flush=True)
     refbench = reference_benchmark(n, args.steps)
     print("reference benchmark: %s" % json.dumps(refbench), file=sys.stderr, flush=True)
+    dropin = dropin_bench(n, args.steps)
+    print("drop-in path, kernel only: %s" % json.dumps(dropin), file=sys.stderr, flush=True)
+    accuracy = fp64_error(rt, n, args.steps)
+    print("accuracy: %s" % json.dumps(accuracy), file=sys.stderr, flush=True)
     batched = None
     if not args.no_batched:
         batched = batched_bench(rt)
@@ -395,7 +495,8 @@ def main():
 
     wall, dev = results[0]
     glups = n ** 3 * args.steps / wall / 1e9
-    achieved = BYTES_PER_LUP * n ** 3 / (dev / args.steps) / 1e9
+    achieved = must_move_bytes(n, args.steps) / dev / 1e9
+    single_step_equiv = BYTES_PER_LUP * n ** 3 * args.steps / dev / 1e9
     traffic = load_traffic()
     cpu = None if args.no_cpu else cpu_baseline(n)
     line = {
@@ -416,7 +517,12 @@ def main():
                    "variant": args.variant,
                    "periodic_glups": round(n ** 3 * args.steps / results[1][0] / 1e9, 2),
                    "hbm_roofline_pct": round(100 * achieved / HBM_PEAK_GBS, 2),
+                   "single_step_equiv_gbs": round(single_step_equiv, 1),
+                   "kernel_s": round(dev, 6),
+                   "must_move_bytes": must_move_bytes(n, args.steps),
                    "stream_copy_gbs": round(copy_gbs, 1),
+                   "fp64_error": accuracy,
+                   "dropin_kernel_only": dropin,
                    "storage16": storage16,
                    "reduce_and_host_copy": aux,
                    "reference_benchmark": refbench,

@@ -1,7 +1,9 @@
 /*
  * stopwatch.h -- wall-clock stopwatch used by generated tracing code.
  * Same names and millisecond unit as the reference
- * (include/physis/stopwatch.h:17-47); implemented on C11 timespec_get.
+ * (include/physis/stopwatch.h:17-47); implemented on the POSIX monotonic
+ * clock where <time.h> declares it, else on C11 timespec_get (strict ISO C
+ * modes).
  */
 #ifndef PHYSIS_STOPWATCH_H_
 #define PHYSIS_STOPWATCH_H_
@@ -13,7 +15,11 @@ typedef struct {
 } __PSStopwatch;
 
 static inline void __PSStopwatchQuery(__PSStopwatch *w) {
+#if defined(CLOCK_MONOTONIC)
+  clock_gettime(CLOCK_MONOTONIC, &w->ts);
+#else
   timespec_get(&w->ts, TIME_UTC);
+#endif
 }
 
 /* Milliseconds from begin to end. */

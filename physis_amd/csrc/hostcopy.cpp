@@ -45,21 +45,73 @@ struct Pool {
   hipStream_t stream[kDmaStreams] = {};  // DMA streams (default priority)
   hipEvent_t start = nullptr;
 
-  // Creates workers up to `want` (kept until ReleaseHostCopyPool).
-  void Ensure(int want) {
+  // Creates workers up to `want` (kept until ReleaseHostCopyPool) and
+  // returns how many exist.  Pinned memory, streams or events that cannot be
+  // made leave fewer workers (0: the copy falls back to one hipMemcpy)
+  // instead of failing the copy.
+  int Ensure(int want) {
     if (!start) {
-      CheckHip(hipEventCreateWithFlags(&start, hipEventDisableTiming), "hipEventCreate");
-      for (auto &st : stream) CheckHip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+      if (hipEventCreateWithFlags(&start, hipEventDisableTiming) != hipSuccess) {
+        start = nullptr;
+        return 0;
+      }
+      for (auto &st : stream) {
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+          st = nullptr;
+          Release();
+          return 0;
+        }
+      }
     }
     for (; n < want; ++n) {
       Worker &k = w[n];
-      for (int b = 0; b < 2; ++b) {
-        CheckHip(hipHostMalloc(&k.buf[b], kChunk, hipHostMallocDefault), "hipHostMalloc(copy stage)");
-        CheckHip(hipEventCreateWithFlags(&k.ev[b], hipEventDisableTiming), "hipEventCreate");
+      bool made = true;
+      for (int b = 0; b < 2 && made; ++b) {
+        if (hipHostMalloc(&k.buf[b], kChunk, hipHostMallocDefault) != hipSuccess) {
+          k.buf[b] = nullptr;
+          made = false;
+        } else if (hipEventCreateWithFlags(&k.ev[b], hipEventDisableTiming) != hipSuccess) {
+          k.ev[b] = nullptr;
+          made = false;
+        }
+      }
+      if (!made) {
+        FreeWorker(k);
+        (void)hipGetLastError();  // clear the failed call's error
+        break;
       }
     }
+    return n;
+  }
+
+  static void FreeWorker(Worker &k) {
+    for (int b = 0; b < 2; ++b) {
+      if (k.buf[b]) (void)hipHostFree(k.buf[b]);
+      if (k.ev[b]) (void)hipEventDestroy(k.ev[b]);
+    }
+    k = Worker{};
+  }
+
+  // Frees every worker, the streams and the start event (caller holds mu).
+  void Release() {
+    for (auto &st : stream)
+      if (st) (void)hipStreamSynchronize(st);
+    for (int i = 0; i < n; ++i) FreeWorker(w[i]);
+    n = 0;
+    for (auto &st : stream) {
+      if (st) (void)hipStreamDestroy(st);
+      st = nullptr;
+    }
+    if (start) (void)hipEventDestroy(start);
+    start = nullptr;
   }
 };
+
+// One blocking copy through the HIP runtime's own staging.
+int PlainCopy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
+  if (hipMemcpyAsync(dst, src, bytes, kind, s) != hipSuccess) return PS_HIP_ERR_DEVICE;
+  return hipStreamSynchronize(s) == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
+}
 
 Pool &ThePool() {
   static Pool p;
@@ -78,8 +130,10 @@ int WorkersFor(size_t bytes) {
 int Pipeline(void *dev, void *host, size_t bytes, bool to_device, hipStream_t s) {
   Pool &P = ThePool();
   std::lock_guard<std::mutex> lk(P.mu);
-  const int W = WorkersFor(bytes);
-  P.Ensure(W);
+  const int W = std::min(WorkersFor(bytes), P.Ensure(WorkersFor(bytes)));
+  if (W < 1)
+    return to_device ? PlainCopy(dev, host, bytes, hipMemcpyHostToDevice, s)
+                     : PlainCopy(host, dev, bytes, hipMemcpyDeviceToHost, s);
   if (hipEventRecord(P.start, s) != hipSuccess) return PS_HIP_ERR_DEVICE;
   for (auto &st : P.stream)
     if (hipStreamWaitEvent(st, P.start, 0) != hipSuccess) return PS_HIP_ERR_DEVICE;
@@ -163,36 +217,18 @@ void PrepareHostCopy(size_t bytes) {
 
 int CopyHostToDevice(void *dst, const void *src, size_t bytes, hipStream_t s) {
   if (bytes >= kPipelineMin) return Pipeline(dst, const_cast<void *>(src), bytes, true, s);
-  if (hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return PS_HIP_ERR_DEVICE;
-  return hipStreamSynchronize(s) == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
+  return PlainCopy(dst, src, bytes, hipMemcpyHostToDevice, s);
 }
 
 int CopyDeviceToHost(void *dst, const void *src, size_t bytes, hipStream_t s) {
   if (bytes >= kPipelineMin) return Pipeline(const_cast<void *>(src), dst, bytes, false, s);
-  if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s) != hipSuccess) return PS_HIP_ERR_DEVICE;
-  return hipStreamSynchronize(s) == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
+  return PlainCopy(dst, src, bytes, hipMemcpyDeviceToHost, s);
 }
 
 void ReleaseHostCopyPool() {
   Pool &P = ThePool();
   std::lock_guard<std::mutex> lk(P.mu);
-  for (auto &st : P.stream)
-    if (st) (void)hipStreamSynchronize(st);
-  for (int i = 0; i < P.n; ++i) {
-    Worker &k = P.w[i];
-    for (int b = 0; b < 2; ++b) {
-      (void)hipHostFree(k.buf[b]);
-      (void)hipEventDestroy(k.ev[b]);
-    }
-    k = Worker{};
-  }
-  P.n = 0;
-  for (auto &st : P.stream) {
-    if (st) (void)hipStreamDestroy(st);
-    st = nullptr;
-  }
-  if (P.start) (void)hipEventDestroy(P.start);
-  P.start = nullptr;
+  P.Release();
 }
 
 }  // namespace physis_hip

@@ -200,6 +200,8 @@ class Server {
 
  private:
   int Admit(int idx, double now, hipStream_t s, bool batched);
+  int Allocate(Live &lv, const PSHipRequest &r);
+  void PretuneJacobi();
   void Retire(Live &lv, double now, hipStream_t s);
   void FreeRetired();
   void Finish(Live &lv);  // residual and step count after a completed tick
@@ -242,12 +244,15 @@ std::map<std::pair<int, int>, JacobiShape> &JacobiTuned() {
   return t;
 }
 
-// Times every candidate shape for kTuneSweeps sweeps on the request's own
-// (not yet initialised) grids and returns the fastest.
+std::pair<int, int> JacobiKey(const PSHipRequest &r) { return {r.storage, (r.n + 63) / 64}; }
+
+// Times every candidate shape for kTuneSweeps sweeps on a request's
+// (zero-filled) grids and returns the fastest.  Called before a trace
+// starts, with nothing else on the device, so that the timings are not
+// disturbed by other requests' work and the scheduler loop never waits on
+// them.
 JacobiShape Server::TuneJacobi(Live &lv, hipStream_t s) {
-  const std::pair<int, int> key(lv.rq->storage, (lv.rq->n + 63) / 64);
-  auto it = JacobiTuned().find(key);
-  if (it != JacobiTuned().end()) return it->second;
+  CheckHip(hipDeviceSynchronize(), "sync before tuning");
   hipEvent_t e0, e1;
   CheckHip(hipEventCreate(&e0), "event");
   CheckHip(hipEventCreate(&e1), "event");
@@ -268,16 +273,29 @@ JacobiShape Server::TuneJacobi(Live &lv, hipStream_t s) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  JacobiTuned()[key] = best;
   return best;
 }
 
-int Server::Admit(int idx, double now, hipStream_t s, bool batched) {
-  const PSHipRequest &r = reqs_[idx];
-  Live lv;
-  lv.idx = idx;
-  lv.rq = &r;
-  lv.admitted = now;
+// Tunes the Jacobi launch shape of every (storage, size class) the trace
+// holds that the process has not tuned yet, on temporary grids of the
+// class's first request, before the trace's clock starts.  A class whose
+// grids cannot be allocated keeps the default shape.
+void Server::PretuneJacobi() {
+  for (int i = 0; i < n_; ++i) {
+    const PSHipRequest &r = reqs_[i];
+    if (r.kind != 1 || JacobiTuned().count(JacobiKey(r))) continue;
+    if ((int64_t)RequestBytes(r) > budget_) continue;  // never admitted
+    Live lv;
+    lv.rq = &r;
+    if (Allocate(lv, r) == PS_HIP_OK) JacobiTuned()[JacobiKey(r)] = TuneJacobi(lv, main_);
+    CheckHip(hipStreamSynchronize(main_), "tuning sync");
+    Release(lv);
+  }
+}
+
+// Device memory of a request: its grids, the fused-step buffer and its
+// scratch; on failure everything allocated so far is released.
+int Server::Allocate(Live &lv, const PSHipRequest &r) {
   const int dim[3] = {r.n, r.n, r.n};
   auto grid = [&](__PSGrid **g) { return GridNewDevice(g, 4, 3, dim, r.storage); };
   int st = grid(&lv.A);
@@ -301,13 +319,24 @@ int Server::Admit(int idx, double now, hipStream_t s, bool batched) {
     (void)hipGetLastError();
     st = PS_HIP_ERR_NOMEM;
   }
-  if (st != PS_HIP_OK) {
-    Release(lv);
-    return st;
-  }
+  if (st != PS_HIP_OK) Release(lv);
+  return st;
+}
+
+int Server::Admit(int idx, double now, hipStream_t s, bool batched) {
+  const PSHipRequest &r = reqs_[idx];
+  Live lv;
+  lv.idx = idx;
+  lv.rq = &r;
+  lv.admitted = now;
+  const int st = Allocate(lv, r);
+  if (st != PS_HIP_OK) return st;
   lv.bytes = RequestBytes(r);
   used_ += (int64_t)lv.bytes;
-  if (r.kind == 1 && batched) lv.shape = TuneJacobi(lv, s);
+  if (r.kind == 1 && batched) {
+    auto it = JacobiTuned().find(JacobiKey(r));
+    lv.shape = it != JacobiTuned().end() ? it->second : kJacobiCandidates[0];
+  }
   // Initial state.
   if (r.kind == 0) {
     FillUniformOn(lv.A, r.seed, 0.f, 1.f, s);
@@ -461,6 +490,7 @@ void Server::LaunchDiffusion(int storage, std::vector<Live *> &v) {
 // batch and every Jacobi request starts its next tick as soon as its last
 // one has completed.
 int Server::RunContinuous() {
+  PretuneJacobi();
   t0_ = std::chrono::steady_clock::now();
   for (int i = 0; i < n_; ++i) out_[i] = PSHipResult{PS_HIP_ERR_INVALID, 0, 0.0, 0.0, 0.0, 0.0};
   for (auto &g : groups_) CheckHip(hipEventCreateWithFlags(&g.ev, hipEventDisableTiming), "hipEventCreate");

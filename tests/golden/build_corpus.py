@@ -30,18 +30,32 @@ SKIPPED = {}
 # The diffusion benchmark of the reference (examples/diffusion-benchmark):
 # its Physis program, translated by physisc-hip, linked with the benchmark's
 # own C++ harness built for the Physis variant (main.cc with -DPHYSIS).
+# A second build with TRACE_KERNEL (the reference's tracing configuration
+# key): its PSStencilRun waits for its kernels and reports their time with
+# --physis-trace, which times the drop-in path without the host copies.
 BENCH_DIR = "/root/reference/examples/diffusion-benchmark"
 BENCH_EXE = os.path.join(OUT, "diffusion_benchmark_physis")
+BENCH_TRACED_EXE = os.path.join(OUT, "diffusion_benchmark_physis_traced")
+# The same traced program driven by a timing harness of this repository
+# (tests/golden/dropin_timing.cc): a warm run, then the timed run.
+DROPIN_EXE = os.path.join(OUT, "diffusion_dropin_timing")
 
 
 def build_benchmark(phc):
     if not os.path.isdir(BENCH_DIR):
         return True
+    cfg = os.path.join(OUT, "trace_kernel.cfg")
+    with open(cfg, "w") as f:
+        f.write("TRACE_KERNEL = true\n")
     try:
-        phc.build(os.path.join(BENCH_DIR, "diffusion3d_physis.c"), BENCH_EXE,
-                  extra_cxx=[os.path.join(BENCH_DIR, f) for f in ("main.cc", "diffusion3d.cc", "baseline.cc")],
-                  cxx_flags=["-DPHYSIS", "-I" + BENCH_DIR, "-I" + os.path.join(ROOT, "include"), "-w"])
-        os.remove(BENCH_EXE + ".hip")
+        for exe, config in ((BENCH_EXE, None), (BENCH_TRACED_EXE, cfg)):
+            phc.build(os.path.join(BENCH_DIR, "diffusion3d_physis.c"), exe, config=config,
+                      extra_cxx=[os.path.join(BENCH_DIR, f) for f in ("main.cc", "diffusion3d.cc", "baseline.cc")],
+                      cxx_flags=["-DPHYSIS", "-I" + BENCH_DIR, "-I" + os.path.join(ROOT, "include"), "-w"])
+            os.remove(exe + ".hip")
+        phc.build(os.path.join(BENCH_DIR, "diffusion3d_physis.c"), DROPIN_EXE, config=cfg,
+                  extra_cxx=[os.path.join(HERE, "dropin_timing.cc")], cxx_flags=["-w"])
+        os.remove(DROPIN_EXE + ".hip")
         return True
     except subprocess.CalledProcessError:
         return False

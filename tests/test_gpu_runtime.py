@@ -221,3 +221,66 @@ def test_user_type_grid_roundtrip_with_padding(rt):
         L.__PSHipUtypeCopyout(dev, out2.ctypes.data)
         assert np.array_equal(out2.view(np.uint8), out.view(np.uint8))
         L.__PSHipUtypeDevFree(dev)
+
+
+def _abi(rt):
+    """The reference-named C entry points, declared for ctypes."""
+    L = rt.lib()
+    f = {n: getattr(L, n) for n in ("__PSGridNew", "__PSGridFree", "__PSGridCopyin", "__PSGridCopyout",
+                                    "__PSGridSet", "__PSGridSwap", "__PSGridMirror", "__PSGridGetID")}
+    f["__PSGridNew"].restype = rt._GP
+    f["__PSGridNew"].argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]
+    f["__PSGridFree"].argtypes = [rt._GP, ctypes.c_void_p]
+    f["__PSGridCopyin"].argtypes = [rt._GP, ctypes.c_void_p, ctypes.c_void_p]
+    f["__PSGridCopyout"].argtypes = [rt._GP, ctypes.c_void_p, ctypes.c_void_p]
+    f["__PSGridSwap"].argtypes = [rt._GP]
+    f["__PSGridMirror"].argtypes = [rt._GP]
+    f["__PSGridGetID"].argtypes = [rt._GP]
+    f["__PSGridSet"].restype = None  # variadic: arguments passed as typed ctypes values
+    return f
+
+
+@pytest.mark.parametrize("dims,dtype", [((7,), np.float32), ((5, 3), np.float64), ((4, 3, 2), np.float32),
+                                        ((6, 5, 4), np.float64), ((64, 2, 3), np.int32)])
+def test_grid_set_swap_mirror_through_the_c_abi(rt, dims, dtype):
+    """__PSGridSet writes one element at (x[, y[, z]]) (PSIndex varargs, x
+    fastest; reference libphysis_rt_cuda.cc:164-177), leaving the others;
+    __PSGridSwap and __PSGridMirror are no-ops on a single device, as on the
+    reference's CUDA target (libphysis_rt_cuda.cc:141-142), and the grid id
+    is 0."""
+    f = _abi(rt)
+    d = (ctypes.c_int * 3)(*(list(dims) + [1] * (3 - len(dims))))
+    elm = np.dtype(dtype).itemsize
+    g = f["__PSGridNew"](elm, len(dims), d, None)
+    assert g
+    shape = tuple(reversed(dims))
+    a = (np.arange(int(np.prod(dims))) * 3 + 1).astype(dtype).reshape(shape)
+    f["__PSGridCopyin"](g, a.ctypes.data, None)
+    want = a.copy()
+    points = [tuple(0 for _ in dims), tuple(n - 1 for n in dims), tuple(n // 2 for n in dims)]
+    for i, p in enumerate(points):
+        v = np.array([-7.5 - i], dtype)
+        f["__PSGridSet"](g, ctypes.c_void_p(v.ctypes.data), *[ctypes.c_int32(c) for c in p])
+        want[tuple(reversed(p))] = v[0]
+    f["__PSGridSwap"](g)
+    f["__PSGridMirror"](g)
+    assert f["__PSGridGetID"](g) == 0
+    out = np.empty_like(a)
+    f["__PSGridCopyout"](g, out.ctypes.data, None)
+    assert np.array_equal(out, want)
+    f["__PSGridFree"](g, None)
+
+
+def test_grid_set_on_bf16_storage_rounds_like_a_copy(rt):
+    """On a grid with BF16 storage __PSGridSet rounds the FP32 value to
+    nearest even, as PSGridCopyin does."""
+    f = _abi(rt)
+    g = rt.Grid((8, 4, 2), storage=rt.STORAGE_BF16)
+    v = np.array([1.0 + 2.0 ** -8 + 2.0 ** -12], np.float32)   # rounds up to 1 + 2^-7
+    f["__PSGridSet"](g.handle, ctypes.c_void_p(v.ctypes.data), ctypes.c_int32(3), ctypes.c_int32(2),
+                     ctypes.c_int32(1))
+    out = g.copyout()
+    assert out[1, 2, 3] == np.float32(1.0 + 2.0 ** -7)
+    out[1, 2, 3] = 0
+    assert np.all(out == 0)
+    g.free()

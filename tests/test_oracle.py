@@ -236,3 +236,76 @@ def test_himeno_polynomial_isolates_each_term(field, term):
     np.testing.assert_allclose(p1[1:-1, 1:-1, 1:-1], want[1:-1, 1:-1, 1:-1], atol=1e-12)
     # Boundary cells are outside the interior domain and keep their values.
     assert np.array_equal(p1[0], p[0]) and np.array_equal(p1[:, :, -1], p[:, :, -1])
+
+
+def periodic_matrix(n, cminus, cplus):
+    """1-D periodic operator of one axis: neighbour j-1 weighted cminus, j+1
+    cplus, indices wrapped (i + n) % n."""
+    A = np.zeros((n, n))
+    for j in range(n):
+        A[j, (j - 1) % n] += cminus
+        A[j, (j + 1) % n] += cplus
+    return A
+
+
+def test_periodic_asymmetric_vs_dense_operator():
+    """Periodic branch with six different neighbour coefficients: a w/e, n/s
+    or b/t swap, or a wrong wrap, changes the result (the Fourier-mode test
+    uses symmetric coefficients and would not see a swap).  The FP32 term
+    order is the clamped branch's (one shared expression), pinned bit for bit
+    by the reference's Neumann-condition fixture."""
+    nx, ny, nz = 5, 4, 3
+    ce, cw, cn, cs, ct, cb, cc = 0.11, 0.07, 0.05, 0.13, 0.02, 0.17, 0.45
+    Ax = periodic_matrix(nx, cw, ce)
+    Ay = periodic_matrix(ny, cn, cs)
+    Az = periodic_matrix(nz, cb, ct)
+    Ix, Iy, Iz = np.eye(nx), np.eye(ny), np.eye(nz)
+    A = (cc * np.kron(Iz, np.kron(Iy, Ix)) + np.kron(Iz, np.kron(Iy, Ax)) + np.kron(Iz, np.kron(Ay, Ix))
+         + np.kron(Az, np.kron(Iy, Ix)))
+    field = O.uniform_field((nx, ny, nz), seed=4).astype(np.float64)
+    for steps in (1, 4):
+        want = np.linalg.matrix_power(A, steps) @ field.ravel()
+        g1, g2 = O.diffusion7(field, [ce, cw, cn, cs, ct, cb, cc], periodic=True, steps=steps, dtype=np.float64)
+        got = g2 if steps % 2 else g1
+        np.testing.assert_allclose(got.ravel(), want, atol=1e-13)
+
+
+HIMENO_ORIGINAL = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "himeno_original_jacobi.exe")
+
+
+@pytest.mark.parametrize("shape,sweeps", [((17, 9, 7), 1), ((17, 9, 7), 2), ((33, 12, 10), 5), ((5, 4, 3), 3)])
+def test_himeno_oracle_equals_reference_original_jacobi(tmp_path, shape, sweeps):
+    """The Himeno restatement (of himenobmtxpa_physis.c:331-361) against the
+    benchmark's original C solver, jacobi() of himenobmtxpa_original.c:299-346,
+    built from the reference sources (oracle/Makefile ref) and run on the
+    same random inputs: the pressure field is bitwise equal after every
+    sweep count.  The original's indices (i, j, k), k fastest, are z, y, x
+    here, and its terms come in the port's order under that permutation, so
+    the FP32 roundings agree one for one; its residual is a float sum in
+    loop order, the oracle's an FP64 sum."""
+    import subprocess
+    if not os.path.exists(HIMENO_ORIGINAL):
+        pytest.skip("the reference's Himeno original is not built (make -C oracle ref)")
+    nx, ny, nz = shape
+    p = O.uniform_field(shape, seed=90, lo=-1.0, hi=1.0)
+    coefs = []
+    for i in range(12):
+        if i == 3:
+            c = O.uniform_field(shape, seed=91 + i, lo=0.15, hi=0.18)
+        elif i == 10:
+            c = (O.uniform_field(shape, seed=91 + i) < 0.8).astype(np.float32)  # bnd: 0 or 1
+        elif i == 11:
+            c = O.uniform_field(shape, seed=91 + i, lo=-0.1, hi=0.1)
+        else:
+            c = O.uniform_field(shape, seed=91 + i, lo=0.9, hi=1.1)
+        coefs.append(c)
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    np.concatenate([a.ravel() for a in [p] + coefs]).astype(np.float32).tofile(src)
+    subprocess.run([HIMENO_ORIGINAL, str(nx), str(ny), str(nz), str(sweeps), str(src), str(dst)], check=True,
+                   timeout=60)
+    out = np.fromfile(dst, np.float32)
+    got_p, got_gosa = out[:-1].reshape(nz, ny, nx), float(out[-1])
+    p0, p1, gosa = O.himeno(p, coefs, 0.8, sweeps)
+    want = p1 if sweeps % 2 else p0
+    assert np.array_equal(got_p, want)
+    assert abs(got_gosa - gosa[-1]) <= 1e-4 * abs(gosa[-1])
