@@ -70,6 +70,19 @@ extern int PSHipStreamCopy(__PSGrid *dst, const __PSGrid *src);
  * fixed order (bitwise reproducible); MAX and MIN are exact. */
 extern int PSHipReduceGrid(double *out, enum PSReduceOp op, __PSGrid *g);
 
+/* PSHipReduceGrid for callers that cannot allocate device or pinned memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  It calls no hipMalloc,
+ * hipFree or other allocating routine: the partials and the result live in
+ * a file-scope device array of 1 + 16384 doubles (128 KiB + 8 bytes) that
+ * every device the library runs on holds until the process ends, and the
+ * result is copied to *out.  Pass 1 never uses more than 16384 workgroups,
+ * so every grid fits; a size that would not returns PS_HIP_ERR_UNSUPPORTED.
+ * Same kernels and work split as PSHipReduceGrid, so the same result bits.
+ * Every call on a device shares that device's array: calls that overlap on
+ * one device (from several host threads) race. */
+extern int PSHipReduceGridNoAlloc(double *out, enum PSReduceOp op, __PSGrid *g);
+
 /* A device buffer of at least `bytes`, owned by the runtime and reused by
  * every caller on its stream (fused time steps use it as a third grid
  * buffer).  NULL if it cannot be allocated. */

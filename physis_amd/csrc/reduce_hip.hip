@@ -19,7 +19,9 @@
 #include "physis/hip/wave_reduce.h"
 
 #include <cmath>
+#include <cstdio>
 #include <limits>
+#include <mutex>
 
 namespace physis_hip {
 namespace {
@@ -191,6 +193,44 @@ int ReduceAsync(const void *p, int64_t n, int elm_size, int storage, enum PSRedu
 // Scratch doubles a reduction of any size needs: result + partials.
 size_t ReduceScratchDoubles() { return 1 + kMaxBlocks; }
 
+// ---- the reduction without allocation (PSHipReduceGridNoAlloc) -----------
+// Its result and partials live in `scratch`, a file-scope device array of
+// 1 + kMaxBlocks doubles (128 KiB + 8 bytes, held by every device the
+// library runs on until the process ends).  Host code never uses the
+// array's name as an address: NoAllocScratchInit resolves its device
+// address once per device, with that device current, and the entry point
+// passes that address to the kernels and the copy.
+constexpr size_t kNoAllocDoubles = 1 + kMaxBlocks;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+namespace {
+constexpr int kMaxDevices = 64;
+double *g_scratch_addr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Workgroups (partials) of pass 1 for n elements of this storage.
+int ReduceBlocks(int64_t n, int elm_size, int storage) {
+  if (storage == PS_HIP_STORAGE_BF16) return BlocksFor<psk::BF16>(n);
+  if (storage == PS_HIP_STORAGE_FP16) return BlocksFor<psk::FP16>(n);
+  if (elm_size == 8) return BlocksFor<psk::Native<double>>(n);
+  return BlocksFor<psk::Native<float>>(n);
+}
+}  // namespace
+
+int NoAllocScratchInit() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return PS_HIP_ERR_DEVICE;
+  if (dev < 0 || dev >= kMaxDevices) return PS_HIP_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_addr[dev]) return PS_HIP_OK;
+  void *addr = nullptr;
+  if (hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)) != hipSuccess || !addr) {
+    (void)hipGetLastError();
+    return PS_HIP_ERR_DEVICE;
+  }
+  g_scratch_addr[dev] = static_cast<double *>(addr);
+  return PS_HIP_OK;
+}
+
 }  // namespace physis_hip
 
 using namespace physis_hip;
@@ -215,6 +255,34 @@ int PSHipReduceGrid(double *out, enum PSReduceOp op, __PSGrid *g) {
   if (hipStreamSynchronize(s) != hipSuccess) return PS_HIP_ERR_DEVICE;
   *out = *static_cast<volatile double *>(host);
   return PS_HIP_OK;
+}
+
+// PSHipReduceGrid without allocating: the partials and the result go to the
+// device's `scratch` array; the result is copied to *out.  Same kernels and
+// work split as PSHipReduceGrid, so the same bits.  Calls that overlap on one
+// device race on the array.
+int PSHipReduceGridNoAlloc(double *out, enum PSReduceOp op, __PSGrid *g) {
+  if (!out || !g || !g->p0) return PS_HIP_ERR_INVALID;
+  if (op < PS_MAX || op > PS_PROD) return PS_HIP_ERR_INVALID;
+  GridRec *r = Rec(g);
+  Runtime &rt = Runtime::Get();
+  hipStream_t s = rt.stream();
+  const int dev = rt.device();
+  if (dev < 0 || dev >= kMaxDevices) return PS_HIP_ERR_UNSUPPORTED;
+  if (!g_scratch_addr[dev]) {
+    const int st = NoAllocScratchInit();  // the runtime's device is current
+    if (st != PS_HIP_OK) return st;
+  }
+  if (1 + (size_t)ReduceBlocks(g->num_elms, g->elm_size, r->storage) > kNoAllocDoubles) {
+    fprintf(stderr, "physis-hip: PSHipReduceGridNoAlloc: %lld elements need more than its %zu-double scratch\n",
+            (long long)g->num_elms, kNoAllocDoubles);
+    return PS_HIP_ERR_UNSUPPORTED;
+  }
+  double *buf = g_scratch_addr[dev];
+  int st = ReduceTo(g->p0, g->num_elms, g->elm_size, r->storage, op, buf + 1, buf, s);
+  if (st != PS_HIP_OK) return st;
+  if (hipMemcpyAsync(out, buf, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) return PS_HIP_ERR_DEVICE;
+  return hipStreamSynchronize(s) == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
 }
 
 // PSReduce(&v, op, g) on a float grid (reference libphysis_rt_ref.cc:150-153).

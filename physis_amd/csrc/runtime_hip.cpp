@@ -43,6 +43,9 @@ void Runtime::EnsureDevice() {
   // Default priority: streams of different priorities are not used.
   CheckHip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking),
            "hipStreamCreate");
+  // The no-allocation reduction's scratch address on this device; a failure
+  // here is reported by that entry point when it is called.
+  (void)NoAllocScratchInit();
   inited_ = true;
 }
 

@@ -110,6 +110,9 @@ int FillPatternOn(__PSGrid *g, int profile, float value, hipStream_t s);
 int ReduceAsync(const void *p, int64_t n, int elm_size, int storage, enum PSReduceOp op, double *result_dev,
                 double *partials_dev, hipStream_t s);
 size_t ReduceScratchDoubles();
+// Resolves the device address of the no-allocation reduction's scratch array
+// on the current device (once per device); PS_HIP_OK or an error status.
+int NoAllocScratchInit();
 // Himeno sweeps on stream s with caller scratch: gosa_dev[i] (may be NULL)
 // receives sweep i's FP64 sum of ss^2; partials needs HimenoPartials(p0).
 int HimenoEnqueue(__PSGrid *p0, __PSGrid *p1, __PSGrid *const *grids, float omega, int sweeps,
