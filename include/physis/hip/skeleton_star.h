@@ -39,10 +39,11 @@ typedef float v4f_t __attribute__((ext_vector_type(4)));
 template <class S>
 __device__ __forceinline__ void Load4(const typename S::stored_t *p, float (&v)[4]) {
   if constexpr (sizeof(typename S::stored_t) == 4) {
-    const v4f_t q = *reinterpret_cast<const v4f_t *>(p);
+    const v4f_t q = GlobalLoad(reinterpret_cast<const v4f_t *>(p));
     v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
   } else {
-    const uint2 q = *reinterpret_cast<const uint2 *>(p);
+    typedef uint32_t v2u_t __attribute__((ext_vector_type(2)));
+    const v2u_t q = GlobalLoad(reinterpret_cast<const v2u_t *>(p));
     v[0] = S::widen((uint16_t)(q.x & 0xffffu));
     v[1] = S::widen((uint16_t)(q.x >> 16));
     v[2] = S::widen((uint16_t)(q.y & 0xffffu));
@@ -55,13 +56,13 @@ template <class S>
 __device__ __forceinline__ void Store4(typename S::stored_t *p, const float (&v)[4]) {
   if constexpr (sizeof(typename S::stored_t) == 4) {
     v4f_t q = {v[0], v[1], v[2], v[3]};
-    __builtin_nontemporal_store(q, reinterpret_cast<v4f_t *>(p));
+    GlobalStoreNT(reinterpret_cast<v4f_t *>(p), q);
   } else {
     typedef uint32_t v2u_t __attribute__((ext_vector_type(2)));
     v2u_t q;
     q.x = (uint32_t)S::narrow(v[0]) | ((uint32_t)S::narrow(v[1]) << 16);
     q.y = (uint32_t)S::narrow(v[2]) | ((uint32_t)S::narrow(v[3]) << 16);
-    __builtin_nontemporal_store(q, reinterpret_cast<v2u_t *>(p));
+    GlobalStoreNT(reinterpret_cast<v2u_t *>(p), q);
   }
 }
 

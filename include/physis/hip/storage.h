@@ -11,17 +11,34 @@
 
 namespace psk {
 
+// Grid elements live in HBM.  Every grid access goes through these two
+// helpers, which address global memory explicitly: a grid pointer a kernel
+// loads from memory (a batch's argument table) is generic to the compiler,
+// and an access through a generic pointer becomes a flat instruction, which
+// also counts against the LDS wait counter, so that every wait for an LDS
+// read would also wait for the grid loads in flight.
+template <class T>
+__device__ __forceinline__ T GlobalLoad(const T *p) {
+  return *(const __attribute__((address_space(1))) T *)p;
+}
+template <class T>
+__device__ __forceinline__ void GlobalStoreNT(T *p, T v) {
+  __builtin_nontemporal_store(v, (__attribute__((address_space(1))) T *)p);
+}
+template <class T>
+__device__ __forceinline__ void GlobalStore(T *p, T v) {
+  *(__attribute__((address_space(1))) T *)p = v;
+}
+
 // Native storage: the stored type is the value type.
 template <class T>
 struct Native {
   using value_t = T;
   using stored_t = T;
   static constexpr int kBytes = sizeof(T);
-  __device__ __forceinline__ static T load(const stored_t *p) { return *p; }
-  __device__ __forceinline__ static void store(stored_t *p, T v) {
-    __builtin_nontemporal_store(v, p);
-  }
-  __device__ __forceinline__ static void store_cached(stored_t *p, T v) { *p = v; }
+  __device__ __forceinline__ static T load(const stored_t *p) { return GlobalLoad(p); }
+  __device__ __forceinline__ static void store(stored_t *p, T v) { GlobalStoreNT(p, v); }
+  __device__ __forceinline__ static void store_cached(stored_t *p, T v) { GlobalStore(p, v); }
   __device__ __forceinline__ static T widen(stored_t s) { return s; }
   __device__ __forceinline__ static stored_t narrow(T v) { return v; }
 };
@@ -41,11 +58,9 @@ struct BF16 {
     __builtin_memcpy(&s, &h, 2);
     return s;
   }
-  __device__ __forceinline__ static float load(const stored_t *p) { return widen(*p); }
-  __device__ __forceinline__ static void store(stored_t *p, float v) {
-    __builtin_nontemporal_store(narrow(v), p);
-  }
-  __device__ __forceinline__ static void store_cached(stored_t *p, float v) { *p = narrow(v); }
+  __device__ __forceinline__ static float load(const stored_t *p) { return widen(GlobalLoad(p)); }
+  __device__ __forceinline__ static void store(stored_t *p, float v) { GlobalStoreNT(p, narrow(v)); }
+  __device__ __forceinline__ static void store_cached(stored_t *p, float v) { GlobalStore(p, narrow(v)); }
 };
 
 // IEEE binary16 storage of an FP32 value (hardware conversion, nearest even).
@@ -64,11 +79,9 @@ struct FP16 {
     __builtin_memcpy(&s, &h, 2);
     return s;
   }
-  __device__ __forceinline__ static float load(const stored_t *p) { return widen(*p); }
-  __device__ __forceinline__ static void store(stored_t *p, float v) {
-    __builtin_nontemporal_store(narrow(v), p);
-  }
-  __device__ __forceinline__ static void store_cached(stored_t *p, float v) { *p = narrow(v); }
+  __device__ __forceinline__ static float load(const stored_t *p) { return widen(GlobalLoad(p)); }
+  __device__ __forceinline__ static void store(stored_t *p, float v) { GlobalStoreNT(p, narrow(v)); }
+  __device__ __forceinline__ static void store_cached(stored_t *p, float v) { GlobalStore(p, narrow(v)); }
 };
 
 }  // namespace psk
