@@ -74,6 +74,11 @@ struct FP16 {
     return (float)h;
   }
   __device__ __forceinline__ static uint16_t narrow(float v) {
+    // v is an FP32 result: the empty asm keeps the compiler from fusing the
+    // FMA that produced it with this conversion into one FP16-rounding
+    // v_fma_mixlo_f16 (one rounding instead of FP32 then FP16), which it
+    // would do in some kernels and not in others.
+    asm("" : "+v"(v));
     _Float16 h = (_Float16)v;
     uint16_t s;
     __builtin_memcpy(&s, &h, 2);

@@ -56,14 +56,14 @@ struct BatchPlan {
   __host__ __device__ static st *out_ptr(const Args &a) { return static_cast<st *>(a.out); }
   template <class C>
   __device__ static float compute(const C &ctx, const Args &a, const int x, const int y, const int z) {
-    return Diffusion7Body<kPeriodic, false, false, false>(ctx, a.k, x, y, z, a.dim[0], a.dim[1], a.dim[2]);
+    return Diffusion7Body<kPeriodic, false, false, false, kFma16<S>>(ctx, a.k, x, y, z, a.dim[0], a.dim[1], a.dim[2]);
   }
   // compute() where the stride skeleton supplies clamped-coordinate
   // neighbours along the flagged axes.
   static constexpr bool kExactNeighbours = true;
   template <bool kX, bool kY, bool kZ, class C>
   __device__ static float compute_exact(const C &ctx, const Args &a, const int x, const int y, const int z) {
-    return Diffusion7Body<kPeriodic, kX, kY, kZ>(ctx, a.k, x, y, z, a.dim[0], a.dim[1], a.dim[2]);
+    return Diffusion7Body<kPeriodic, kX, kY, kZ, kFma16<S>>(ctx, a.k, x, y, z, a.dim[0], a.dim[1], a.dim[2]);
   }
 };
 

@@ -4,8 +4,10 @@
 // (periodic form, PSGridGetPeriodic) with coefficients, plus hand-tuned
 // kernels for the same update.  Every variant evaluates
 //     cc*c + cw*w + ce*e + cs*s + cn*n + cb*b + ct*t
-// left to right in FP32 with no contraction into FMA, exactly as the
-// sequential ref target does, so results are bitwise equal to it.
+// left to right in FP32; on FP32 grids with no contraction into FMA, exactly
+// as the sequential ref target does, so results are bitwise equal to it; on
+// 16-bit grids with the products after the first fused into the running sum
+// (Diffusion7Body's kFma), the same in every variant.
 #include "runtime_hip.h"
 #include "diffusion7_body.h"
 #include "physis/hip/skeleton.h"
@@ -50,14 +52,14 @@ struct DiffusionPlan {
   }
   template <class C>
   __device__ static float compute(const C &ctx, const Args &a, const int x, const int y, const int z) {
-    return Diffusion7Body<kPeriodic, false, false, false>(ctx, a.k, x, y, z, a.dim[0], a.dim[1], a.dim[2]);
+    return Diffusion7Body<kPeriodic, false, false, false, kFma16<S>>(ctx, a.k, x, y, z, a.dim[0], a.dim[1], a.dim[2]);
   }
   // compute() for a point whose x (kX), y (kY) and z (kZ) neighbours in ctx
   // are the elements at the clamped coordinates (skeleton_star_tb.h).
   static constexpr bool kExactNeighbours = true;
   template <bool kX, bool kY, bool kZ, class C>
   __device__ static float compute_exact(const C &ctx, const Args &a, const int x, const int y, const int z) {
-    return Diffusion7Body<kPeriodic, kX, kY, kZ>(ctx, a.k, x, y, z, a.dim[0], a.dim[1], a.dim[2]);
+    return Diffusion7Body<kPeriodic, kX, kY, kZ, kFma16<S>>(ctx, a.k, x, y, z, a.dim[0], a.dim[1], a.dim[2]);
   }
 };
 
@@ -93,7 +95,18 @@ Diffusion7Direct(const typename S::stored_t *__restrict__ in, typename S::stored
     const float tv = S::load(in + col + (int64_t)Nb<kPeriodic>(z, 1, nz) * plane);
     const float wv = S::load(in + ow + zo), ev = S::load(in + oe + zo);
     const float nv = S::load(in + on + zo), sv = S::load(in + os + zo);
-    const float v = k.cc * cv + k.cw * wv + k.ce * ev + k.cs * sv + k.cn * nv + k.cb * bv + k.ct * tv;
+    float v;
+    if constexpr (kFma16<S>) {  // same evaluation as Diffusion7Body
+      v = k.cc * cv;
+      v = __builtin_fmaf(k.cw, wv, v);
+      v = __builtin_fmaf(k.ce, ev, v);
+      v = __builtin_fmaf(k.cs, sv, v);
+      v = __builtin_fmaf(k.cn, nv, v);
+      v = __builtin_fmaf(k.cb, bv, v);
+      v = __builtin_fmaf(k.ct, tv, v);
+    } else {
+      v = k.cc * cv + k.cw * wv + k.ce * ev + k.cs * sv + k.cn * nv + k.cb * bv + k.ct * tv;
+    }
     S::store(out + col + zo, v);
     bv = cv;
     cv = tv;
