@@ -216,6 +216,10 @@ typedef struct {
  * steps between residual checks.  Results are the same in both modes. */
 extern int PSHipServeTrace(const PSHipRequest *reqs, int n, int64_t budget_bytes, int tick_steps, int mode,
                            PSHipResult *results);
+/* Tunes, ahead of serving, the Jacobi launch shape of every (storage, size
+ * class) among the requests (a trace tunes the classes left untuned when
+ * it starts); the choices are kept for the process. */
+extern int PSHipServeTune(const PSHipRequest *reqs, int n, int64_t budget_bytes, int tick_steps);
 
 #ifdef __cplusplus
 }

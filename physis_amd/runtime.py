@@ -328,6 +328,21 @@ class Result(ctypes.Structure):
 SERVE_BATCHED, SERVE_SEQUENTIAL = 0, 1
 
 
+def _requests(requests):
+    n = len(requests)
+    return (Request * n)(*[Request(r["kind"], r["n"], r.get("storage", STORAGE_NATIVE), r.get("seed", 1),
+                                   r.get("arrival", 0.0), r.get("max_steps", 2000), r.get("tol", 1e-5))
+                           for r in requests])
+
+
+def serve_tune(requests, budget_bytes, tick_steps=50):
+    """Tunes the Jacobi launch shapes of the requests' size classes ahead of
+    serving (PSHipServeTune); later traces reuse them."""
+    L = lib()
+    L.PSHipServeTune.argtypes = [ctypes.POINTER(Request), ctypes.c_int, ctypes.c_int64, ctypes.c_int]
+    check(L.PSHipServeTune(_requests(requests), len(requests), budget_bytes, tick_steps), "PSHipServeTune")
+
+
 def serve_trace(requests, budget_bytes, tick_steps=50, mode=SERVE_BATCHED):
     """Serves a list of request dicts (kind, n, storage, seed, arrival,
     max_steps, tol) in real time; returns one result dict per request."""
