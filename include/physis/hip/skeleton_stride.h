@@ -45,17 +45,62 @@ struct StrideTB2Lds {
   float u1[2][BY][kRow];
 };
 
-// Publishes NP points of a row into LDS row r (pads filled for wrap plans).
+// Word 3 of a GFX9 buffer resource for raw (untyped, unswizzled) access.
+constexpr int kBufferWord3 = 0x00020000;
+
+// One grid row of nx elements addressed through a buffer resource: the
+// hardware's range check returns 0 for loads past the row's end and drops
+// stores there, so lanes need no guard, branch or exec-mask update, and
+// their offsets are 32-bit constants (no 64-bit address arithmetic per
+// access).  Stores are non-temporal, like Storage::store.
+//
+// The row must be the same for every lane of the wave (a wave owns whole
+// rows): its address and length are broadcast from the first lane, so the
+// resource sits in scalar registers even where the compiler cannot prove the
+// values uniform (pointers a batched kernel reads from its argument table);
+// otherwise it would loop over the distinct values lane by lane.
+template <class S>
+struct BufRow {
+  using st = typename S::stored_t;
+  static_assert(sizeof(st) == 4 || sizeof(st) == 2, "float grids, stored in 32 or 16 bits");
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ static st *Uniform(const st *p) {
+    const uint64_t a = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    return reinterpret_cast<st *>(((uint64_t)hi << 32) | lo);
+  }
+  __device__ __forceinline__ BufRow(const st *row, int nx)
+      : r(__builtin_amdgcn_make_buffer_rsrc(Uniform(row), (short)0,
+                                            __builtin_amdgcn_readfirstlane(nx * (int)sizeof(st)), kBufferWord3)) {}
+  __device__ __forceinline__ float load(int x) const {
+    if constexpr (sizeof(st) == 4)
+      return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, x * 4, 0, 0));
+    else
+      return S::widen(__builtin_amdgcn_raw_buffer_load_b16(r, x * 2, 0, 0));
+  }
+  __device__ __forceinline__ void store(int x, float v) const {
+    if constexpr (sizeof(st) == 4)
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, x * 4, 0, 2);
+    else
+      __builtin_amdgcn_raw_buffer_store_b16(S::narrow(v), r, x * 2, 0, 2);
+  }
+};
+
+// Publishes the NP points of every lane of a row into LDS row r (kRow =
+// 64*NP + 8 floats: lanes past the row's end write inside it, never read as
+// in-row points).  Wrap plans then fill the pads with the wrapped row ends;
+// those writes come after, so they win over a lane past the end that wrote
+// the right pad's position.  Clamped plans never select the pads.
 template <bool W, int NP>
 __device__ __forceinline__ void StridePut(float *r, const float (&v)[NP], int lane, int nx) {
 #pragma unroll
-  for (int k = 0; k < NP; ++k) {
-    const int x = lane + 64 * k;
-    if (x < nx) {
-      r[x + 4] = v[k];
-      if (W && x == nx - 1) r[3] = v[k];
-      if (W && x == 0) r[nx + 4] = v[k];
-    }
+  for (int k = 0; k < NP; ++k) r[lane + 64 * k + 4] = v[k];
+  if (W) {
+#pragma unroll
+    for (int k = 0; k < NP; ++k)
+      if (lane + 64 * k == nx - 1) r[3] = v[k];
+    if (lane == 0) r[nx + 4] = v[0];
   }
 }
 
@@ -66,13 +111,12 @@ __device__ __forceinline__ int NextPlane(int z, int nz) {
   return W ? (z + 1 == nz ? 0 : z + 1) : min(z + 1, nz - 1);
 }
 
+// The NP points of every lane of a row (0 past the row's end).
 template <class S, int NP>
 __device__ __forceinline__ void StrideLoad(const typename S::stored_t *row, float (&v)[NP], int lane, int nx) {
+  const BufRow<S> br(row, nx);
 #pragma unroll
-  for (int k = 0; k < NP; ++k) {
-    const int x = lane + 64 * k;
-    v[k] = x < nx ? S::load(row + x) : 0.f;
-  }
+  for (int k = 0; k < NP; ++k) v[k] = br.load(lane + 64 * k);
 }
 
 // One map over rows [y0, y0+BY) x planes [zb, ze) of a grid nx x ny x nz
@@ -83,11 +127,14 @@ __device__ __forceinline__ void StrideStep(const typename Plan::Args &a, int nx,
   using S = typename Plan::Storage;
   using st = typename S::stored_t;
   constexpr bool W = Plan::kWrap;
-  const int lane = threadIdx.x, wy = threadIdx.y;
+  nx = __builtin_amdgcn_readfirstlane(nx);
+  ny = __builtin_amdgcn_readfirstlane(ny);
+  nz = __builtin_amdgcn_readfirstlane(nz);
+  const int lane = threadIdx.x, wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int y = y0 + wy;
   const int64_t plane = (int64_t)nx * ny;
-  const st *in = Plan::stage_ptr(a);
-  st *out = Plan::out_ptr(a);
+  const st *in = BufRow<S>::Uniform(Plan::stage_ptr(a));
+  st *out = BufRow<S>::Uniform(Plan::out_ptr(a));
   const int64_t row = (int64_t)FixIndex<W>(y, ny) * nx;
   const bool halo_lo = wy == 0, halo_hi = wy == BY - 1;
   const int64_t hlo_row = (int64_t)FixIndex<W>(y0 - 1, ny) * nx;
@@ -116,21 +163,21 @@ __device__ __forceinline__ void StrideStep(const typename Plan::Args &a, int nx,
     if (halo_hi) StridePut<W, NP>(R[BY + 1], hhi, lane, nx);
     __syncthreads();
     if (active) {
-      st *o = out + (int64_t)z * plane + (int64_t)y * nx;
+      // Points past the row's end compute on in-bounds LDS values; their
+      // stores are dropped by the row's range.
+      const BufRow<S> o(out + (int64_t)z * plane + (int64_t)y * nx, nx);
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
         const int x = lane + 64 * k;
-        if (x < nx) {
-          StarCtx cx;
-          cx.c = c[k];
-          cx.w = R[wy + 1][x + 3];
-          cx.e = R[wy + 1][x + 5];
-          cx.n = R[wy][x + 4];
-          cx.s = R[wy + 2][x + 4];
-          cx.b = b[k];
-          cx.t = t[k];
-          S::store(o + x, ComputeAt<Plan, false, true, true>(cx, a, x, y, z));
-        }
+        StarCtx cx;
+        cx.c = c[k];
+        cx.w = R[wy + 1][x + 3];
+        cx.e = R[wy + 1][x + 5];
+        cx.n = R[wy][x + 4];
+        cx.s = R[wy + 2][x + 4];
+        cx.b = b[k];
+        cx.t = t[k];
+        o.store(x, ComputeAt<Plan, false, true, true>(cx, a, x, y, z));
       }
     }
 #pragma unroll
@@ -156,13 +203,18 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
   constexpr bool W = Plan::kWrap;
   // wy is the same for every lane of a wave (blockDim.x == 64): broadcast,
   // it keeps row offsets scalar and row-role branches free of exec masks.
+  // Grid pointers and extents are the same for the whole workgroup: broadcast
+  // so that row addresses are computed in scalar registers.
+  nx = __builtin_amdgcn_readfirstlane(nx);
+  ny = __builtin_amdgcn_readfirstlane(ny);
+  nz = __builtin_amdgcn_readfirstlane(nz);
   const int lane = threadIdx.x;
   const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int y = ya + wy;
   const int yf = FixIndex<W>(y, ny);
   const int64_t plane = (int64_t)nx * ny;
-  const st *in = Plan::stage_ptr(a1);
-  st *out = Plan::out_ptr(a2);
+  const st *in = BufRow<S>::Uniform(Plan::stage_ptr(a1));
+  st *out = BufRow<S>::Uniform(Plan::out_ptr(a2));
   const int64_t row = (int64_t)yf * nx;
   const bool halo_lo = wy == 0, halo_hi = wy == BY - 1;
   // A wave owns at most one of the two halo rows (BY >= 3).
@@ -171,16 +223,9 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
   const int64_t hrow = (int64_t)FixIndex<W>(halo_lo ? ya - 1 : ya + BY, ny) * nx;
   const bool out_row = wy >= 1 && wy <= BY - 2 && y >= 0 && y < ny;
   auto zo = [&](int z) { return (int64_t)FixIndex<W>(z, nz) * plane; };
-  // Every lane loads without a branch; points past the row end read the
-  // row's last point and are never published or stored.  (Loads outside
-  // divergent branches let the compiler wait for just the load it needs.)
-  int xc[NP];
-#pragma unroll
-  for (int q = 0; q < NP; ++q) xc[q] = min(lane + 64 * q, nx - 1);
-  auto ld = [&](const st *r, float (&v)[NP]) {
-#pragma unroll
-    for (int q = 0; q < NP; ++q) v[q] = S::load(r + xc[q]);
-  };
+  // Every lane loads without a branch (points past the row end read 0
+  // through the row's range and are never stored).
+  auto ld = [&](const st *r, float (&v)[NP]) { StrideLoad<S, NP>(r, v, lane, nx); };
   // u1 at plane k (this wave's row) from u0 planes b, c, t.
   auto level1 = [&](float (*R0)[StrideTB2Lds<NP, BY>::kRow], int kf, const float (&b)[NP], const float (&c)[NP],
                     const float (&t)[NP], float (&u)[NP]) {
@@ -201,7 +246,7 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
   // u2 output row at plane k - 1 from u1 planes b, c, t.
   auto level2 = [&](float (*R1)[StrideTB2Lds<NP, BY>::kRow], int k, const float (&b)[NP], const float (&c)[NP],
                     const float (&t)[NP]) {
-    st *o = out + (int64_t)(k - 1) * plane + (int64_t)y * nx;
+    const BufRow<S> o(out + (int64_t)(k - 1) * plane + (int64_t)y * nx, nx);
     const int rn = !W && y == 0 ? wy : wy - 1, rs = !W && y == ny - 1 ? wy : wy + 1;
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
@@ -214,8 +259,7 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
       cx.s = R1[rs][x + 4];
       cx.b = b[q];
       cx.t = t[q];
-      const float v = ComputeAt<Plan, false, true, false>(cx, a2, x, y, k - 1);
-      if (x < nx) S::store(o + x, v);
+      o.store(x, ComputeAt<Plan, false, true, false>(cx, a2, x, y, k - 1));
     }
   };
 
@@ -246,10 +290,9 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
     level1(R0, z0, Q0[i0], Q0[i1], Q0[i2], Q1[i2]);
     if constexpr (kMid) {
       if (out_row && k >= zb && k < ze) {
-        st *m = Plan::out_ptr(a1) + (int64_t)k * plane + (int64_t)y * nx;
+        const BufRow<S> m(BufRow<S>::Uniform(Plan::out_ptr(a1)) + (int64_t)k * plane + (int64_t)y * nx, nx);
 #pragma unroll
-        for (int q = 0; q < NP; ++q)
-          if (lane + 64 * q < nx) S::store(m + lane + 64 * q, Q1[i2][q]);
+        for (int q = 0; q < NP; ++q) m.store(lane + 64 * q, Q1[i2][q]);
       }
     }
     ld(in + (int64_t)z2 * plane + row, Q0[i0]);
