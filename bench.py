@@ -213,7 +213,7 @@ def batched_bench(rt, n_grids=256, steps=200, seed=2024):
             "speedup": round(seq / bat, 2), "tuned_config": cfg, "tune_s": round(tune_s, 2)}
 
 
-def served_bench(rt, n_req=24, rate=40.0, seed=7, budget_gib=48, tick=50):
+def served_bench(rt, n_req=24, rate=40.0, seed=7, budget_gib=48, tick=50, verbose=False):
     """Served workload: Poisson arrivals at a fixed rate of mixed requests (70%
     7-point diffusion, 30% 19-point Jacobi), cubic edges uniform in [32, 384],
     FP32 or BF16 storage per request, each run to residual <= 1e-5 or 2000
@@ -244,8 +244,15 @@ def served_bench(rt, n_req=24, rate=40.0, seed=7, budget_gib=48, tick=50):
         out[name] = {"solves_per_s": round(ok / wall, 3), "wall_s": round(wall, 3),
                      "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
                      "p99_latency_s": round(float(np.percentile(lat, 99)), 4), "completed": ok}
+        if verbose:
+            for q, r, l in zip(reqs, res, lat):
+                print("%s kind %d n %3d storage %d arrival %.3f admitted %.3f finished %.3f latency %.3f steps %d"
+                      % (name, q["kind"], q["n"], q["storage"], q["arrival"], r["admitted"], r["finished"], l,
+                         r["steps"]), file=sys.stderr, flush=True)
         if name == "served":
             served = res
+            if verbose:
+                time.sleep(0.25)  # an idle gap that separates the two modes in a kernel trace
     same = all(a["steps"] == b["steps"] and a["checksum"] == b["checksum"] for a, b in zip(served, res))
     out["identical_results"] = same
     out["throughput_ratio"] = round(out["served"]["solves_per_s"] / max(out["sequential"]["solves_per_s"], 1e-9), 2)
@@ -427,6 +434,9 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the served request-mix leg")
     ap.add_argument("--himeno-only", action="store_true",
                     help="run only the Himeno 257x129x129 leg (single and fused sweeps) and print its JSON")
+    ap.add_argument("--only", choices=["himeno", "served", "batched", "reduce", "storage16"],
+                    help="run only this leg and print its JSON (the profiles' commands); served also lists "
+                         "every request on stderr")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the stencil path on a single MI355X "
@@ -435,8 +445,14 @@ def main():
 
     from physis_amd import runtime as rt
     rt.init(["bench"])
-    if args.himeno_only:
+    if args.himeno_only or args.only == "himeno":
         print(json.dumps(himeno_bench(rt)), flush=True)
+        return
+    if args.only:
+        leg = {"served": lambda: served_bench(rt, verbose=True), "batched": lambda: batched_bench(rt),
+               "reduce": lambda: reduce_and_copy_bench(rt, args.size),
+               "storage16": lambda: storage16_bench(rt, args.size, args.steps)}[args.only]
+        print(json.dumps(leg()), flush=True)
         return
     n = args.size
     print("device: %s" % rt.device_name(), file=sys.stderr, flush=True)
