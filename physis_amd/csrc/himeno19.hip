@@ -4,6 +4,9 @@
 // ss*ss per sweep (the reference port leaves gosa at 0, :387-392; the
 // original accumulates it in float, himenobmtxpa_original.c:334).
 //
+// Grids stored in 16 bits evaluate the update with fused multiply-adds
+// (HimenoUpdate below); FP32 grids evaluate it as written.
+//
 // p0 is the only grid read at offsets: it is staged in the skeleton's LDS
 // ring (radius 1 in x, y and z, including the xy/yz/xz diagonals).  The 12
 // coefficient grids are read at the point itself, straight from HBM.
@@ -20,6 +23,42 @@ namespace physis_hip {
 namespace {
 
 enum { A0, A1, A2, A3, B0, B1, B2, C0, C1, C2, BND, WRK1, NCOEF };
+
+// One point of jacobi_kernel (himenobmtxpa_physis.c:331-361): G(n) is
+// coefficient grid n at the point, P(dx, dy, dz) the pressure at an offset;
+// returns the new pressure and leaves ss in *ss.  kFma: the same terms in
+// the same order with each product after the first fused into the running
+// sum (and omega*ss into the update), used for grids stored in 16 bits,
+// which are judged against FP64 within the storage's rounding; FP32 grids
+// evaluate the expression as written, as the sequential reference does.
+template <bool kFma, class G, class PF>
+__device__ __forceinline__ float HimenoUpdate(const G &g, const PF &P, float omega, float *ss_out) {
+  float s0, ss;
+  if constexpr (kFma) {
+    s0 = g(A0) * P(0, 0, 1);
+    s0 = __builtin_fmaf(g(A1), P(0, 1, 0), s0);
+    s0 = __builtin_fmaf(g(A2), P(1, 0, 0), s0);
+    s0 = __builtin_fmaf(g(B0), P(0, 1, 1) - P(0, -1, 1) - P(0, 1, -1) + P(0, -1, -1), s0);
+    s0 = __builtin_fmaf(g(B1), P(1, 1, 0) - P(1, -1, 0) - P(-1, 1, 0) + P(-1, -1, 0), s0);
+    s0 = __builtin_fmaf(g(B2), P(1, 0, 1) - P(1, 0, -1) - P(-1, 0, 1) + P(-1, 0, -1), s0);
+    s0 = __builtin_fmaf(g(C0), P(0, 0, -1), s0);
+    s0 = __builtin_fmaf(g(C1), P(0, -1, 0), s0);
+    s0 = __builtin_fmaf(g(C2), P(-1, 0, 0), s0);
+    s0 = s0 + g(WRK1);
+    ss = __builtin_fmaf(s0, g(A3), -P(0, 0, 0)) * g(BND);
+    *ss_out = ss;
+    return __builtin_fmaf(omega, ss, P(0, 0, 0));
+  } else {
+    s0 = g(A0) * P(0, 0, 1) + g(A1) * P(0, 1, 0) + g(A2) * P(1, 0, 0) +
+         g(B0) * (P(0, 1, 1) - P(0, -1, 1) - P(0, 1, -1) + P(0, -1, -1)) +
+         g(B1) * (P(1, 1, 0) - P(1, -1, 0) - P(-1, 1, 0) + P(-1, -1, 0)) +
+         g(B2) * (P(1, 0, 1) - P(1, 0, -1) - P(-1, 0, 1) + P(-1, 0, -1)) + g(C0) * P(0, 0, -1) +
+         g(C1) * P(0, -1, 0) + g(C2) * P(-1, 0, 0) + g(WRK1);
+    ss = (s0 * g(A3) - P(0, 0, 0)) * g(BND);
+    *ss_out = ss;
+    return P(0, 0, 0) + omega * ss;
+  }
+}
 
 // kOff32: every grid is below 4 GiB, so coefficient addresses are a uniform
 // base plus a 32-bit byte offset (one VGPR per load instead of a 64-bit add).
@@ -67,14 +106,9 @@ struct HimenoPlan {
       else
         return S::load(a.c[n] + o);
     };
-    float s0, ss;
-    s0 = g(A0) * P.at(0, 0, 0, 1) + g(A1) * P.at(0, 0, 1, 0) + g(A2) * P.at(0, 1, 0, 0) +
-         g(B0) * (P.at(0, 0, 1, 1) - P.at(0, 0, -1, 1) - P.at(0, 0, 1, -1) + P.at(0, 0, -1, -1)) +
-         g(B1) * (P.at(0, 1, 1, 0) - P.at(0, 1, -1, 0) - P.at(0, -1, 1, 0) + P.at(0, -1, -1, 0)) +
-         g(B2) * (P.at(0, 1, 0, 1) - P.at(0, 1, 0, -1) - P.at(0, -1, 0, 1) + P.at(0, -1, 0, -1)) +
-         g(C0) * P.at(0, 0, 0, -1) + g(C1) * P.at(0, 0, -1, 0) + g(C2) * P.at(0, -1, 0, 0) + g(WRK1);
-    ss = (s0 * g(A3) - P.at(0, 0, 0, 0)) * g(BND);
-    const float v = P.at(0, 0, 0, 0) + a.omega * ss;
+    float ss;
+    const float v = HimenoUpdate<sizeof(st) == 2>(
+        g, [&](int dx, int dy, int dz) { return P.at(0, dx, dy, dz); }, a.omega, &ss);
     if constexpr (kOff32)
       S::store(reinterpret_cast<st *>(reinterpret_cast<char *>(a.q) + ob), v);
     else
@@ -273,14 +307,8 @@ HimenoTB(TBArgs a) {
               const float *rg = dz < 0 ? r0 : dz > 0 ? r2 : r1;
               return rg[c + dy * TW + dx];
             };
-            float s0v, ss;
-            s0v = g[A0] * P(0, 0, 1) + g[A1] * P(0, 1, 0) + g[A2] * P(1, 0, 0) +
-                  g[B0] * (P(0, 1, 1) - P(0, -1, 1) - P(0, 1, -1) + P(0, -1, -1)) +
-                  g[B1] * (P(1, 1, 0) - P(1, -1, 0) - P(-1, 1, 0) + P(-1, -1, 0)) +
-                  g[B2] * (P(1, 0, 1) - P(1, 0, -1) - P(-1, 0, 1) + P(-1, 0, -1)) + g[C0] * P(0, 0, -1) +
-                  g[C1] * P(0, -1, 0) + g[C2] * P(-1, 0, 0) + g[WRK1];
-            ss = (s0v * g[A3] - P(0, 0, 0)) * g[BND];
-            v[r] = P(0, 0, 0) + a.omega * ss;
+            float ss;
+            v[r] = HimenoUpdate<sizeof(st) == 2>([&](int n) { return g[n]; }, P, a.omega, &ss);
             if (owned[r] && z >= zb && z < ze) acc[t - 1] += (double)ss * (double)ss;
             if (t == T && owned[r]) S::store(reinterpret_cast<st *>(out + ((uint32_t)z * plane + col[r])), v[r]);
           } else if (t < T) {
