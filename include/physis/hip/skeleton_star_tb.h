@@ -178,23 +178,68 @@ __device__ __forceinline__ void StarTB2Tile(const typename Plan::Args &a1, const
   };
 
   if constexpr (kRenamed) {
-    // A wave owns at most one of the two halo rows (BY >= 3): one register
-    // set and one row offset serve both.
-    const bool halo = halo_lo || halo_hi;
-    const int hslot = halo_lo ? 0 : BY + 1;
     // Every lane loads, without a branch: a lane past the row end reads the
-    // row's last four points (never stored or published).  Loads outside
-    // divergent branches let the compiler wait for just the load it needs
-    // rather than for all of them.
+    // row's last four points (never stored or published).  A load inside a
+    // branch makes the compiler wait for every load in flight where the
+    // paths join, since it cannot tell which were issued.
     const int xl = has ? xs : nx - 4;
     const int64_t lrow = (int64_t)yf * nx + xl;
-    const int64_t hrow = (int64_t)FixIndex<W>(halo_lo ? ya - 1 : ya + BY, ny) * nx + xl;
     auto ldu = [&](int64_t off, float (&v)[4]) { Load4<S>(in + off, v); };
-    float Q0[3][4], Q1[3][4] = {}, hv[4] = {};
+    // The two halo rows (ya - 1 into slot 0, ya + BY into slot BY + 1) are
+    // loaded by all lanes of the workgroup together, kHN elements per lane
+    // of the 2 * kW row elements, also without a branch (lanes past a row
+    // end, or past the rows, read an in-bounds element and publish
+    // nothing).  They stay in storage form until published, so that no
+    // conversion waits on a load before the plane is needed.
+    constexpr int kW = 256 * NWX, kT = 64 * NWX * BY;
+    constexpr int kHN = (2 * kW + kT - 1) / kT;
+    const int tid = threadIdx.x + (int)threadIdx.y * 64 * NWX;
+    int hoff[kHN], hx[kHN];
+    bool hhi[kHN], hpub[kHN];
+    {
+      const int hy_lo = FixIndex<W>(ya - 1, ny), hy_hi = FixIndex<W>(ya + BY, ny);
+#pragma unroll
+      for (int j = 0; j < kHN; ++j) {
+        const int t = tid + j * kT;
+        const int e = t % (2 * kW);
+        hhi[j] = e >= kW;
+        hx[j] = hhi[j] ? e - kW : e;
+        hpub[j] = t < 2 * kW && hx[j] < nx;
+        hoff[j] = (hhi[j] ? hy_hi : hy_lo) * nx + min(hx[j], nx - 1);
+      }
+    }
+    st hraw[kHN];
+    auto ldh = [&](int64_t zo) {
+#pragma unroll
+      for (int j = 0; j < kHN; ++j) hraw[j] = GlobalLoad(in + zo + hoff[j]);
+    };
+    auto puth = [&](int buf) {
+#pragma unroll
+      for (int j = 0; j < kHN; ++j) {
+        if (!hpub[j]) continue;
+        float *r = &L0[buf][hhi[j] ? BY + 1 : 0][0];
+        const float v = S::widen(hraw[j]);
+        r[hx[j] + 4] = v;
+        if (W) {
+          if (hx[j] == nx - 1) r[3] = v;
+          if (hx[j] == 0) r[nx + 4] = v;
+        } else {
+          if (hx[j] == 0) r[3] = v;
+          if (hx[j] == nx - 1) r[nx + 4] = v;
+        }
+      }
+    };
+    float Q0[3][4], Q1[3][4] = {};
+    // Issued in the order the loop leaves its loads in flight (the halo
+    // plane before the newest u0 plane; the scheduling barriers keep the
+    // order), so that the waits at the loop head are the loop's own.
     ldu(zoff(zb - 2) + lrow, Q0[0]);
     ldu(zoff(zb - 1) + lrow, Q0[1]);
+    __builtin_amdgcn_sched_barrier(0);
+    ldh(zoff(zb - 1));
+    __builtin_amdgcn_sched_barrier(0);
     ldu(zoff(zb) + lrow, Q0[2]);
-    if (halo) ldu(zoff(zb - 1) + hrow, hv);
+    __builtin_amdgcn_sched_barrier(0);
     // Plane indices of k (z0), k + 1 (z1) and k + 2 (z2), as below.
     int z0 = FixIndex<W>(zb - 1, nz), z1 = FixIndex<W>(zb, nz), z2 = FixIndex<W>(zb + 1, nz);
     auto advance = [&](int z) { return W ? (z + 1 == nz ? 0 : z + 1) : min(z + 1, nz - 1); };
@@ -206,11 +251,11 @@ __device__ __forceinline__ void StarTB2Tile(const typename Plan::Args &a1, const
       constexpr int i0 = r, i1 = (r + 1) % 3, i2 = (r + 2) % 3;
       const int buf = (k + 2) & 1;
       put(&L0[buf][wy + 1][0], Q0[i1]);
-      if (halo) put(&L0[buf][hslot][0], hv);
+      puth(buf);
       put(&L1[buf][wy][0], Q1[i1]);
       // At k == ze the planes loaded below are not used (z1, z2 are in-grid
       // indices whatever k is, so the loads stay in bounds).
-      if (halo) ldu((int64_t)z1 * plane + hrow, hv);
+      ldh((int64_t)z1 * plane);
       __syncthreads();
       // Lanes past the row end compute on in-bounds LDS garbage that is
       // never published or stored.
