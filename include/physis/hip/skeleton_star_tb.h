@@ -42,7 +42,7 @@ namespace psk {
 
 struct TBGeometry {
   int tiles_y;  // y tiles, each writing BY-2 rows
-  int zchunk;
+  int zchunk;   // planes per workgroup; 0: resident workgroups (below)
   int remap;
 };
 
@@ -333,16 +333,61 @@ __device__ __forceinline__ void StarTB2Tile(const typename Plan::Args &a1, const
   }
 }
 
+// The work of a launch is the sequence of (tile, plane) positions, tile-major
+// (position tile * nz + z).  With zchunk > 0, workgroup (b, c) takes planes
+// [c * zchunk, (c + 1) * zchunk) of tile b (tiles remapped so that each XCD
+// takes a contiguous range).  With zchunk == 0 the grid holds only as many
+// workgroups as the device keeps resident, and workgroup v (again numbered
+// so that each XCD takes a contiguous range) takes the positions
+// [v * total / n, (v + 1) * total / n): every workgroup gets the same number
+// of planes (so no partial last wave of workgroups), in one or two z
+// segments of consecutive tiles, each segment re-reading three halo planes
+// instead of one per chunk.
 template <class Plan, int NWX, int BY, bool kRenamed = false, bool kMid = false>
 __global__ void __launch_bounds__(64 * NWX * BY, kRenamed ? 8 : 1)
 ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
   __shared__ StarTB2Lds<NWX, BY> L;
-  const int tile = geo.remap ? XcdTile(blockIdx.x, geo.tiles_y) : (int)blockIdx.x;
   int nx, ny, nz;
   Plan::stage_dims(a1, nx, ny, nz);
-  const int zb = (int)blockIdx.y * geo.zchunk;
-  const int ze = min(zb + geo.zchunk, nz);
-  StarTB2Tile<Plan, NWX, BY, kRenamed, kMid>(a1, a2, nx, ny, nz, tile * (BY - 2) - 1, zb, ze, L);
+  int64_t p, pe;
+  if (geo.zchunk > 0) {
+    const int tile = geo.remap ? XcdTile(blockIdx.x, geo.tiles_y) : (int)blockIdx.x;
+    const int zb = (int)blockIdx.y * geo.zchunk;
+    p = (int64_t)tile * nz + zb;
+    pe = (int64_t)tile * nz + min(zb + geo.zchunk, nz);
+  } else {
+    const int64_t total = (int64_t)geo.tiles_y * nz;
+    const int v = XcdTile(blockIdx.x, gridDim.x);
+    p = total * v / gridDim.x;
+    pe = total * (v + 1) / gridDim.x;
+  }
+  while (p < pe) {
+    const int tile = (int)(p / nz);
+    const int zb = (int)(p - (int64_t)tile * nz);
+    const int ze = (int)min<int64_t>(nz, zb + (pe - p));
+    StarTB2Tile<Plan, NWX, BY, kRenamed, kMid>(a1, a2, nx, ny, nz, tile * (BY - 2) - 1, zb, ze, L);
+    p += ze - zb;
+    // The next segment's first writes go to LDS rows this one still reads.
+    if (p < pe) __syncthreads();
+  }
+}
+
+// Workgroups of `kernel` (blocks of `threads`) the current device keeps
+// resident at once, cached per device and per Tag (one Tag per kernel).
+template <class Plan, int NWX, int BY, bool kRenamed, bool kMid>
+struct TB2KernelId {};
+template <class Tag>
+inline int ResidentWorkgroups(const void *kernel, int threads) {
+  static int cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (dev >= 0 && dev < 64 && cache[dev] > 0) return cache[dev];
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  const int n = per_cu * cus;
+  if (dev >= 0 && dev < 64) cache[dev] = n;
+  return n;
 }
 
 // Whether a grid can take the fused two-step path with NWX waves per row.
@@ -360,13 +405,27 @@ inline bool StarTB2Supported(int nx, int ny, int nz) {
 // Timed on MI355X with the rotating-queue loop (GLUPS, clamped): 128^3 298,
 // 192^3 673, 256^3 1011, 320^3 837, 384^3 1033, 448^3 1104, 512^3 1255-1314,
 // each within run-to-run noise (2-4%) of the best 8-, 16- or 32-plane chunk.
+// resident: launch only the workgroups the device keeps resident, each with
+// an equal share of the (tile, plane) positions (zchunk == 0 above), when
+// every share is at least kResidentPlanes planes (1), always (2), or with
+// the largest multiple of the tile count that fits (3); 0: the chunked
+// launch.  Chosen for grids stored in 16 bits, whose fused passes are bound
+// by latency rather than by HBM bandwidth, so that a last partial wave of
+// workgroups costs them most.  MI355X, 512^3 clamped, 200 steps (GLUPS,
+// chunks of 16 planes / resident / resident aligned to the tiles): FP32
+// 1300 / 1207 / 1213, BF16 1558 / 1628 / 1453, FP16 1598 / 1670 / 1483.
+// FP32 passes fetch more bytes resident (0.787 GB per pass against 0.744)
+// and run slower even aligned (0.648 GB, but 430 workgroups for 512 slots).
 struct TB2Shape {
   int nwx, by, zchunk;
+  int resident = 0;
 };
+constexpr int kResidentPlanes = 32;
 constexpr int kTB2Waves = 4096, kTB2WavesFull = 16384;
 
-inline TB2Shape ChooseTB2Shape(int nx, int ny, int nz) {
+inline TB2Shape ChooseTB2Shape(int nx, int ny, int nz, int elem_bytes = 4) {
   TB2Shape sh;
+  sh.resident = elem_bytes < 4 ? 1 : 0;
   sh.nwx = nx <= 256 ? 1 : (nx <= 512 ? 2 : 4);
   sh.by = sh.nwx == 4 ? 4 : 8;
   const int tiles_y = (ny + sh.by - 3) / (sh.by - 2);
@@ -385,15 +444,29 @@ inline TB2Shape ChooseTB2Shape(int nx, int ny, int nz) {
 
 template <class Plan, int NWX, int BY, bool kRenamed = false, bool kMid = false>
 inline hipError_t LaunchZMarchStarTB2(const typename Plan::Args &a1, const typename Plan::Args &a2, int ny,
-                                      int nz, int zchunk, bool remap, hipStream_t stream) {
+                                      int nz, int zchunk, bool remap, hipStream_t stream, int resident = 0) {
+  const auto kernel = ZMarchStar4TB2<Plan, NWX, BY, kRenamed, kMid>;
+  const dim3 block(64 * NWX, BY, 1);
   TBGeometry geo;
   geo.tiles_y = (ny + BY - 3) / (BY - 2);
+  geo.remap = remap ? 1 : 0;
+  if (resident) {
+    const int slots = ResidentWorkgroups<TB2KernelId<Plan, NWX, BY, kRenamed, kMid>>(
+        reinterpret_cast<const void *>(kernel), 64 * NWX * BY);
+    // resident == 3: the largest multiple of tiles_y that fits, so that
+    // every tile splits into the same z segments and the workgroups of
+    // neighbouring tiles read their shared halo rows at the same time.
+    const int n = resident == 3 && slots >= geo.tiles_y ? slots / geo.tiles_y * geo.tiles_y : slots;
+    if (n > 0 && (resident >= 2 || (int64_t)geo.tiles_y * nz >= (int64_t)kResidentPlanes * n)) {
+      geo.zchunk = 0;
+      hipLaunchKernelGGL(kernel, dim3(n, 1, 1), block, 0, stream, a1, a2, geo);
+      return hipGetLastError();
+    }
+  }
   if (zchunk <= 0 || zchunk > nz) zchunk = nz;
   geo.zchunk = zchunk;
-  geo.remap = remap ? 1 : 0;
   dim3 grid(geo.tiles_y, (nz + zchunk - 1) / zchunk, 1);
-  hipLaunchKernelGGL((ZMarchStar4TB2<Plan, NWX, BY, kRenamed, kMid>), grid, dim3(64 * NWX, BY, 1), 0, stream, a1,
-                     a2, geo);
+  hipLaunchKernelGGL(kernel, grid, block, 0, stream, a1, a2, geo);
   return hipGetLastError();
 }
 
@@ -403,11 +476,13 @@ inline hipError_t LaunchZMarchStarTB2(const typename Plan::Args &a1, const typen
 template <class Plan>
 inline hipError_t LaunchZMarchStarTB2Auto(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx,
                                           int ny, int nz, hipStream_t stream, bool mid = false) {
-  const TB2Shape sh = ChooseTB2Shape(nx, ny, nz);
+  const TB2Shape sh = ChooseTB2Shape(nx, ny, nz, (int)sizeof(typename Plan::Storage::stored_t));
 #define PSK_TB2_AUTO(NWX, BY)                                                                          \
   if (sh.nwx == NWX && StarTB2Supported<NWX>(nx, ny, nz))                                               \
-    return mid ? LaunchZMarchStarTB2<Plan, NWX, BY, true, true>(a1, a2, ny, nz, sh.zchunk, true, stream) \
-               : LaunchZMarchStarTB2<Plan, NWX, BY, true, false>(a1, a2, ny, nz, sh.zchunk, true, stream);
+    return mid ? LaunchZMarchStarTB2<Plan, NWX, BY, true, true>(a1, a2, ny, nz, sh.zchunk, true, stream,     \
+                                                                 sh.resident)                                 \
+               : LaunchZMarchStarTB2<Plan, NWX, BY, true, false>(a1, a2, ny, nz, sh.zchunk, true, stream,    \
+                                                                  sh.resident);
   PSK_TB2_AUTO(1, 8)
   PSK_TB2_AUTO(2, 8)
   PSK_TB2_AUTO(4, 4)

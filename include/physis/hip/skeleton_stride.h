@@ -216,11 +216,48 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
   const st *in = BufRow<S>::Uniform(Plan::stage_ptr(a1));
   st *out = BufRow<S>::Uniform(Plan::out_ptr(a2));
   const int64_t row = (int64_t)yf * nx;
-  const bool halo_lo = wy == 0, halo_hi = wy == BY - 1;
-  // A wave owns at most one of the two halo rows (BY >= 3).
-  const bool halo = halo_lo || halo_hi;
-  const int hslot = halo_lo ? 0 : BY + 1;
-  const int64_t hrow = (int64_t)FixIndex<W>(halo_lo ? ya - 1 : ya + BY, ny) * nx;
+  // The two halo rows (ya - 1 into slot 0, ya + BY into slot BY + 1) are
+  // loaded by all lanes of the workgroup together, kHN elements per lane of
+  // the 2 * 64 * NP row elements, without a branch (a load inside a branch
+  // makes the compiler wait for every load in flight where the paths join).
+  // Lanes past a row end, or past the rows, read an in-bounds element and
+  // publish nothing.  Elements stay in storage form until published.
+  constexpr int kW = 64 * NP, kT = 64 * BY;
+  constexpr int kHN = (2 * kW + kT - 1) / kT;
+  const int tid = lane + wy * 64;
+  int hoff[kHN], hx[kHN];
+  bool hhi[kHN], hpub[kHN];
+  {
+    const int hy_lo = FixIndex<W>(ya - 1, ny), hy_hi = FixIndex<W>(ya + BY, ny);
+#pragma unroll
+    for (int j = 0; j < kHN; ++j) {
+      const int t = tid + j * kT;
+      const int e = t % (2 * kW);
+      hhi[j] = e >= kW;
+      hx[j] = hhi[j] ? e - kW : e;
+      hpub[j] = t < 2 * kW && hx[j] < nx;
+      hoff[j] = (hhi[j] ? hy_hi : hy_lo) * nx + min(hx[j], nx - 1);
+    }
+  }
+  st hraw[kHN];
+  auto ldh = [&](int64_t zo) {
+#pragma unroll
+    for (int j = 0; j < kHN; ++j) hraw[j] = GlobalLoad(in + zo + hoff[j]);
+  };
+  // Clamped plans never read the pads; wrap plans read the wrapped row ends.
+  auto puth = [&](float (*R0)[StrideTB2Lds<NP, BY>::kRow]) {
+#pragma unroll
+    for (int j = 0; j < kHN; ++j) {
+      if (!hpub[j]) continue;
+      float *r = R0[hhi[j] ? BY + 1 : 0];
+      const float v = S::widen(hraw[j]);
+      r[hx[j] + 4] = v;
+      if (W) {
+        if (hx[j] == nx - 1) r[3] = v;
+        if (hx[j] == 0) r[nx + 4] = v;
+      }
+    }
+  };
   const bool out_row = wy >= 1 && wy <= BY - 2 && y >= 0 && y < ny;
   auto zo = [&](int z) { return (int64_t)FixIndex<W>(z, nz) * plane; };
   // Every lane loads without a branch (points past the row end read 0
@@ -268,11 +305,17 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
   // r = (k - zb + 1) % 3, u0 planes k-1, k, k+1 are Q0[r], Q0[r+1], Q0[r+2]
   // and u1 planes k-2, k-1 are Q1[r], Q1[r+1]; u1 plane k goes to Q1[r+2]
   // and u0 plane k+2 is loaded into Q0[r] once u1 no longer needs it.
-  float Q0[3][NP], Q1[3][NP] = {}, hv[NP] = {};
+  // Issued in the order the loop leaves its loads in flight (the halo plane
+  // before the newest u0 plane; the scheduling barriers keep the order), so
+  // that the waits at the loop head are the loop's own.
+  float Q0[3][NP], Q1[3][NP] = {};
   ld(in + zo(zb - 2) + row, Q0[0]);
   ld(in + zo(zb - 1) + row, Q0[1]);
+  __builtin_amdgcn_sched_barrier(0);
+  ldh(zo(zb - 1));
+  __builtin_amdgcn_sched_barrier(0);
   ld(in + zo(zb) + row, Q0[2]);
-  if (halo) ld(in + zo(zb - 1) + hrow, hv);
+  __builtin_amdgcn_sched_barrier(0);
   // Wrapped/clamped indices of planes k, k+1, k+2.
   int z0 = FixIndex<W>(zb - 1, nz), z1 = FixIndex<W>(zb, nz), z2 = FixIndex<W>(zb + 1, nz);
   auto step = [&](int k, auto rc) {
@@ -282,10 +325,10 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
     auto &R0 = L.u0[buf];
     auto &R1 = L.u1[buf];
     StridePut<W, NP>(R0[wy + 1], Q0[i1], lane, nx);
-    if (halo) StridePut<W, NP>(R0[hslot], hv, lane, nx);
+    puth(R0);
     StridePut<W, NP>(R1[wy], Q1[i1], lane, nx);
     // z1 is an in-grid index for every k: at k == ze the load is unused.
-    if (halo) ld(in + (int64_t)z1 * plane + hrow, hv);
+    ldh((int64_t)z1 * plane);
     __syncthreads();
     level1(R0, z0, Q0[i0], Q0[i1], Q0[i2], Q1[i2]);
     if constexpr (kMid) {

@@ -167,9 +167,23 @@ struct TBArgs {
 // first use.  The register queues hold Q = T + D planes and the march is
 // unrolled Q times, so a plane's registers never move (a register copy of a
 // load in flight would wait for it).
+// Which prefetch loads are issued without a branch (bit 0: level-0 planes,
+// bit 1: coefficients) and whether the shell values of intermediate levels
+// are prefetched with the coefficients (bit 2) instead of loaded where a
+// boundary point needs them.  Branch-free loads keep the compiler's waits
+// from cancelling the prefetch distance, but let it hoist every load of an
+// unrolled period, which costs registers.  MI355X, ms per sweep, 257x129x129
+// (385^3), shapes degree / rows / z-chunk: BF16 2/8/16 0.0412 -> 0.0294
+// with all three, 2/8/32 (385^3) 0.388 -> 0.308; BF16 3/16/32 0.0351 ->
+// 0.0343 with bits 0 and 2 (all three spill registers: 0.073); FP32
+// 2/8/32 0.0320 -> 0.0304 with bits 0 and 2 (0.0371 with all three).
+template <class S, int T>
+constexpr int kTBBranchFree = sizeof(typename S::stored_t) == 2 && T == 2 ? 7 : 5;
+
 template <class S, int T, int NW, int R, int D>
 __global__ void __launch_bounds__(64 * NW)
 HimenoTB(TBArgs a) {
+  constexpr int kMode = kTBBranchFree<S, T>;
   using st = typename S::stored_t;
   constexpr int BY = NW * R;
   constexpr int TW = 66, TH = BY + 2, PLANE = TW * TH, NT = 64 * NW;
@@ -221,40 +235,75 @@ HimenoTB(TBArgs a) {
     goff[j] = inb ? (uint32_t)(gy * nx + gx) * (uint32_t)sizeof(st) : 0u;
     gkind[j] = !inb ? 0 : (gx >= 1 && gx < nx - 1 && gy >= 1 && gy < ny - 1) ? 2 : 1;
   }
-  // Queues: plane z lives in slot (z - s0) mod Q.
-  float pre[Q][PER];
+  // A branch-free prefetch (kMode) loads from an in-bounds address (an
+  // element outside the grid or the march reads the grid's first element);
+  // what it loads for such an element is masked where it is published, or
+  // never used.  The coefficient queue holds widened values (16-bit queues
+  // cost more registers than they save); level-0 planes stay in storage
+  // form until published.  The planes and values of march step s live in
+  // slot (s - s0) mod Q.
+  st pre[Q][PER];
   float cq[Q][R][NCOEF];
-  auto load0 = [&](float (&dst)[PER], int z) {
+  // Shell values of the intermediate levels (t = 1 .. T-1) a step needs, at
+  // the grid's boundary points; slot of step s as above.
+  float shq[Q][T > 1 ? T - 1 : 1][R];
+  auto load0 = [&](st (&dst)[PER], int z) {
     const bool zin = z >= 0 && z < nz, zint = z >= 1 && z < nz - 1;
     const uint32_t zo = (uint32_t)(zin ? z : 0) * plane;
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      float v = 0.f;
-      if (zin && gkind[j] != 0) v = ld(gkind[j] == 2 && zint ? in : shell0, zo + goff[j]);
-      dst[j] = v;
+      const char *base = gkind[j] == 2 && zint ? in : shell0;
+      if constexpr (kMode & 1) {
+        dst[j] = psk::GlobalLoad(reinterpret_cast<const st *>(base + (gkind[j] != 0 ? zo + goff[j] : 0u)));
+      } else {
+        st v = 0;
+        if (zin && gkind[j] != 0) v = psk::GlobalLoad(reinterpret_cast<const st *>(base + zo + goff[j]));
+        dst[j] = v;
+      }
     }
   };
-  auto store0 = [&](const float (&src)[PER], int z) {
+  auto store0 = [&](const st (&src)[PER], int z) {
+    const bool zin = z >= 0 && z < nz;
     float *rg = ring[0][(z + 8) & 3];
 #pragma unroll
     for (int j = 0; j < PER; ++j)
-      if (PLANE % NT == 0 || tid + j * NT < PLANE) rg[tid + j * NT] = src[j];
+      if (PLANE % NT == 0 || tid + j * NT < PLANE) rg[tid + j * NT] = zin && gkind[j] != 0 ? S::widen(src[j]) : 0.f;
   };
+  // Coefficients of plane z, used only at interior points of interior
+  // planes.
   auto loadc = [&](float (&dst)[R][NCOEF], int z) {
     const bool zok = z >= 1 && z < nz - 1;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const bool ok = col_int[r] && zok;
-      const uint32_t o = col[r] + (uint32_t)(ok ? z : 0) * plane;
+      const uint32_t o = col_int[r] && zok ? col[r] + (uint32_t)z * plane : 0u;
 #pragma unroll
-      for (int n = 0; n < NCOEF; ++n) dst[r][n] = ok ? ld(static_cast<const char *>(a.c[n]), o) : 0.f;
+      for (int n = 0; n < NCOEF; ++n) {
+        if constexpr (kMode & 2)
+          dst[r][n] = S::load(reinterpret_cast<const st *>(static_cast<const char *>(a.c[n]) + o));
+        else
+          dst[r][n] = col_int[r] && zok ? S::load(reinterpret_cast<const st *>(static_cast<const char *>(a.c[n]) + o)) : 0.f;
+      }
+    }
+  };
+  // Shell values level t reads at step s (plane s - t + 1), for t < T.
+  auto loadsh = [&](float (&dst)[T > 1 ? T - 1 : 1][R], int s) {
+#pragma unroll
+    for (int t = 1; t < T; ++t) {
+      const int z = s - t + 1;
+      const bool zin = z >= 0 && z < nz, zint = z >= 1 && z < nz - 1;
+      const char *sh = static_cast<const char *>(a.shell[(a.k0 + t) & 1]);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const uint32_t o = zin && col_in[r] && !(col_int[r] && zint) ? col[r] + (uint32_t)z * plane : 0u;
+        dst[t - 1][r] = S::load(reinterpret_cast<const st *>(sh + o));
+      }
     }
   };
 
   const int s0 = zb - (T - 1);        // first march step (level 1 at plane s0)
   const int send = ze + T - 1;        // one past the last step
   {
-    float tmp[PER];
+    st tmp[PER];
     load0(tmp, s0 - 1);
     store0(tmp, s0 - 1);
     load0(tmp, s0);
@@ -264,6 +313,7 @@ HimenoTB(TBArgs a) {
     constexpr int d = decltype(dc)::value;
     load0(pre[(d + 1) % Q], s0 + 1 + d);  // planes s0+1 .. s0+D
     loadc(cq[d % Q], s0 + d);             // planes s0 .. s0+D-1
+    if constexpr (T > 1 && (kMode & 4)) loadsh(shq[d % Q], s0 + d);
   }, std::make_integer_sequence<int, D>{});
 
   double acc[T];
@@ -283,8 +333,10 @@ HimenoTB(TBArgs a) {
     constexpr int J = decltype(jc)::value;
     store0(pre[(J + 1) % Q], s + 1);
     __syncthreads();
-    if (s + 1 + D < ze + T) load0(pre[(J + 1 + D) % Q], s + 1 + D);
+    // Past the last step the planes loaded are not used.
+    load0(pre[(J + 1 + D) % Q], s + 1 + D);
     loadc(cq[(J + D) % Q], s + D);
+    if constexpr (T > 1 && (kMode & 4)) loadsh(shq[(J + D) % Q], s + D);
     psk::StaticFor([&](auto tc) {
       constexpr int t = decltype(tc)::value + 1;
       const int z = s - t + 1;
@@ -311,8 +363,11 @@ HimenoTB(TBArgs a) {
             v[r] = HimenoUpdate<sizeof(st) == 2>([&](int n) { return g[n]; }, P, a.omega, &ss);
             if (owned[r] && z >= zb && z < ze) acc[t - 1] += (double)ss * (double)ss;
             if (t == T && owned[r]) S::store(reinterpret_cast<st *>(out + ((uint32_t)z * plane + col[r])), v[r]);
-          } else if (t < T) {
-            v[r] = ld(static_cast<const char *>(a.shell[(a.k0 + t) & 1]), (uint32_t)z * plane + col[r]);
+          } else if constexpr (t < T) {
+            if constexpr (kMode & 4)
+              v[r] = shq[J][t - 1][r];
+            else
+              v[r] = ld(static_cast<const char *>(a.shell[(a.k0 + t) & 1]), (uint32_t)z * plane + col[r]);
           }
         }
       }
@@ -328,12 +383,15 @@ HimenoTB(TBArgs a) {
     }, std::make_integer_sequence<int, T>{});
   };
 
-  for (int s = s0; s < send; s += Q) {
-    psk::StaticFor([&](auto jc) {
-      constexpr int J = decltype(jc)::value;
-      if (s + J < send) step(jc, s + J);
-    }, std::make_integer_sequence<int, Q>{});
+  // Whole periods of Q steps without a guard, then the remaining steps.
+  int s = s0;
+  for (; s + Q <= send; s += Q) {
+    psk::StaticFor([&](auto jc) { step(jc, s + decltype(jc)::value); }, std::make_integer_sequence<int, Q>{});
   }
+  psk::StaticFor([&](auto jc) {
+    constexpr int J = decltype(jc)::value;
+    if (s + J < send) step(jc, s + J);
+  }, std::make_integer_sequence<int, Q>{});
 
   const int blocks = gridDim.x * gridDim.y;
   const int bid = blockIdx.x + blockIdx.y * gridDim.x;
