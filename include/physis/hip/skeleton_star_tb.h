@@ -40,10 +40,12 @@
 
 namespace psk {
 
+constexpr int kMaxZChunks = 64;
 struct TBGeometry {
   int tiles_y;  // y tiles, each writing BY-2 rows
-  int zchunk;   // planes per workgroup; 0: resident workgroups (below)
+  int zchunk;   // planes per workgroup; 0: resident workgroups, -1: zs (below)
   int remap;
+  int zs[kMaxZChunks + 1];  // zchunk == -1: chunk c is planes [zs[c], zs[c+1])
 };
 
 // Shared memory of one fused tile: u0 rows (with the two halo rows) and u1
@@ -342,7 +344,8 @@ __device__ __forceinline__ void StarTB2Tile(const typename Plan::Args &a1, const
 // [v * total / n, (v + 1) * total / n): every workgroup gets the same number
 // of planes (so no partial last wave of workgroups), in one or two z
 // segments of consecutive tiles, each segment re-reading three halo planes
-// instead of one per chunk.
+// instead of one per chunk.  With zchunk == -1 the chunks are zs's (a guided
+// schedule: long chunks dispatched first, short ones last).
 template <class Plan, int NWX, int BY, bool kRenamed = false, bool kMid = false>
 __global__ void __launch_bounds__(64 * NWX * BY, kRenamed ? 8 : 1)
 ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
@@ -350,11 +353,13 @@ ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
   int nx, ny, nz;
   Plan::stage_dims(a1, nx, ny, nz);
   int64_t p, pe;
-  if (geo.zchunk > 0) {
+  if (geo.zchunk != 0) {
     const int tile = geo.remap ? XcdTile(blockIdx.x, geo.tiles_y) : (int)blockIdx.x;
-    const int zb = (int)blockIdx.y * geo.zchunk;
+    const int c = (int)blockIdx.y;
+    const int zb = geo.zchunk > 0 ? c * geo.zchunk : geo.zs[c];
+    const int ze = geo.zchunk > 0 ? min(zb + geo.zchunk, nz) : geo.zs[c + 1];
     p = (int64_t)tile * nz + zb;
-    pe = (int64_t)tile * nz + min(zb + geo.zchunk, nz);
+    pe = (int64_t)tile * nz + ze;
   } else {
     const int64_t total = (int64_t)geo.tiles_y * nz;
     const int v = XcdTile(blockIdx.x, gridDim.x);
@@ -405,27 +410,49 @@ inline bool StarTB2Supported(int nx, int ny, int nz) {
 // Timed on MI355X with the rotating-queue loop (GLUPS, clamped): 128^3 298,
 // 192^3 673, 256^3 1011, 320^3 837, 384^3 1033, 448^3 1104, 512^3 1255-1314,
 // each within run-to-run noise (2-4%) of the best 8-, 16- or 32-plane chunk.
-// resident: launch only the workgroups the device keeps resident, each with
-// an equal share of the (tile, plane) positions (zchunk == 0 above), when
-// every share is at least kResidentPlanes planes (1), always (2), or with
-// the largest multiple of the tile count that fits (3); 0: the chunked
-// launch.  Chosen for grids stored in 16 bits, whose fused passes are bound
-// by latency rather than by HBM bandwidth, so that a last partial wave of
-// workgroups costs them most.  MI355X, 512^3 clamped, 200 steps (GLUPS,
-// chunks of 16 planes / resident / resident aligned to the tiles): FP32
-// 1300 / 1207 / 1213, BF16 1558 / 1628 / 1453, FP16 1598 / 1670 / 1483.
-// FP32 passes fetch more bytes resident (0.787 GB per pass against 0.744)
-// and run slower even aligned (0.648 GB, but 430 workgroups for 512 slots).
+// Grids large enough for every resident workgroup to get kResidentPlanes
+// planes take the guided schedule instead (kGuidedIfLong).
+//
+// How the fused kernel's workgroups split the z-march (TB2Shape::sched):
+//   kChunked          chunks of zchunk planes per tile;
+//   kResident(IfLong) only the workgroups the device keeps resident, each
+//                     with an equal share of the (tile, plane) positions
+//                     (zchunk == 0 above), IfLong: when every share is at
+//                     least kResidentPlanes planes, else chunked;
+//   kResidentAligned  as kResident with the largest multiple of the tile
+//                     count that fits, so tiles split into the same z
+//                     segments;
+//   kGuided(2)        chunks that shrink as the march is dispatched: each
+//                     the larger of zchunk planes (8 if 0) and the remaining
+//                     planes' share of one (two) resident waves of
+//                     workgroups, so the last workgroups are short while most
+//                     planes are in long chunks that re-read few halo planes;
+//   kGuidedIfLong     kGuided when every resident share would be at least
+//                     kResidentPlanes planes, else chunked.
+// MI355X, 512^3 clamped, 200 steps, GLUPS (chunks of 16 / 32 planes /
+// resident / resident aligned / guided / guided2): FP32 1299 / 1290 / 1207 /
+// 1213 / 1332 / 1294; BF16 1556 / 1527 / 1628 / 1453 / 1647 / 1553.  FP32
+// passes fetch more bytes resident (0.787 GB per pass against 0.744 with
+// 16-plane chunks) and run slower even aligned (0.648 GB, but 430 workgroups
+// for 512 slots).
+enum TB2Sched : int {
+  kChunked = 0,
+  kResidentIfLong = 1,
+  kResident = 2,
+  kResidentAligned = 3,
+  kGuided = 4,
+  kGuided2 = 5,
+  kGuidedIfLong = 6,
+};
 struct TB2Shape {
   int nwx, by, zchunk;
-  int resident = 0;
+  int sched = kGuidedIfLong;
 };
 constexpr int kResidentPlanes = 32;
 constexpr int kTB2Waves = 4096, kTB2WavesFull = 16384;
 
-inline TB2Shape ChooseTB2Shape(int nx, int ny, int nz, int elem_bytes = 4) {
+inline TB2Shape ChooseTB2Shape(int nx, int ny, int nz) {
   TB2Shape sh;
-  sh.resident = elem_bytes < 4 ? 1 : 0;
   sh.nwx = nx <= 256 ? 1 : (nx <= 512 ? 2 : 4);
   sh.by = sh.nwx == 4 ? 4 : 8;
   const int tiles_y = (ny + sh.by - 3) / (sh.by - 2);
@@ -444,20 +471,34 @@ inline TB2Shape ChooseTB2Shape(int nx, int ny, int nz, int elem_bytes = 4) {
 
 template <class Plan, int NWX, int BY, bool kRenamed = false, bool kMid = false>
 inline hipError_t LaunchZMarchStarTB2(const typename Plan::Args &a1, const typename Plan::Args &a2, int ny,
-                                      int nz, int zchunk, bool remap, hipStream_t stream, int resident = 0) {
+                                      int nz, int zchunk, bool remap, hipStream_t stream, int sched = kChunked) {
   const auto kernel = ZMarchStar4TB2<Plan, NWX, BY, kRenamed, kMid>;
   const dim3 block(64 * NWX, BY, 1);
   TBGeometry geo;
   geo.tiles_y = (ny + BY - 3) / (BY - 2);
   geo.remap = remap ? 1 : 0;
-  if (resident) {
+  if (sched != kChunked) {
     const int slots = ResidentWorkgroups<TB2KernelId<Plan, NWX, BY, kRenamed, kMid>>(
         reinterpret_cast<const void *>(kernel), 64 * NWX * BY);
-    // resident == 3: the largest multiple of tiles_y that fits, so that
-    // every tile splits into the same z segments and the workgroups of
-    // neighbouring tiles read their shared halo rows at the same time.
-    const int n = resident == 3 && slots >= geo.tiles_y ? slots / geo.tiles_y * geo.tiles_y : slots;
-    if (n > 0 && (resident >= 2 || (int64_t)geo.tiles_y * nz >= (int64_t)kResidentPlanes * n)) {
+    const bool is_long = slots > 0 && (int64_t)geo.tiles_y * nz >= (int64_t)kResidentPlanes * slots;
+    if (slots > 0 && (sched == kGuided || sched == kGuided2 || (sched == kGuidedIfLong && is_long))) {
+      const int k = sched == kGuided2 ? 2 : 1;
+      const int64_t lo = zchunk > 0 && sched != kGuidedIfLong ? zchunk : 8;
+      int n = 0, z = 0;
+      while (z < nz && n < kMaxZChunks) {
+        const int64_t share = ((int64_t)(nz - z) * geo.tiles_y + (int64_t)k * slots - 1) / ((int64_t)k * slots);
+        int c = (int)(share > lo ? share : lo);
+        if (n == kMaxZChunks - 1 || c > nz - z) c = nz - z;
+        geo.zs[n++] = z;
+        z += c;
+      }
+      geo.zs[n] = nz;
+      geo.zchunk = -1;
+      hipLaunchKernelGGL(kernel, dim3(geo.tiles_y, n, 1), block, 0, stream, a1, a2, geo);
+      return hipGetLastError();
+    }
+    const int n = sched == kResidentAligned && slots >= geo.tiles_y ? slots / geo.tiles_y * geo.tiles_y : slots;
+    if (n > 0 && (sched == kResident || sched == kResidentAligned || (sched == kResidentIfLong && is_long))) {
       geo.zchunk = 0;
       hipLaunchKernelGGL(kernel, dim3(n, 1, 1), block, 0, stream, a1, a2, geo);
       return hipGetLastError();
@@ -476,13 +517,13 @@ inline hipError_t LaunchZMarchStarTB2(const typename Plan::Args &a1, const typen
 template <class Plan>
 inline hipError_t LaunchZMarchStarTB2Auto(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx,
                                           int ny, int nz, hipStream_t stream, bool mid = false) {
-  const TB2Shape sh = ChooseTB2Shape(nx, ny, nz, (int)sizeof(typename Plan::Storage::stored_t));
+  const TB2Shape sh = ChooseTB2Shape(nx, ny, nz);
 #define PSK_TB2_AUTO(NWX, BY)                                                                          \
   if (sh.nwx == NWX && StarTB2Supported<NWX>(nx, ny, nz))                                               \
     return mid ? LaunchZMarchStarTB2<Plan, NWX, BY, true, true>(a1, a2, ny, nz, sh.zchunk, true, stream,     \
-                                                                 sh.resident)                                 \
+                                                                 sh.sched)                                 \
                : LaunchZMarchStarTB2<Plan, NWX, BY, true, false>(a1, a2, ny, nz, sh.zchunk, true, stream,    \
-                                                                  sh.resident);
+                                                                  sh.sched);
   PSK_TB2_AUTO(1, 8)
   PSK_TB2_AUTO(2, 8)
   PSK_TB2_AUTO(4, 4)

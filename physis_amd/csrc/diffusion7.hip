@@ -122,7 +122,7 @@ struct Variant {
   bool remap;
   int rows = 1;  // family 1: tile rows per thread
   bool renamed = false;  // family 4: the loop schedule without queue moves
-  int resident = 0;  // family 4: resident workgroups with equal shares (psk::TB2Shape)
+  int sched = psk::kChunked;  // family 4: how workgroups split the z-march (psk::TB2Sched)
 };
 
 const Variant kVariants[] = {
@@ -168,10 +168,13 @@ const Variant kVariants[] = {
     {"tb2r_w2_by8_z8", 4, 2, 8, 8, true, 1, true},
     {"tb2r_w1_by8_z16", 4, 1, 8, 16, true, 1, true},
     {"tb2r_w1_by8_z8", 4, 1, 8, 8, true, 1, true},
-    {"tb2p_w2_by8", 4, 2, 8, 32, true, 1, true, 2},
-    {"tb2p_w1_by8", 4, 1, 8, 32, true, 1, true, 2},
-    {"tb2p_w1_by16", 4, 1, 16, 32, true, 1, true, 2},
-    {"tb2a_w2_by8", 4, 2, 8, 32, true, 1, true, 3},
+    {"tb2p_w2_by8", 4, 2, 8, 32, true, 1, true, psk::kResident},
+    {"tb2p_w1_by8", 4, 1, 8, 32, true, 1, true, psk::kResident},
+    {"tb2a_w2_by8", 4, 2, 8, 32, true, 1, true, psk::kResidentAligned},
+    {"tb2g_w2_by8_m8", 4, 2, 8, 8, true, 1, true, psk::kGuided},
+    {"tb2g_w2_by8_m16", 4, 2, 8, 16, true, 1, true, psk::kGuided},
+    {"tb2g_w1_by8_m8", 4, 1, 8, 8, true, 1, true, psk::kGuided},
+    {"tb2g2_w2_by8_m8", 4, 2, 8, 8, true, 1, true, psk::kGuided2},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 // Measured on MI355X at 512^3 FP32 (bench.py --sweep): star4_by8_z128 ~710
@@ -201,9 +204,9 @@ hipError_t LaunchPair(const Variant &v, const void *src, void *dst, void *mid, c
   if (v.renamed && v.bx == NWX && v.by == BY) {                                                            \
     if (!psk::StarTB2Supported<NWX>(dim[0], dim[1], dim[2])) return hipErrorNotSupported;                \
     return mid ? psk::LaunchZMarchStarTB2<P, NWX, BY, true, true>(a1, a2, dim[1], dim[2], v.zchunk, v.remap, s,  \
-                                                                  v.resident)                                  \
+                                                                  v.sched)                                  \
                : psk::LaunchZMarchStarTB2<P, NWX, BY, true, false>(a1, a2, dim[1], dim[2], v.zchunk, v.remap, s, \
-                                                                   v.resident);                                \
+                                                                   v.sched);                                \
   }
   PSH_TBR(2, 8)
   PSH_TBR(1, 8)
@@ -390,9 +393,8 @@ int PSHipDiffusion7Run(__PSGrid *g1, __PSGrid *g2, const __PSDomain *dom, const 
   if (variant < 0 || variant > kNumVariants) return PS_HIP_ERR_INVALID;
   // Variant 0: two fused steps per pass with the shape psk::ChooseTB2Shape
   // picks for this grid.
-  const psk::TB2Shape sh =
-      psk::ChooseTB2Shape(g1->dim[0], g1->dim[1], g1->dim[2], Rec(g1)->storage == PS_HIP_STORAGE_NATIVE ? 4 : 2);
-  const Variant autov{"tb2_auto", 4, sh.nwx, sh.by, sh.zchunk, true, 1, true, sh.resident};
+  const psk::TB2Shape sh = psk::ChooseTB2Shape(g1->dim[0], g1->dim[1], g1->dim[2]);
+  const Variant autov{"tb2_auto", 4, sh.nwx, sh.by, sh.zchunk, true, 1, true, sh.sched};
   const Variant &v = variant == 0 ? autov : kVariants[variant - 1];
   Coef7 k{coef[0], coef[1], coef[2], coef[3], coef[4], coef[5], coef[6]};
   psk::Box box = psk::BoxFromDomain(*dom, 3);
