@@ -34,6 +34,7 @@
 namespace psk {
 
 typedef float v4f_t __attribute__((ext_vector_type(4)));
+typedef uint32_t v2u_t __attribute__((ext_vector_type(2)));
 
 // Four consecutive elements from HBM as FP32.
 template <class S>
@@ -42,13 +43,28 @@ __device__ __forceinline__ void Load4(const typename S::stored_t *p, float (&v)[
     const v4f_t q = GlobalLoad(reinterpret_cast<const v4f_t *>(p));
     v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
   } else {
-    typedef uint32_t v2u_t __attribute__((ext_vector_type(2)));
     const v2u_t q = GlobalLoad(reinterpret_cast<const v2u_t *>(p));
     v[0] = S::widen((uint16_t)(q.x & 0xffffu));
     v[1] = S::widen((uint16_t)(q.x >> 16));
     v[2] = S::widen((uint16_t)(q.y & 0xffffu));
     v[3] = S::widen((uint16_t)(q.y >> 16));
   }
+}
+
+// Four consecutive 16-bit elements as loaded (two packed pairs), and their
+// widening: lets a loop carry a prefetched plane in storage form and widen it
+// where it is first used, so no wait for the load falls at the loop's end.
+template <class S>
+__device__ __forceinline__ v2u_t LoadRaw4(const typename S::stored_t *p) {
+  static_assert(sizeof(typename S::stored_t) == 2, "16-bit storage");
+  return GlobalLoad(reinterpret_cast<const v2u_t *>(p));
+}
+template <class S>
+__device__ __forceinline__ void Widen4(const v2u_t q, float (&v)[4]) {
+  v[0] = S::widen_lo(q.x);
+  v[1] = S::widen_hi(q.x);
+  v[2] = S::widen_lo(q.y);
+  v[3] = S::widen_hi(q.y);
 }
 
 // Four consecutive elements to HBM (streaming store).
@@ -58,10 +74,9 @@ __device__ __forceinline__ void Store4(typename S::stored_t *p, const float (&v)
     v4f_t q = {v[0], v[1], v[2], v[3]};
     GlobalStoreNT(reinterpret_cast<v4f_t *>(p), q);
   } else {
-    typedef uint32_t v2u_t __attribute__((ext_vector_type(2)));
     v2u_t q;
-    q.x = (uint32_t)S::narrow(v[0]) | ((uint32_t)S::narrow(v[1]) << 16);
-    q.y = (uint32_t)S::narrow(v[2]) | ((uint32_t)S::narrow(v[3]) << 16);
+    q.x = S::narrow2(v[0], v[1]);
+    q.y = S::narrow2(v[2], v[3]);
     GlobalStoreNT(reinterpret_cast<v2u_t *>(p), q);
   }
 }
@@ -73,6 +88,20 @@ template <class S>
 __device__ __forceinline__ float RoundTrip(float v) {
   if constexpr (sizeof(typename S::stored_t) == 4) return v;
   else return S::widen(S::narrow(v));
+}
+
+// RoundTrip of N values, converted in pairs (one instruction per pair).
+template <class S, int N>
+__device__ __forceinline__ void RoundTripN(float (&v)[N]) {
+  if constexpr (sizeof(typename S::stored_t) == 2) {
+#pragma unroll
+    for (int j = 0; j + 1 < N; j += 2) {
+      const uint32_t r = S::narrow2(v[j], v[j + 1]);
+      v[j] = S::widen_lo(r);
+      v[j + 1] = S::widen_hi(r);
+    }
+    if constexpr (N % 2 == 1) v[N - 1] = RoundTrip<S>(v[N - 1]);
+  }
 }
 
 // The seven taps of one point; at() is resolved at compile time once the

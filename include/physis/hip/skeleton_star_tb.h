@@ -43,7 +43,7 @@ namespace psk {
 constexpr int kMaxZChunks = 64;
 struct TBGeometry {
   int tiles_y;  // y tiles, each writing BY-2 rows
-  int zchunk;   // planes per workgroup; 0: resident workgroups, -1: zs (below)
+  int zchunk;   // planes per workgroup; -1: the chunks zs (below)
   int remap;
   int zs[kMaxZChunks + 1];  // zchunk == -1: chunk c is planes [zs[c], zs[c+1])
 };
@@ -152,8 +152,9 @@ __device__ __forceinline__ void StarTB2Tile(const typename Plan::Args &a1, const
       q.s = sv[j];
       q.b = b[j];
       q.t = t[j];
-      u[j] = RoundTrip<S>(ComputeAt<Plan, true, true, true>(q, a1, xs + j, yf, kf));
+      u[j] = ComputeAt<Plan, true, true, true>(q, a1, xs + j, yf, kf);
     }
+    RoundTripN<S>(u);
   };
   // One u2 output row at plane k - 1 from u1 planes b, c, t.  Clamped plans:
   // at a y face the row's own u1 values stand in for the missing neighbour
@@ -232,6 +233,18 @@ __device__ __forceinline__ void StarTB2Tile(const typename Plan::Args &a1, const
       }
     };
     float Q0[3][4], Q1[3][4] = {};
+    // 16-bit grids carry the newest u0 plane in storage form (praw) and
+    // widen it into its queue slot after the next barrier, where it is first
+    // used: a widened value carried around the loop would make the compiler
+    // wait for its load at the loop's end.
+    constexpr bool kRaw = sizeof(st) == 2;
+    v2u_t praw = {};
+    auto ldnext = [&](int64_t off, float (&v)[4]) {
+      if constexpr (kRaw)
+        praw = LoadRaw4<S>(in + off);
+      else
+        ldu(off, v);
+    };
     // Issued in the order the loop leaves its loads in flight (the halo
     // plane before the newest u0 plane; the scheduling barriers keep the
     // order), so that the waits at the loop head are the loop's own.
@@ -240,7 +253,7 @@ __device__ __forceinline__ void StarTB2Tile(const typename Plan::Args &a1, const
     __builtin_amdgcn_sched_barrier(0);
     ldh(zoff(zb - 1));
     __builtin_amdgcn_sched_barrier(0);
-    ldu(zoff(zb) + lrow, Q0[2]);
+    ldnext(zoff(zb) + lrow, Q0[2]);
     __builtin_amdgcn_sched_barrier(0);
     // Plane indices of k (z0), k + 1 (z1) and k + 2 (z2), as below.
     int z0 = FixIndex<W>(zb - 1, nz), z1 = FixIndex<W>(zb, nz), z2 = FixIndex<W>(zb + 1, nz);
@@ -259,6 +272,7 @@ __device__ __forceinline__ void StarTB2Tile(const typename Plan::Args &a1, const
       // indices whatever k is, so the loads stay in bounds).
       ldh((int64_t)z1 * plane);
       __syncthreads();
+      if constexpr (kRaw) Widen4<S>(praw, Q0[i2]);
       // Lanes past the row end compute on in-bounds LDS garbage that is
       // never published or stored.
       level1(buf, z0, Q0[i0], Q0[i1], Q0[i2], Q1[i2]);
@@ -269,7 +283,7 @@ __device__ __forceinline__ void StarTB2Tile(const typename Plan::Args &a1, const
           Store4<S>(Plan::out_ptr(a1) + (int64_t)k * plane + (int64_t)y * nx + xs, Q1[i2]);
       }
       // Q0[i0] (plane k-1) is dead: it receives plane k+2.
-      ldu((int64_t)z2 * plane + lrow, Q0[i0]);
+      ldnext((int64_t)z2 * plane + lrow, Q0[i0]);
       if (out_row && k - 1 >= zb) level2(buf, k, Q1[i0], Q1[i1], Q1[i2]);
       z0 = z1;
       z1 = z2;
@@ -335,46 +349,21 @@ __device__ __forceinline__ void StarTB2Tile(const typename Plan::Args &a1, const
   }
 }
 
-// The work of a launch is the sequence of (tile, plane) positions, tile-major
-// (position tile * nz + z).  With zchunk > 0, workgroup (b, c) takes planes
-// [c * zchunk, (c + 1) * zchunk) of tile b (tiles remapped so that each XCD
-// takes a contiguous range).  With zchunk == 0 the grid holds only as many
-// workgroups as the device keeps resident, and workgroup v (again numbered
-// so that each XCD takes a contiguous range) takes the positions
-// [v * total / n, (v + 1) * total / n): every workgroup gets the same number
-// of planes (so no partial last wave of workgroups), in one or two z
-// segments of consecutive tiles, each segment re-reading three halo planes
-// instead of one per chunk.  With zchunk == -1 the chunks are zs's (a guided
-// schedule: long chunks dispatched first, short ones last).
+// Workgroup (b, c) takes z chunk c of y tile b (tiles remapped so that each
+// XCD takes a contiguous range): planes [c * zchunk, (c + 1) * zchunk) with
+// zchunk > 0, else [zs[c], zs[c + 1]) (a guided schedule: long chunks
+// dispatched first, short ones last).
 template <class Plan, int NWX, int BY, bool kRenamed = false, bool kMid = false>
 __global__ void __launch_bounds__(64 * NWX * BY, kRenamed ? 8 : 1)
 ZMarchStar4TB2(typename Plan::Args a1, typename Plan::Args a2, TBGeometry geo) {
   __shared__ StarTB2Lds<NWX, BY> L;
   int nx, ny, nz;
   Plan::stage_dims(a1, nx, ny, nz);
-  int64_t p, pe;
-  if (geo.zchunk != 0) {
-    const int tile = geo.remap ? XcdTile(blockIdx.x, geo.tiles_y) : (int)blockIdx.x;
-    const int c = (int)blockIdx.y;
-    const int zb = geo.zchunk > 0 ? c * geo.zchunk : geo.zs[c];
-    const int ze = geo.zchunk > 0 ? min(zb + geo.zchunk, nz) : geo.zs[c + 1];
-    p = (int64_t)tile * nz + zb;
-    pe = (int64_t)tile * nz + ze;
-  } else {
-    const int64_t total = (int64_t)geo.tiles_y * nz;
-    const int v = XcdTile(blockIdx.x, gridDim.x);
-    p = total * v / gridDim.x;
-    pe = total * (v + 1) / gridDim.x;
-  }
-  while (p < pe) {
-    const int tile = (int)(p / nz);
-    const int zb = (int)(p - (int64_t)tile * nz);
-    const int ze = (int)min<int64_t>(nz, zb + (pe - p));
-    StarTB2Tile<Plan, NWX, BY, kRenamed, kMid>(a1, a2, nx, ny, nz, tile * (BY - 2) - 1, zb, ze, L);
-    p += ze - zb;
-    // The next segment's first writes go to LDS rows this one still reads.
-    if (p < pe) __syncthreads();
-  }
+  const int tile = geo.remap ? XcdTile(blockIdx.x, geo.tiles_y) : (int)blockIdx.x;
+  const int c = (int)blockIdx.y;
+  const int zb = geo.zchunk > 0 ? c * geo.zchunk : geo.zs[c];
+  const int ze = geo.zchunk > 0 ? min(zb + geo.zchunk, nz) : geo.zs[c + 1];
+  StarTB2Tile<Plan, NWX, BY, kRenamed, kMid>(a1, a2, nx, ny, nz, tile * (BY - 2) - 1, zb, ze, L);
 }
 
 // Workgroups of `kernel` (blocks of `threads`) the current device keeps
@@ -414,32 +403,24 @@ inline bool StarTB2Supported(int nx, int ny, int nz) {
 // planes take the guided schedule instead (kGuidedIfLong).
 //
 // How the fused kernel's workgroups split the z-march (TB2Shape::sched):
-//   kChunked          chunks of zchunk planes per tile;
-//   kResident(IfLong) only the workgroups the device keeps resident, each
-//                     with an equal share of the (tile, plane) positions
-//                     (zchunk == 0 above), IfLong: when every share is at
-//                     least kResidentPlanes planes, else chunked;
-//   kResidentAligned  as kResident with the largest multiple of the tile
-//                     count that fits, so tiles split into the same z
-//                     segments;
-//   kGuided(2)        chunks that shrink as the march is dispatched: each
-//                     the larger of zchunk planes (8 if 0) and the remaining
-//                     planes' share of one (two) resident waves of
-//                     workgroups, so the last workgroups are short while most
-//                     planes are in long chunks that re-read few halo planes;
-//   kGuidedIfLong     kGuided when every resident share would be at least
-//                     kResidentPlanes planes, else chunked.
-// MI355X, 512^3 clamped, 200 steps, GLUPS (chunks of 16 / 32 planes /
-// resident / resident aligned / guided / guided2): FP32 1299 / 1290 / 1207 /
-// 1213 / 1332 / 1294; BF16 1556 / 1527 / 1628 / 1453 / 1647 / 1553.  FP32
-// passes fetch more bytes resident (0.787 GB per pass against 0.744 with
-// 16-plane chunks) and run slower even aligned (0.648 GB, but 430 workgroups
-// for 512 slots).
+//   kChunked       chunks of zchunk planes per tile;
+//   kGuided(2)     chunks that shrink as the march is dispatched: each the
+//                  larger of zchunk planes (8 if 0) and the remaining planes'
+//                  share of one (two) waves of the workgroups the device keeps
+//                  resident, so the last workgroups are short while most
+//                  planes are in long chunks that re-read few halo planes;
+//   kGuidedIfLong  kGuided when every resident workgroup's share of the whole
+//                  grid is at least kResidentPlanes planes, else chunked.
+// MI355X, 512^3 clamped, 200 steps, GLUPS (chunks of 16 / 32 planes / guided
+// / guided2): FP32 1299 / 1290 / 1332 / 1294; BF16 1556 / 1527 / 1647 /
+// 1553.  Measured and dropped: only the resident workgroups, each taking an
+// equal share of the (tile, plane) positions in one or two segments: FP32
+// 1207 (its passes fetch 0.787 GB against 0.744 with 16-plane chunks), BF16
+// 1628; the same with the largest multiple of the tile count, so tiles split
+// into the same segments: FP32 1213 (0.648 GB, but 430 workgroups for 512
+// slots), BF16 1453.
 enum TB2Sched : int {
   kChunked = 0,
-  kResidentIfLong = 1,
-  kResident = 2,
-  kResidentAligned = 3,
   kGuided = 4,
   kGuided2 = 5,
   kGuidedIfLong = 6,
@@ -481,7 +462,7 @@ inline hipError_t LaunchZMarchStarTB2(const typename Plan::Args &a1, const typen
     const int slots = ResidentWorkgroups<TB2KernelId<Plan, NWX, BY, kRenamed, kMid>>(
         reinterpret_cast<const void *>(kernel), 64 * NWX * BY);
     const bool is_long = slots > 0 && (int64_t)geo.tiles_y * nz >= (int64_t)kResidentPlanes * slots;
-    if (slots > 0 && (sched == kGuided || sched == kGuided2 || (sched == kGuidedIfLong && is_long))) {
+    if (slots > 0 && (sched != kGuidedIfLong || is_long)) {
       const int k = sched == kGuided2 ? 2 : 1;
       const int64_t lo = zchunk > 0 && sched != kGuidedIfLong ? zchunk : 8;
       int n = 0, z = 0;
@@ -495,12 +476,6 @@ inline hipError_t LaunchZMarchStarTB2(const typename Plan::Args &a1, const typen
       geo.zs[n] = nz;
       geo.zchunk = -1;
       hipLaunchKernelGGL(kernel, dim3(geo.tiles_y, n, 1), block, 0, stream, a1, a2, geo);
-      return hipGetLastError();
-    }
-    const int n = sched == kResidentAligned && slots >= geo.tiles_y ? slots / geo.tiles_y * geo.tiles_y : slots;
-    if (n > 0 && (sched == kResident || sched == kResidentAligned || (sched == kResidentIfLong && is_long))) {
-      geo.zchunk = 0;
-      hipLaunchKernelGGL(kernel, dim3(n, 1, 1), block, 0, stream, a1, a2, geo);
       return hipGetLastError();
     }
   }

@@ -277,8 +277,9 @@ __device__ __forceinline__ void StrideTB2(const typename Plan::Args &a1, const t
       cx.s = R0[wy + 2][x + 4];
       cx.b = b[q];
       cx.t = t[q];
-      u[q] = RoundTrip<S>(ComputeAt<Plan, false, true, true>(cx, a1, x, yf, kf));
+      u[q] = ComputeAt<Plan, false, true, true>(cx, a1, x, yf, kf);
     }
+    RoundTripN<S>(u);
   };
   // u2 output row at plane k - 1 from u1 planes b, c, t.
   auto level2 = [&](float (*R1)[StrideTB2Lds<NP, BY>::kRow], int k, const float (&b)[NP], const float (&c)[NP],

@@ -43,8 +43,11 @@ struct Native {
   __device__ __forceinline__ static stored_t narrow(T v) { return v; }
 };
 
+typedef float v2f_t __attribute__((ext_vector_type(2)));
+
 // bfloat16 storage of an FP32 value: the upper 16 bits, rounded to nearest
-// even (gfx950's v_cvt_pk_bf16_f32, one instruction); NaN stays NaN.
+// even (gfx950's v_cvt_pk_bf16_f32, one instruction for two values); NaN
+// stays NaN.
 struct BF16 {
   using value_t = float;
   using stored_t = uint16_t;
@@ -58,6 +61,17 @@ struct BF16 {
     __builtin_memcpy(&s, &h, 2);
     return s;
   }
+  // a and b rounded as narrow() rounds them, packed (a in the low half) by
+  // one instruction; the halves of such a pair widened.
+  __device__ __forceinline__ static uint32_t narrow2(float a, float b) {
+    typedef __bf16 v2bf_t __attribute__((ext_vector_type(2)));
+    const v2bf_t h = __builtin_convertvector((v2f_t){a, b}, v2bf_t);
+    uint32_t r;
+    __builtin_memcpy(&r, &h, 4);
+    return r;
+  }
+  __device__ __forceinline__ static float widen_lo(uint32_t r) { return __uint_as_float(r << 16); }
+  __device__ __forceinline__ static float widen_hi(uint32_t r) { return __uint_as_float(r & 0xffff0000u); }
   __device__ __forceinline__ static float load(const stored_t *p) { return widen(GlobalLoad(p)); }
   __device__ __forceinline__ static void store(stored_t *p, float v) { GlobalStoreNT(p, narrow(v)); }
   __device__ __forceinline__ static void store_cached(stored_t *p, float v) { GlobalStore(p, narrow(v)); }
@@ -84,6 +98,20 @@ struct FP16 {
     __builtin_memcpy(&s, &h, 2);
     return s;
   }
+  // a and b rounded as narrow() rounds them, packed (a in the low half) by
+  // one instruction (v_cvt_pk_f16_f32; the empty asm as in narrow()); the
+  // halves of such a pair widened.
+  __device__ __forceinline__ static uint32_t narrow2(float a, float b) {
+    typedef _Float16 v2h_t __attribute__((ext_vector_type(2)));
+    asm("" : "+v"(a));
+    asm("" : "+v"(b));
+    const v2h_t h = __builtin_convertvector((v2f_t){a, b}, v2h_t);
+    uint32_t r;
+    __builtin_memcpy(&r, &h, 4);
+    return r;
+  }
+  __device__ __forceinline__ static float widen_lo(uint32_t r) { return widen((uint16_t)(r & 0xffffu)); }
+  __device__ __forceinline__ static float widen_hi(uint32_t r) { return widen((uint16_t)(r >> 16)); }
   __device__ __forceinline__ static float load(const stored_t *p) { return widen(GlobalLoad(p)); }
   __device__ __forceinline__ static void store(stored_t *p, float v) { GlobalStoreNT(p, narrow(v)); }
   __device__ __forceinline__ static void store_cached(stored_t *p, float v) { GlobalStore(p, narrow(v)); }

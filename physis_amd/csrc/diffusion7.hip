@@ -168,9 +168,6 @@ const Variant kVariants[] = {
     {"tb2r_w2_by8_z8", 4, 2, 8, 8, true, 1, true},
     {"tb2r_w1_by8_z16", 4, 1, 8, 16, true, 1, true},
     {"tb2r_w1_by8_z8", 4, 1, 8, 8, true, 1, true},
-    {"tb2p_w2_by8", 4, 2, 8, 32, true, 1, true, psk::kResident},
-    {"tb2p_w1_by8", 4, 1, 8, 32, true, 1, true, psk::kResident},
-    {"tb2a_w2_by8", 4, 2, 8, 32, true, 1, true, psk::kResidentAligned},
     {"tb2g_w2_by8_m8", 4, 2, 8, 8, true, 1, true, psk::kGuided},
     {"tb2g_w2_by8_m16", 4, 2, 8, 16, true, 1, true, psk::kGuided},
     {"tb2g_w1_by8_m8", 4, 1, 8, 8, true, 1, true, psk::kGuided},
