@@ -213,6 +213,37 @@ def batched_bench(rt, n_grids=256, steps=200, seed=2024):
             "speedup": round(seq / bat, 2), "tuned_config": cfg, "tune_s": round(tune_s, 2)}
 
 
+def batched_profile(rt, n_grids=256, steps=200, seed=2024, config=(0, 8, 64, 2)):
+    """The batched workload's kernels alone (the profiles' command): the same
+    256 grids, one launch configuration set explicitly (default: the mixed
+    width classes, 8 rows, 64-plane chunks, two fused steps), one warm run and
+    one timed run of `steps` steps without a graph, so a profiler sees every
+    dispatch as its own."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    shapes = [tuple(int(round(v)) for v in np.exp(rng.uniform(np.log(32), np.log(256), 3))) for _ in range(n_grids)]
+    pairs = [(rt.Grid(s), rt.Grid(s)) for s in shapes]
+    for i, (a, _) in enumerate(pairs):
+        a.fill_uniform(1000 + i)
+    lups = sum(s[0] * s[1] * s[2] for s in shapes) * steps
+    batch = rt.Batch(pairs, [COEF] * n_grids, rt.CLAMPED)
+    batch.configure(*config)
+    batch.run(steps, graph=False)
+    rt.synchronize()
+    t0 = time.perf_counter()
+    batch.run(steps, graph=False)
+    rt.synchronize()
+    sec = time.perf_counter() - t0
+    cfg = batch.config_name()
+    batch.destroy()
+    for a, b in pairs:
+        a.free()
+        b.free()
+    return {"workload": "%d grids, edges log-uniform [32,256] per axis, seed %d, %d steps, FP32 clamped, no graph"
+                        % (n_grids, seed, steps), "config": cfg, "lups": lups, "s": round(sec, 4),
+            "glups": round(lups / sec / 1e9, 1)}
+
+
 def served_bench(rt, n_req=24, rate=40.0, seed=7, budget_gib=48, tick=50, verbose=False):
     """Served workload: Poisson arrivals at a fixed rate of mixed requests (70%
     7-point diffusion, 30% 19-point Jacobi), cubic edges uniform in [32, 384],
@@ -297,6 +328,20 @@ def reduce_and_copy_bench(rt, n, reps=10):
     del host, back
     g.free()
     return out
+
+
+def diffusion_leg(rt, n, steps, warmup, boundary=0):
+    """The headline run alone (the profiles' command): FP32 n^3, `warmup`
+    then `steps` timed steps through PSHipDiffusion7Run."""
+    g1, g2 = rt.Grid((n, n, n)), rt.Grid((n, n, n))
+    g1.fill_uniform(1)
+    if warmup:
+        rt.diffusion7_run(g1, g2, COEF, boundary, warmup)
+    wall, dev = time_steps(rt, g1, g2, boundary, steps, 0)
+    g1.free()
+    g2.free()
+    return {"workload": "7-point diffusion %d^3 FP32 %s, %d steps" % (n, "periodic" if boundary else "clamped", steps),
+            "glups": round(n ** 3 * steps / wall / 1e9, 1), "kernel_us_per_step": round(dev / steps * 1e6, 2)}
 
 
 def storage16_bench(rt, n, steps=200):
@@ -434,7 +479,10 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the served request-mix leg")
     ap.add_argument("--himeno-only", action="store_true",
                     help="run only the Himeno 257x129x129 leg (single and fused sweeps) and print its JSON")
-    ap.add_argument("--only", choices=["himeno", "served", "batched", "reduce", "storage16"],
+    ap.add_argument("--batch-config", default="0,8,64,2",
+                    help="np,by,zchunk,degree of the batched_profile leg (np 0: width classes, -2: packed rows)")
+    ap.add_argument("--only", choices=["diffusion", "himeno", "served", "batched", "batched_profile", "reduce",
+                                       "storage16"],
                     help="run only this leg and print its JSON (the profiles' commands); served also lists "
                          "every request on stderr")
     args = ap.parse_args()
@@ -450,6 +498,9 @@ def main():
         return
     if args.only:
         leg = {"served": lambda: served_bench(rt, verbose=True), "batched": lambda: batched_bench(rt),
+               "diffusion": lambda: diffusion_leg(rt, args.size, args.steps, args.warmup),
+               "batched_profile": lambda: batched_profile(
+                   rt, config=tuple(int(v) for v in args.batch_config.split(","))),
                "reduce": lambda: reduce_and_copy_bench(rt, args.size),
                "storage16": lambda: storage16_bench(rt, args.size, args.steps)}[args.only]
         print(json.dumps(leg()), flush=True)

@@ -1,0 +1,292 @@
+// skeleton_pack.h -- two fused star-stencil steps on rows of any length,
+// packed across lanes (FP32 grids).
+//
+// The row-tile skeletons give a wave whole rows of a fixed width: a row of
+// nx points occupies 64*NP lane slots (stride tiles) or 256 (star tiles), so
+// rows whose length is not a multiple of that width leave lanes idle, and
+// the stride tiles move one element per lane and instruction.  Here a row
+// takes V = ceil(nx/4) lanes of four consecutive points each (one 16-byte
+// access), and a workgroup of T lanes holds R = floor(T/V) consecutive rows
+// of one z plane, packed lane after lane: lane L holds row L / V, vector
+// vi = L % V, points xs = min(4 vi, nx - 4) .. xs + 3.  The last vector of a
+// row is aligned to the row's end, so it overlaps its neighbour when nx is
+// not a multiple of four: both lanes compute the overlapped points from the
+// same values and write the same bits.  Every access stays inside its row;
+// 16-byte accesses need only 4-byte alignment on this target, so rows need
+// not start on 16 bytes.  Lanes past R*V shadow a lane of the last row (same
+// loads, same LDS words, no stores).
+//
+// LDS holds a plane slot-major: point j of every vector in array j, at the
+// vector's position (row-major over rows and vectors).  A lane's y
+// neighbours are then the words V positions away in the same four arrays,
+// read across the wave as consecutive words (free of bank conflicts, two
+// slots per instruction); its x neighbours inside the vector are its own
+// registers, and the two across the vector's ends are one word each whose
+// array and position are fixed per lane: the neighbouring vector's adjacent
+// point, or at a row end the point itself (clamped) or the row's other end
+// (periodic).  Halo rows and planes are loaded at wrapped or clamped
+// coordinates, so every neighbour a u1 value reads is the element at the
+// wrapped/clamped coordinate (compute_exact).
+//
+// As in skeleton_star_tb.h the workgroup computes u1 = body(u0) on its R
+// rows and writes u2 = body(u1) on the inner R-2 rows, marching z with the
+// planes of its four points in register queues rotated over an unrolled
+// period of three steps, double-buffered LDS (one barrier per step).  u2
+// reads u1 neighbours; at a clamped y face it reads its own row instead of
+// the neighbour row (positions chosen once per lane) and at a z face it
+// keeps the plan's select.  The same FP32 body evaluates both levels, so the
+// result is bitwise equal to two separate steps.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "physis/hip/skeleton.h"
+#include "physis/hip/skeleton_star.h"
+#include "physis/hip/skeleton_stride.h"
+
+namespace psk {
+
+// Rows of a workgroup of 64*nw lanes on rows of nx points (nx >= 4).
+__host__ __device__ constexpr int PackRows(int nx, int nw) { return 64 * nw / ((nx + 3) / 4); }
+
+// LDS of one packed tile for rows of up to NXMAX points: four slot arrays
+// per plane of u0 (the T lane positions and two halo rows of NXMAX/4
+// vectors) and of u1 (the T lane positions), double-buffered by step
+// parity.  Array strides are multiples of 64 words (two arrays per
+// stride-64 read).
+template <int NW, int NXMAX>
+struct PackTB2Lds {
+  static constexpr int kT = 64 * NW;
+  static constexpr int kS0 = (kT + 2 * (NXMAX / 4) + 63) / 64 * 64;
+  static constexpr int kS1 = kT;
+  float u0[2][4 * kS0];
+  float u1[2][4 * kS1];
+};
+
+// Whether a packed tile of nw waves serves rows of nx points (at least one
+// output row per tile).
+__host__ __device__ inline bool PackSupported(int nx, int nw, int nxmax) {
+  return nx >= 4 && nx <= nxmax && PackRows(nx, nw) >= 3;
+}
+
+// One plane of an FP32 grid as a buffer resource of P elements (the same for
+// the whole wave: broadcast from the first lane).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t PackPlane(const float *grid, int64_t zoff, int P) {
+  return __builtin_amdgcn_make_buffer_rsrc(BufRow<Native<float>>::Uniform(grid + zoff), (short)0,
+                                           __builtin_amdgcn_readfirstlane(P * 4), kBufferWord3);
+}
+
+__device__ __forceinline__ void PackLoad(__amdgpu_buffer_rsrc_t r, int byte_off, float (&v)[4]) {
+  const v4f_t q = __builtin_bit_cast(v4f_t, __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0));
+  v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+}
+
+// Non-temporal 16-byte store; dwords past the resource's range are dropped.
+__device__ __forceinline__ void PackStore(__amdgpu_buffer_rsrc_t r, int byte_off, const float (&v)[4]) {
+  typedef uint32_t v4u_t __attribute__((ext_vector_type(4)));
+  const v4f_t q = {v[0], v[1], v[2], v[3]};
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u_t, q), r, byte_off, 0, 2);
+}
+
+// A byte offset past any plane's range (planes here hold < 2^31 bytes): a
+// lane that writes nothing stores there, and the range check drops it, so
+// the store needs no branch.
+constexpr int kPackNoStore = 0x7ffffff0;
+
+// One fused tile: u1 on rows [ya, ya+R) and u2 on rows [ya+1, ya+R-1)
+// (those inside the grid), planes [zb, ze), of an FP32 grid nx x ny x nz
+// (4 <= nx <= NXMAX).  kMid: u1 of the output rows and planes is also
+// stored through out_ptr(a1).  Block: 64*NW threads (one dimension).
+template <class Plan, int NW, int NXMAX, bool kMid = false>
+__device__ __forceinline__ void PackTB2Tile(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx,
+                                            int ny, int nz, int ya, int zb, int ze, PackTB2Lds<NW, NXMAX> &L) {
+  static_assert(std::is_same<typename Plan::Storage, Native<float>>::value, "FP32 grids");
+  constexpr bool W = Plan::kWrap;
+  constexpr int kT = 64 * NW;
+  constexpr int kS0 = PackTB2Lds<NW, NXMAX>::kS0, kS1 = PackTB2Lds<NW, NXMAX>::kS1;
+  nx = __builtin_amdgcn_readfirstlane(nx);
+  ny = __builtin_amdgcn_readfirstlane(ny);
+  nz = __builtin_amdgcn_readfirstlane(nz);
+  ya = __builtin_amdgcn_readfirstlane(ya);
+  const int P = nx * ny;
+  const int64_t plane = P;
+  const int V = (nx + 3) >> 2, R = kT / V;
+  const float *in = Plan::stage_ptr(a1);
+  float *out = Plan::out_ptr(a2);
+  const int lane = threadIdx.x;
+  int r = lane / V;
+  const int vi = lane - r * V;
+  r = min(r, R - 1);  // lanes past the rows shadow the last row
+  const int xs = min(4 * vi, nx - 4);
+  const int y = ya + r;
+  const int yf = FixIndex<W>(y, ny);
+  const int goff = (yf * nx + xs) * 4;  // byte offset in a plane
+  const bool out_row = r >= 1 && r <= R - 2 && y < ny;
+  const int soff = out_row ? goff : kPackNoStore;
+  // Positions: u0 rows at (r + 1) V (the halo rows at 0 and (R + 1) V), u1
+  // rows at r V.
+  const int q0 = (r + 1) * V + vi, q1 = r * V + vi;
+  // u2's y neighbours: at a clamped face the row's own u1; lanes that write
+  // nothing read their own row too (in range).
+  const int qn = (!out_row || (!W && y == 0)) ? q1 : q1 - V;
+  const int qs = (!out_row || (!W && y == ny - 1)) ? q1 : q1 + V;
+  // The x neighbours across the vector's ends, as (array, position offset):
+  // west of point xs, east of point xs + 3.
+  const bool last = vi == V - 1;
+  int wj, wd, ej, ed;
+  if (vi == 0) {
+    wj = W ? 3 : 0;
+    wd = W ? V - 1 : 0;  // periodic: the row's last point; clamped: itself
+  } else {
+    wj = last ? nx - 4 * V + 3 : 3;  // point xs - 1 in vector vi - 1
+    wd = -1;
+  }
+  if (last) {
+    ej = W ? 0 : 3;
+    ed = W ? -vi : 0;  // periodic: the row's first point; clamped: itself
+  } else {
+    ej = vi == V - 2 ? 4 * V - nx : 0;  // point xs + 4 in vector vi + 1
+    ed = 1;
+  }
+  const int aw0 = wj * kS0 + q0 + wd, ae0 = ej * kS0 + q0 + ed;
+  const int aw1 = wj * kS1 + q1 + wd, ae1 = ej * kS1 + q1 + ed;
+
+  // Halo rows ya - 1 (positions [0, V)) and ya + R (positions [(R+1) V,
+  // (R+2) V)): element t of the 2 nx per lane and round, loaded and
+  // published without a branch (a lane past them repeats element t mod 2 nx:
+  // the same load, the same words).  A point in the overlap of a row's last
+  // two vectors is written to both.
+  constexpr int kHN = (2 * NXMAX + kT - 1) / kT;
+  int hoff[kHN], ha[kHN], hb[kHN];
+  {
+    const int hy_lo = FixIndex<W>(ya - 1, ny), hy_hi = FixIndex<W>(ya + R, ny);
+#pragma unroll
+    for (int j = 0; j < kHN; ++j) {
+      int t = lane + j * kT;
+      t = t < 2 * nx ? t : t % (2 * nx);
+      const bool hi = t >= nx;
+      const int x = hi ? t - nx : t;
+      hoff[j] = ((hi ? hy_hi : hy_lo) * nx + x) * 4;
+      const int base = hi ? (R + 1) * V : 0;
+      const int lastpos = base + V - 1, lastslot = x - (nx - 4);
+      hb[j] = lastslot >= 0 ? lastslot * kS0 + lastpos : (x & 3) * kS0 + base + (x >> 2);
+      ha[j] = (x >> 2) < V - 1 ? (x & 3) * kS0 + base + (x >> 2) : hb[j];
+    }
+  }
+  float hraw[kHN];
+  auto ldh = [&](int zf) {
+    const __amdgpu_buffer_rsrc_t rs = PackPlane(in, (int64_t)zf * plane, P);
+#pragma unroll
+    for (int j = 0; j < kHN; ++j)
+      hraw[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, hoff[j], 0, 0));
+  };
+  auto puth = [&](float *R0) {
+#pragma unroll
+    for (int j = 0; j < kHN; ++j) {
+      R0[ha[j]] = hraw[j];
+      R0[hb[j]] = hraw[j];
+    }
+  };
+  auto ld = [&](int zf, float (&v)[4]) { PackLoad(PackPlane(in, (int64_t)zf * plane, P), goff, v); };
+  // The lane's four points into the slot arrays (stride S) at position q.
+  auto put = [&](float *Rb, int q, int S, const float (&v)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) Rb[j * S + q] = v[j];
+  };
+  auto get = [&](const float *Rb, int q, int S, float (&v)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = Rb[j * S + q];
+  };
+
+  // u1 of the lane's points at plane k (index kf) from u0 planes b, c, t.
+  auto level1 = [&](const float *R0, int kf, const float (&b)[4], const float (&c)[4], const float (&t)[4],
+                    float (&u)[4]) {
+    float nv[4], sv[4];
+    get(R0, q0 - V, kS0, nv);
+    get(R0, q0 + V, kS0, sv);
+    const float wl = R0[aw0], er = R0[ae0];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      StarCtx cx;
+      cx.c = c[j];
+      cx.w = j > 0 ? c[j - 1] : wl;
+      cx.e = j < 3 ? c[j + 1] : er;
+      cx.n = nv[j];
+      cx.s = sv[j];
+      cx.b = b[j];
+      cx.t = t[j];
+      u[j] = ComputeAt<Plan, true, true, true>(cx, a1, xs + j, yf, kf);
+    }
+  };
+  // u2 of the lane's points at plane k - 1 from u1 planes b, c, t.
+  auto level2 = [&](const float *R1, int k, const float (&b)[4], const float (&c)[4], const float (&t)[4]) {
+    float nv[4], sv[4], o[4];
+    get(R1, qn, kS1, nv);
+    get(R1, qs, kS1, sv);
+    const float wl = R1[aw1], er = R1[ae1];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      StarCtx cx;
+      cx.c = c[j];
+      cx.w = j > 0 ? c[j - 1] : wl;
+      cx.e = j < 3 ? c[j + 1] : er;
+      cx.n = nv[j];
+      cx.s = sv[j];
+      cx.b = b[j];
+      cx.t = t[j];
+      o[j] = ComputeAt<Plan, true, true, false>(cx, a2, xs + j, y, k - 1);
+    }
+    PackStore(PackPlane(out, (int64_t)(k - 1) * plane, P), soff, o);
+  };
+
+  float Q0[3][4], Q1[3][4] = {};
+  // Issued in the order the loop leaves its loads in flight (the halo plane
+  // before the newest u0 plane; the scheduling barriers keep the order).
+  ld(FixIndex<W>(zb - 2, nz), Q0[0]);
+  ld(FixIndex<W>(zb - 1, nz), Q0[1]);
+  __builtin_amdgcn_sched_barrier(0);
+  ldh(FixIndex<W>(zb - 1, nz));
+  __builtin_amdgcn_sched_barrier(0);
+  ld(FixIndex<W>(zb, nz), Q0[2]);
+  __builtin_amdgcn_sched_barrier(0);
+  // Wrapped/clamped indices of planes k, k + 1, k + 2.
+  int z0 = FixIndex<W>(zb - 1, nz), z1 = FixIndex<W>(zb, nz), z2 = FixIndex<W>(zb + 1, nz);
+  auto step = [&](int k, auto rc) {
+    constexpr int rq = decltype(rc)::value;
+    constexpr int j0 = rq, j1 = (rq + 1) % 3, j2 = (rq + 2) % 3;
+    const int buf = (k + 2) & 1;
+    float *R0 = L.u0[buf];
+    float *R1 = L.u1[buf];
+    put(R0, q0, kS0, Q0[j1]);
+    puth(R0);
+    put(R1, q1, kS1, Q1[j1]);
+    // z1 is an in-grid index for every k: at k == ze the load is unused.
+    ldh(z1);
+    __syncthreads();
+    level1(R0, z0, Q0[j0], Q0[j1], Q0[j2], Q1[j2]);
+    if constexpr (kMid) {
+      if (k >= zb && k < ze) PackStore(PackPlane(Plan::out_ptr(a1), (int64_t)k * plane, P), soff, Q1[j2]);
+    }
+    ld(z2, Q0[j0]);
+    if (k - 1 >= zb) level2(R1, k, Q1[j0], Q1[j1], Q1[j2]);
+    z0 = z1;
+    z1 = z2;
+    z2 = NextPlane<W>(z2, nz);
+  };
+  using C0 = std::integral_constant<int, 0>;
+  using C1 = std::integral_constant<int, 1>;
+  using C2 = std::integral_constant<int, 2>;
+  int k = zb - 1;
+  for (; k + 2 <= ze; k += 3) {
+    step(k, C0{});
+    step(k + 1, C1{});
+    step(k + 2, C2{});
+  }
+  if (k <= ze) step(k, C0{});
+  if (k + 1 <= ze) step(k + 1, C1{});
+}
+
+}  // namespace psk

@@ -180,7 +180,11 @@ extern int PSHipBatchRunOn(void *batch, int steps, void *stream);
  * grids of whole 16-byte rows longer than 128 points: the vectorised star
  * tile) and each class on its own stream (forked from and joined to the
  * caller's).  np = -1: the mixed mode with 16 rows per workgroup for the
- * classes of 1 and 2 points per lane. */
+ * classes of 1 and 2 points per lane.  np = -2: packed mode (FP32 grids
+ * with rows of 32 to 512 points, degree 2 only): the fused pairs of all
+ * grids in one launch on tiles that pack rows of any length across lanes
+ * (four points per lane), `by` (8 or 16) waves per workgroup; single steps
+ * as in the mixed mode. */
 extern int PSHipBatchConfigure(void *batch, int np, int by, int zchunk, int degree);
 extern int PSHipBatchConfigName(void *batch, char *buf, int len);
 /* Times every candidate shape for `steps` steps on the batch (contents

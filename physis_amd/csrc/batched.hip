@@ -22,6 +22,7 @@
 #include "physis/hip/skeleton.h"
 #include "physis/hip/skeleton_star.h"
 #include "physis/hip/skeleton_star_tb.h"
+#include "physis/hip/skeleton_pack.h"
 #include "physis/hip/skeleton_stride.h"
 #include "physis/hip/storage.h"
 
@@ -71,13 +72,36 @@ struct Item {
   int grid, y0, zb, ze;
 };
 
+// A work item and its grids' arguments are the same for the whole
+// workgroup, but the compiler loads tables with vector loads; broadcast from
+// the first lane, the values live in scalar registers instead of holding
+// some twenty vector registers for the whole z-march.
+__device__ __forceinline__ int Uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float Uni(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+template <class T>
+__device__ __forceinline__ T *Uni(T *p) {
+  const uint64_t a = reinterpret_cast<uint64_t>(p);
+  return reinterpret_cast<T *>(((uint64_t)(uint32_t)Uni((int)(a >> 32)) << 32) | (uint32_t)Uni((int)a));
+}
+__device__ __forceinline__ Item Uniform(const Item &it) { return Item{Uni(it.grid), Uni(it.y0), Uni(it.zb), Uni(it.ze)}; }
+__device__ __forceinline__ BatchArgs Uniform(const BatchArgs &a) {
+  BatchArgs u;
+  u.in = Uni(a.in);
+  u.out = Uni(a.out);
+  for (int d = 0; d < 3; ++d) u.dim[d] = Uni(a.dim[d]);
+  u.k = Coef7{Uni(a.k.ce), Uni(a.k.cw), Uni(a.k.cn), Uni(a.k.cs), Uni(a.k.ct), Uni(a.k.cb), Uni(a.k.cc)};
+  return u;
+}
+
 template <class P, int NP, int BY>
 __global__ void __launch_bounds__(64 * BY)
 BatchStep(const typename P::Args *args, const Item *items, int nitems) {
   __shared__ psk::StrideLds<NP, BY> L;
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
-    const Item it = items[w];
-    const typename P::Args a = args[it.grid];
+    const Item it = Uniform(items[w]);
+    const typename P::Args a = Uniform(args[it.grid]);
     psk::StrideStep<P, NP, BY>(a, a.dim[0], a.dim[1], a.dim[2], it.y0, it.zb, it.ze, L);
     __syncthreads();  // the next item reuses the LDS rows
   }
@@ -90,9 +114,9 @@ __global__ void __launch_bounds__(64 * BY)
 BatchTB2(const typename P::Args *in_args, const typename P::Args *out_args, const Item *items, int nitems) {
   __shared__ psk::StrideTB2Lds<NP, BY> L;
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
-    const Item it = items[w];
-    const typename P::Args a1 = in_args[it.grid];
-    const typename P::Args a2 = out_args[it.grid];
+    const Item it = Uniform(items[w]);
+    const typename P::Args a1 = Uniform(in_args[it.grid]);
+    const typename P::Args a2 = Uniform(out_args[it.grid]);
     psk::StrideTB2<P, NP, BY, kMid>(a1, a2, a1.dim[0], a1.dim[1], a1.dim[2], it.y0, it.zb, it.ze, L);
     __syncthreads();
   }
@@ -116,14 +140,33 @@ __global__ void __launch_bounds__(64 * NWX * kStarBY, 8)
 BatchStarTB2(const typename P::Args *in_args, const typename P::Args *out_args, const Item *items, int nitems) {
   __shared__ psk::StarTB2Lds<NWX, kStarBY> L;
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
-    const Item it = items[w];
-    const typename P::Args a1 = in_args[it.grid];
-    const typename P::Args a2 = out_args[it.grid];
+    const Item it = Uniform(items[w]);
+    const typename P::Args a1 = Uniform(in_args[it.grid]);
+    const typename P::Args a2 = Uniform(out_args[it.grid]);
     psk::StarTB2Tile<P, NWX, kStarBY, true, kMid>(a1, a2, a1.dim[0], a1.dim[1], a1.dim[2], it.y0, it.zb, it.ze,
                                                   L);
     __syncthreads();
   }
 }
+
+// Packed mode (FP32): every grid's fused pairs on the packed-row tile
+// (psk::PackTB2Tile: rows of any length, several short rows per wave), all
+// grids in one work list and one launch; rows of up to NXMAX points.
+template <class P, int NW, int NXMAX, bool kMid>
+__global__ void __launch_bounds__(64 * NW, 8)
+BatchPackTB2(const typename P::Args *in_args, const typename P::Args *out_args, const Item *items, int nitems) {
+  __shared__ psk::PackTB2Lds<NW, NXMAX> L;
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const Item it = Uniform(items[w]);
+    const typename P::Args a1 = Uniform(in_args[it.grid]);
+    const typename P::Args a2 = Uniform(out_args[it.grid]);
+    psk::PackTB2Tile<P, NW, NXMAX, kMid>(a1, a2, a1.dim[0], a1.dim[1], a1.dim[2], it.y0, it.zb, it.ze, L);
+    __syncthreads();
+  }
+}
+
+// Longest row the packed kernels are built for, given the batch's longest.
+inline int PackNxMax(int max_nx) { return max_nx <= 256 ? 256 : 512; }
 
 // A launch configuration of the batch.
 struct Config {
@@ -137,6 +180,9 @@ struct Config {
   // 2: the same, with 16 rows per workgroup for the 1- and 2-point classes
   // (their kernels keep full occupancy at 16 rows, which halves the share
   // of halo rows).
+  // 3: packed: the fused pairs of every grid on the packed-row tile in one
+  // launch (`by` = waves per workgroup; FP32 batches of rows of 4 to 512
+  // points), single steps as in mode 1.
   int mixed = 0;
   bool operator<(const Config &o) const {
     return std::tie(np, by, zchunk, degree, mixed) < std::tie(o.np, o.by, o.zchunk, o.degree, o.mixed);
@@ -144,6 +190,8 @@ struct Config {
 };
 
 std::string ConfigName(const Config &c) {
+  if (c.mixed == 3)
+    return "packed_w" + std::to_string(c.by) + "_z" + std::to_string(c.zchunk) + "_t" + std::to_string(c.degree);
   return (c.mixed ? std::string(c.mixed == 2 ? "mixedn16" : "mixed") : "np" + std::to_string(c.np)) + "_by" +
          std::to_string(c.by) + "_z" +
          std::to_string(c.zchunk) + "_t" + std::to_string(c.degree);
@@ -208,6 +256,22 @@ KernelFn TB2Kernel(int storage, bool per, int np, int by, bool mid = false) {
     return per ? TB2KernelFor<psk::BF16, true>(np, by, mid) : TB2KernelFor<psk::BF16, false>(np, by, mid);
   return per ? TB2KernelFor<psk::Native<float>, true>(np, by, mid)
              : TB2KernelFor<psk::Native<float>, false>(np, by, mid);
+}
+
+// Packed fused kernel (FP32 only): nw waves, rows up to nxmax points.
+KernelFn PackKernel(int storage, bool per, int nw, int nxmax, bool mid) {
+  if (storage != PS_HIP_STORAGE_NATIVE) return nullptr;
+#define PSH_K(NW, NXMAX)                                                                                       \
+  if (nw == NW && nxmax == NXMAX) {                                                                            \
+    if (per)                                                                                                   \
+      return mid ? reinterpret_cast<KernelFn>(&BatchPackTB2<BatchPlan<psk::Native<float>, true>, NW, NXMAX, true>) \
+                 : reinterpret_cast<KernelFn>(&BatchPackTB2<BatchPlan<psk::Native<float>, true>, NW, NXMAX, false>); \
+    return mid ? reinterpret_cast<KernelFn>(&BatchPackTB2<BatchPlan<psk::Native<float>, false>, NW, NXMAX, true>)  \
+               : reinterpret_cast<KernelFn>(&BatchPackTB2<BatchPlan<psk::Native<float>, false>, NW, NXMAX, false>); \
+  }
+  PSH_K(8, 256) PSH_K(16, 256) PSH_K(16, 512)
+#undef PSH_K
+  return nullptr;
 }
 
 // Rows per workgroup of class `cls` in mixed configuration c.
@@ -322,6 +386,14 @@ struct Batch {
   // A degree-2 run also needs the one-step kernel of the same shape for its
   // unfused steps.
   bool Supports(const Config &c) const {
+    if (c.mixed == 3) {
+      const int nxmax = PackNxMax(max_nx);
+      if (c.degree != 2 || c.zchunk <= 0 || PackKernel(storage, periodic, c.by, nxmax, false) == nullptr)
+        return false;
+      for (int i = 0; i < Count(); ++i)
+        if (!psk::PackSupported(a[i]->dim[0], c.by, nxmax)) return false;
+      return Supports(SingleSteps(c));
+    }
     if (c.mixed) {
       if (c.zchunk <= 0) return false;
       for (int np : classes)
@@ -345,7 +417,17 @@ struct Batch {
     std::sort(order.begin(), order.end(), [&](int x, int y) { return a[x]->num_elms > a[y]->num_elms; });
     std::vector<Item> items;
     const int rows_out = c.degree == 2 ? PairRows(c.np, c.by) : c.by;
-    for (int g : order) {
+    if (c.mixed == 3) {
+      // Packed tiles of R - 2 output rows (R rows per workgroup), z chunks.
+      for (int g : order) {
+        const int ny = a[g]->dim[1], nz = a[g]->dim[2];
+        const int rows_out = psk::PackRows(a[g]->dim[0], c.by) - 2;
+        const int zc = c.zchunk > nz ? nz : c.zchunk;
+        for (int ty = 0; ty * rows_out < ny; ++ty)
+          for (int zb = 0; zb < nz; zb += zc) items.push_back(Item{g, ty * rows_out - 1, zb, std::min(zb + zc, nz)});
+      }
+    }
+    for (int g : (c.mixed == 3 ? std::vector<int>() : order)) {
       if (c.mixed && cls[g] != c.np) continue;  // another class
       const int ny = a[g]->dim[1], nz = a[g]->dim[2];
       const int zc = c.zchunk <= 0 || c.zchunk > nz ? nz : c.zchunk;
@@ -362,9 +444,11 @@ struct Batch {
     p.items.Assign(items, s);
     p.nitems = (int)items.size();
     // Persistent grid: as many workgroups as fit on the device at once.
-    KernelFn k = c.degree == 2 ? TB2Kernel(storage, false, c.np, c.by) : StepKernel(storage, false, c.np, c.by);
+    KernelFn k = c.mixed == 3 ? PackKernel(storage, periodic, c.by, PackNxMax(max_nx), false)
+                 : c.degree == 2 ? TB2Kernel(storage, false, c.np, c.by)
+                                 : StepKernel(storage, false, c.np, c.by);
     int per_cu = 1, cus = 1;
-    const dim3 blk = c.degree == 2 ? PairBlock(c.np, c.by) : dim3(64, c.by);
+    const dim3 blk = c.mixed == 3 ? dim3(64 * c.by) : c.degree == 2 ? PairBlock(c.np, c.by) : dim3(64, c.by);
     CheckHip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, (int)(blk.x * blk.y), 0), "occupancy");
     hipDeviceProp_t prop;
     CheckHip(hipGetDeviceProperties(&prop, Runtime::Get().device()), "props");
@@ -378,6 +462,11 @@ struct Batch {
   // must happen before a graph capture).
   void Prepare(const Config &c, hipStream_t s) {
     BuildTables(s);
+    if (c.mixed == 3) {
+      PlanFor(c, s);
+      Prepare(SingleSteps(c), s);
+      return;
+    }
     if (!c.mixed) {
       PlanFor(Config{c.np, c.by, c.zchunk, 1}, s);
       PlanFor(c, s);
@@ -401,6 +490,7 @@ struct Batch {
   // Queues `steps` steps of every grid with configuration c.
   hipError_t Enqueue(const Config &c, int steps, hipStream_t s) {
     Prepare(c, s);
+    if (c.mixed == 3) return EnqueuePacked(c, steps, s);
     if (!c.mixed) return EnqueueOne(c, steps, s);
     // One stream per width class: fork from s, run each class's steps, join.
     hipError_t e = hipEventRecord(fork_ev, s);
@@ -411,6 +501,35 @@ struct Batch {
       if (e == hipSuccess) e = hipEventRecord(join_ev[i], cs);
       if (e == hipSuccess) e = hipStreamWaitEvent(s, join_ev[i], 0);
     }
+    return e;
+  }
+
+  // The configuration that runs the single steps of packed configuration c:
+  // the width classes on their own streams.
+  static Config SingleSteps(const Config &c) { return Config{0, 8, c.zchunk, 1, 1}; }
+
+  // Packed mode: every pair fused along psk::FusedPathBuffer's path in one
+  // launch over all grids; an even count's last pass also stores u_(steps-1)
+  // into B, an odd count ends with one single step A -> B; runs of fewer than
+  // two pairs are single steps.
+  hipError_t EnqueuePacked(const Config &c, int steps, hipStream_t s) {
+    const int pairs = steps / 2;
+    if (pairs < 2) return Enqueue(SingleSteps(c), steps, s);
+    Plan &p = PlanFor(c, s);
+    const bool odd = steps & 1;
+    const int nxmax = PackNxMax(max_nx);
+    hipError_t e = hipSuccess;
+    for (int j = 0; j < pairs && e == hipSuccess; ++j) {
+      const int src = psk::FusedPathBuffer(j, pairs), dst = psk::FusedPathBuffer(j + 1, pairs);
+      const bool mid = j == pairs - 1 && !odd;
+      KernelFn k = PackKernel(storage, periodic, c.by, nxmax, mid);
+      const void *ti = mid ? t_in_mid.p : t_in[src].p, *to = t_out[dst].p;
+      const Item *iv = p.items.p;
+      int n = p.nitems;
+      void *args[] = {&ti, &to, &iv, &n};
+      e = hipLaunchKernel(k, dim3(p.blocks), dim3(64 * c.by), args, 0, s);
+    }
+    if (odd && e == hipSuccess) e = Enqueue(SingleSteps(c), 1, s);
     return e;
   }
 
@@ -500,6 +619,12 @@ std::vector<Config> Candidates(const Batch &b) {
       if (b.Supports(c)) out.push_back(c);
     }
   }
+  // Packed tiles (FP32 batches).
+  for (int nw : {8, 16})
+    for (int zc : {16, 32, 64}) {
+      Config c{0, nw, zc, 2, 3};
+      if (b.Supports(c)) out.push_back(c);
+    }
   // One width for all grids: the narrowest that fits the widest row only
   // (wider rows idle lanes).
   const int np = ClassNp(b.max_nx);
@@ -611,11 +736,12 @@ int PSHipBatchRun(void *h, int steps, int use_graph) {
 // Sets the launch configuration explicitly (np, by, zchunk, degree); np = 0
 // selects the mixed mode (each grid in its class, classes on their own
 // streams), np = -1 the mixed mode with 16-row tiles for the 1- and 2-point
-// classes.
+// classes, np = -2 the packed mode with `by` waves per workgroup (FP32
+// batches; fused pairs only, degree 2).
 int PSHipBatchConfigure(void *h, int np, int by, int zchunk, int degree) {
   if (!h) return PS_HIP_ERR_INVALID;
   Batch *b = static_cast<Batch *>(h);
-  Config c{np, by, zchunk, degree, np == 0 ? 1 : np == -1 ? 2 : 0};
+  Config c{np, by, zchunk, degree, np == 0 ? 1 : np == -1 ? 2 : np == -2 ? 3 : 0};
   if (np < 0) c.np = 0;
   if ((degree != 1 && degree != 2) || !b->Supports(c)) return PS_HIP_ERR_UNSUPPORTED;
   b->tuned = c;

@@ -151,13 +151,56 @@ def test_batch_step_counts_bitwise_vs_oracle(rt, cfg, steps):
         g2.free()
 
 
-def test_batched_workload_256_grids_200_steps_equal_single_grid_runs(rt):
+# Rows of 4 to 512 points of every residue mod 4 (the packed tile aligns a
+# row's last vector to its end), one to many rows per workgroup, thin and
+# deep z.
+PACKED = [(4, 9, 5), (5, 3, 7), (7, 17, 3), (13, 2, 9), (32, 5, 7), (33, 40, 6), (37, 1, 9), (63, 2, 3),
+          (64, 64, 33), (65, 9, 1), (127, 6, 5), (139, 17, 12), (200, 3, 70), (255, 11, 4), (256, 6, 5)]
+PACKED_WIDE = PACKED + [(257, 5, 6), (300, 4, 6), (511, 9, 3), (512, 3, 4)]
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+@pytest.mark.parametrize("cfg,shapes", [((-2, 8, 16, 2), PACKED), ((-2, 16, 64, 2), PACKED),
+                                        ((-2, 16, 8, 2), PACKED_WIDE)])
+@pytest.mark.parametrize("steps", [4, 9])
+@pytest.mark.parametrize("graph", [False, True])
+def test_packed_batch_bitwise_vs_oracle(rt, boundary, cfg, shapes, steps, graph):
+    """The packed mode (rows of any length packed across lanes, four points
+    per lane, two fused steps per launch over all grids): every grid equals
+    the oracle bitwise, for even counts (the last pass stores the
+    intermediate step) and odd ones (a single step follows)."""
+    pairs, fields, coefs, batch = make_batch(rt, shapes, seed=21, boundary=boundary)
+    batch.configure(*cfg)
+    print("config", batch.config_name(), "steps", steps, "graph", graph, flush=True)
+    batch.run(steps, graph=graph)
+    check_against_oracle(pairs, fields, coefs, boundary, steps)
+    batch.destroy()
+    for g1, g2 in pairs:
+        g1.free()
+        g2.free()
+
+
+def test_packed_mode_rejects_rows_it_does_not_serve(rt):
+    # Rows shorter than 4 points, rows over 256 points for 8 waves per
+    # workgroup, and one step per launch are not served by the packed mode.
+    for shapes, cfg in ((NARROW, (-2, 8, 16, 2)), (PACKED_WIDE, (-2, 8, 16, 2)), (PACKED, (-2, 16, 16, 1))):
+        pairs, _, _, batch = make_batch(rt, shapes, seed=1, boundary=0)
+        with pytest.raises(rt.PhysisError):
+            batch.configure(*cfg)
+        batch.destroy()
+        for g1, g2 in pairs:
+            g1.free()
+            g2.free()
+
+
+@pytest.mark.parametrize("mode", ["tuned", "packed"])
+def test_batched_workload_256_grids_200_steps_equal_single_grid_runs(rt, mode):
     """The batched workload at its size: 256 FP32 grids with edges drawn
     log-uniform in [32, 256] per axis (seed 2024), fields uniform [0, 1),
-    clamped, 200 steps with the autotuned launch shape and one hipGraph.
-    Every grid (both buffers) ends bitwise equal to the same grid advanced
-    alone by the plain single-grid run, and the three smallest grids equal
-    the oracle."""
+    clamped, 200 steps with the autotuned launch shape (or the packed mode
+    with 8 waves and 64-plane chunks) and one hipGraph.  Every grid (both
+    buffers) ends bitwise equal to the same grid advanced alone by the plain
+    single-grid run, and the three smallest grids equal the oracle."""
     coef = [0.1] * 6 + [0.4]
     steps = 200
     rng = np.random.default_rng(2024)
@@ -167,8 +210,11 @@ def test_batched_workload_256_grids_200_steps_equal_single_grid_runs(rt):
         a.fill_uniform(1000 + i)
         b.fill_uniform(5000 + i)
     batch = rt.Batch(pairs, [coef] * len(pairs), rt.CLAMPED)
-    batch.tune(8)
-    print("tuned", batch.config_name(), flush=True)
+    if mode == "tuned":
+        batch.tune(8)
+    else:
+        batch.configure(-2, 8, 64, 2)
+    print(mode, batch.config_name(), flush=True)
     for i, (a, b) in enumerate(pairs):
         a.fill_uniform(1000 + i)
         b.fill_uniform(5000 + i)
