@@ -17,6 +17,7 @@
 #include "physis/runtime.h"
 #include "physis/hip/skeleton.h"
 #include "physis/hip/skeleton_star.h"
+#include "physis/hip/skeleton_pack.h"
 #include "physis/hip/skeleton_star_tb.h"
 #include "physis/hip/skeleton_stride.h"
 #include "physis/hip/storage.h"
@@ -120,14 +121,21 @@ inline bool FusedPairSupported(int nx, int ny, int nz) {
 }
 
 // Two fused steps src -> dst for a star plan.  mid: also store the
-// intermediate step through out_ptr(a1).  Returns the tile that ran: 1 the
-// vectorised tile (rows of whole 16-byte vectors), 2 the stride tile, 0 none
+// intermediate step through out_ptr(a1).  Returns the tile that ran: 3 the
+// packed-row tile (FP32 grids with rows of 4 to 512 points, unless they fill
+// whole 256-point star waves), 1 the vectorised tile (rows of whole 16-byte
+// vectors), 2 the stride tile (other rows of 16-bit or double grids), 0 none
 // (the row width is not supported).
 template <class P>
 inline int LaunchFusedPair(const typename P::Args &a1, const typename P::Args &a2, int nx, int ny, int nz,
                            hipStream_t s, bool mid) {
-  hipError_t e = LaunchZMarchStarTB2Auto<P>(a1, a2, nx, ny, nz, s, mid);
-  int tile = 1;
+  hipError_t e = hipErrorNotSupported;
+  int tile = 3;
+  if (nx % 256 != 0) e = LaunchPackTB2Grid<P>(a1, a2, nx, ny, nz, s, mid);
+  if (e == hipErrorNotSupported) {
+    e = LaunchZMarchStarTB2Auto<P>(a1, a2, nx, ny, nz, s, mid);
+    tile = 1;
+  }
   if (e == hipErrorNotSupported) {
     e = LaunchStrideTB2Grid<P>(a1, a2, nx, ny, nz, s, mid);
     tile = 2;
@@ -179,7 +187,7 @@ inline void RunPingPong(typename P::Args ab, typename P::Args ba, int iter, int 
     }
     if (__ps_trace)
       fprintf(__ps_trace, "Physis: fused pairs: %d passes on the %s tile\n", iter,
-              tile == 1 ? "vectorised" : "stride");
+              tile == 1 ? "vectorised" : tile == 3 ? "packed" : "stride");
     return;
   }
   for (int i = 0; i < iter; ++i) {

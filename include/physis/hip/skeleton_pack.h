@@ -41,6 +41,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "physis/hip/skeleton.h"
@@ -287,6 +288,86 @@ __device__ __forceinline__ void PackTB2Tile(const typename Plan::Args &a1, const
   }
   if (k <= ze) step(k, C0{});
   if (k + 1 <= ze) step(k + 1, C1{});
+}
+
+// Two fused maps over one whole FP32 grid on the packed tile: workgroup
+// (blockIdx.x, blockIdx.y) = (tile of PackRows - 2 output rows, z chunk);
+// tiles are renumbered per XCD.
+template <class Plan, int NW, int NXMAX, bool kMid>
+__global__ void __launch_bounds__(64 * NW, 8)
+PackTB2Grid(typename Plan::Args a1, typename Plan::Args a2, int nx, int ny, int nz, int zchunk) {
+  __shared__ PackTB2Lds<NW, NXMAX> L;
+  const int rows_out = PackRows(nx, NW) - 2;
+  const int ty = XcdTile(blockIdx.x, gridDim.x);
+  const int zb = (int)blockIdx.y * zchunk;
+  PackTB2Tile<Plan, NW, NXMAX, kMid>(a1, a2, nx, ny, nz, ty * rows_out - 1, zb, min(zb + zchunk, nz), L);
+}
+
+// Compute units of the current device (queried once).
+inline int PackDeviceCUs() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+  }
+  return cus;
+}
+
+// z chunk of a whole-grid packed launch: the one in [4, 64] that minimises
+// rounds x (chunk + 4), where a round is one workgroup per slot of the
+// device (four 8-wave or two 16-wave workgroups per CU, the LDS and
+// register limits) and the 4 planes stand for a chunk's halo planes and
+// pipeline fill.  A rule that only asks for enough waves leaves a last round
+// nearly empty when the workgroup count falls just past a multiple of the
+// slots (300^3: 8-plane chunks fill 2.08 rounds).
+inline int PackZChunk(int tiles_y, int nz, int nw) {
+  const int64_t slots = (int64_t)PackDeviceCUs() * (nw == 8 ? 4 : 2);
+  int best = 4;
+  int64_t best_cost = -1;
+  for (int z = 4; z <= 64; ++z) {
+    const int64_t groups = (int64_t)tiles_y * ((nz + z - 1) / z);
+    const int64_t cost = (groups + slots - 1) / slots * (std::min(z, nz) + 4);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = z;
+    }
+  }
+  return best;
+}
+
+// Host launch of PackTB2Grid for an FP32 plan and rows of 4 to 512 points
+// (8 waves per workgroup up to 256 points, else 16), z chunk by PackZChunk;
+// hipErrorNotSupported for other grids and storage kinds.
+// mid: also store the intermediate level through out_ptr(a1).
+template <class Plan>
+inline hipError_t LaunchPackTB2Grid(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx, int ny,
+                                    int nz, hipStream_t stream, bool mid = false) {
+  if constexpr (!std::is_same<typename Plan::Storage, Native<float>>::value) {
+    return hipErrorNotSupported;
+  } else {
+    if (nx < 4 || nx > 512 || ny < 1 || nz < 1) return hipErrorNotSupported;
+    const bool wide = nx > 256;
+    const int nw = wide ? 16 : 8;
+    if (!PackSupported(nx, nw, wide ? 512 : 256)) return hipErrorNotSupported;
+    const int rows_out = PackRows(nx, nw) - 2;
+    const int tiles_y = (ny + rows_out - 1) / rows_out;
+    const int zchunk = PackZChunk(tiles_y, nz, nw);
+    const dim3 grid(tiles_y, (nz + zchunk - 1) / zchunk, 1), block(64 * nw, 1, 1);
+#define PSK_PACK(NW, NXMAX)                                                                                       \
+  if (mid)                                                                                                        \
+    hipLaunchKernelGGL((PackTB2Grid<Plan, NW, NXMAX, true>), grid, block, 0, stream, a1, a2, nx, ny, nz, zchunk);  \
+  else                                                                                                            \
+    hipLaunchKernelGGL((PackTB2Grid<Plan, NW, NXMAX, false>), grid, block, 0, stream, a1, a2, nx, ny, nz, zchunk);
+    if (wide) {
+      PSK_PACK(16, 512)
+    } else {
+      PSK_PACK(8, 256)
+    }
+#undef PSK_PACK
+    return hipGetLastError();
+  }
 }
 
 }  // namespace psk

@@ -637,7 +637,13 @@ std::vector<Config> Candidates(const Batch &b) {
   return out;
 }
 
+// Before tuning: the packed mode where it serves the batch (FP32 rows of 4
+// to 512 points), else one width for all grids.
 Config DefaultConfig(const Batch &b) {
+  for (int nw : {8, 16}) {
+    const Config c{0, nw, 16, 2, 3};
+    if (b.Supports(c)) return c;
+  }
   return Config{ClassNp(b.max_nx), 8, 32, 2};
 }
 

@@ -12,6 +12,7 @@
 #include "diffusion7_body.h"
 #include "physis/hip/skeleton.h"
 #include "physis/hip/skeleton_star.h"
+#include "physis/hip/skeleton_pack.h"
 #include "physis/hip/skeleton_star_tb.h"
 #include "physis/hip/skeleton_stride.h"
 #include "physis/hip/storage.h"
@@ -226,8 +227,9 @@ hipError_t LaunchPair(const Variant &v, const void *src, void *dst, void *mid, c
   return hipErrorInvalidValue;
 }
 
-// A fused pair on the stride skeleton (rows of any length up to 512 points),
-// for whole-grid runs whose rows are not whole 16-byte vectors.
+// A fused pair for whole-grid runs whose rows are not whole 16-byte vectors
+// (rows of any length up to 512 points): the packed-row tile for FP32 grids,
+// the stride tile for 16-bit storage.
 template <class S, bool kPeriodic>
 hipError_t LaunchStridePair(const void *src, void *dst, void *mid, const int *dim, const Coef7 &k, hipStream_t s) {
   using P = DiffusionPlan<S, kPeriodic>;
@@ -239,6 +241,8 @@ hipError_t LaunchStridePair(const void *src, void *dst, void *mid, const int *di
   a2.g2 = static_cast<st *>(dst);
   for (int d = 0; d < 3; ++d) a1.dim[d] = a2.dim[d] = dim[d];
   a1.k = a2.k = k;
+  const hipError_t e = psk::LaunchPackTB2Grid<P>(a1, a2, dim[0], dim[1], dim[2], s, mid != nullptr);
+  if (e != hipErrorNotSupported) return e;
   return psk::LaunchStrideTB2Grid<P>(a1, a2, dim[0], dim[1], dim[2], s, mid != nullptr);
 }
 
@@ -400,11 +404,16 @@ int PSHipDiffusion7Run(__PSGrid *g1, __PSGrid *g2, const __PSDomain *dom, const 
   const bool periodic = boundary == PS_HIP_PERIODIC;
   if (v.family == 4) {
     const Variant &one = kVariants[kStarVariant - 1];
-    // The default variant fuses rows that are not whole 16-byte vectors on
-    // the stride skeleton (whole-grid runs, rows up to 512 points).
-    const bool star_pair = PairApplies(v, g1->dim, box);
+    // The default variant fuses whole-grid runs of FP32 grids on the
+    // packed-row tile unless the rows fill whole 256-point star waves (rows of
+    // 4 to 512 points; the star tile leaves lanes idle at other widths), and
+    // rows that are not whole 16-byte vectors of 16-bit grids on the stride
+    // skeleton (rows up to 512 points).
     bool whole = true;
     for (int d = 0; d < 3; ++d) whole = whole && box.lo[d] == 0 && box.hi[d] == g1->dim[d];
+    const bool pack_first = variant == 0 && whole && storage == PS_HIP_STORAGE_NATIVE && g1->dim[0] % 256 != 0 &&
+                            g1->dim[0] >= 4 && g1->dim[0] <= 512;
+    const bool star_pair = !pack_first && PairApplies(v, g1->dim, box);
     const bool stride_pair = !star_pair && variant == 0 && whole && g1->dim[0] <= 512;
     auto pair = [&](const void *src, void *dst, void *mid) {
       return star_pair ? LaunchPairStorage(v, storage, periodic, src, dst, mid, g1->dim, k, s)

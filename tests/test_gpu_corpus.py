@@ -102,16 +102,18 @@ def test_reference_diffusion_benchmark_matches_its_cpu_baseline(tmp_path, size, 
     assert len(got) == len(want) and got == want
 
 
-RAGGED = [(37, 29, 23, 5), (130, 17, 19, 4), (383, 6, 9, 3), (510, 7, 5, 3), (19, 3, 2, 6), (64, 48, 40, 4)]
+RAGGED = [(37, 29, 23, 5), (130, 17, 19, 4), (383, 6, 9, 3), (510, 7, 5, 3), (19, 3, 2, 6), (64, 48, 40, 4),
+          (256, 9, 6, 3)]
 
 
 @pytest.mark.parametrize("periodic", [0, 1])
 @pytest.mark.parametrize("shape", RAGGED)
 def test_star_program_ragged_rows_bitwise_vs_oracle(tmp_path, shape, periodic):
     """A star-form DSL program (tests/golden/programs/star_ragged.c)
-    translated by physisc-hip, on rows that are not whole 16-byte vectors
-    (the fused pairs run on the stride skeleton) and on an aligned shape:
-    both grids equal the oracle bitwise after 2*ITER maps."""
+    translated by physisc-hip, on rows of any length (the fused pairs of
+    FP32 grids run on the packed-row tile unless the rows fill whole
+    256-point waves of the vectorised tile): both grids equal the oracle
+    bitwise after 2*ITER maps."""
     import numpy as np
     from oracle import oracle as O
     nx, ny, nz, it = shape
@@ -126,7 +128,7 @@ def test_star_program_ragged_rows_bitwise_vs_oracle(tmp_path, shape, periodic):
                                 steps=2 * it)
     assert np.array_equal(got[0], want1) and np.array_equal(got[1], want2)
     # Every pair ran fused, on the tile the row width calls for.
-    tile = "vectorised" if nx % 4 == 0 else "stride"
+    tile = "vectorised" if nx % 256 == 0 else "packed"
     assert ("Physis: fused pairs: %d passes on the %s tile" % (it, tile)) in out.stderr.decode()
 
 

@@ -97,8 +97,8 @@ def test_every_step_count_leaves_both_grids_as_the_reference(rt, shape, boundary
     """The default run fuses every pair of steps, the last pass also storing
     its intermediate step, along a path of three buffers that depends on the
     parity of the pair count; odd counts run one step first.  For each count
-    from 0 to 9 and 100/101, on rows of whole 16-byte vectors (vectorised
-    tile), ragged rows (stride tile) and 512-point rows, both grids equal the
+    from 0 to 9 and 100/101, on rows of 64, 37 and 130 points (packed-row
+    tile) and of 512 points (vectorised tile), both grids equal the
     oracle bitwise: g1 and g2 hold the last two time levels exactly as the
     reference's ping-pong leaves them."""
     field = O.uniform_field(shape, seed=11)
