@@ -152,11 +152,20 @@ BatchStarTB2(const typename P::Args *in_args, const typename P::Args *out_args, 
 // Packed mode (FP32): every grid's fused pairs on the packed-row tile
 // (psk::PackTB2Tile: rows of any length, several short rows per wave), all
 // grids in one work list and one launch; rows of up to NXMAX points.
+//
+// The workgroups take the list in rounds of gridDim.x items; within a round
+// each XCD (the hardware deals workgroups to the eight XCDs round robin)
+// takes a contiguous run (psk::XcdTile), so the items that share halo rows
+// and planes (adjacent z chunks and row tiles of one grid, adjacent in the
+// list) run at the same time on one XCD and meet in its L2.
 template <class P, int NW, int NXMAX, bool kMid>
 __global__ void __launch_bounds__(64 * NW, 8)
 BatchPackTB2(const typename P::Args *in_args, const typename P::Args *out_args, const Item *items, int nitems) {
   __shared__ psk::PackTB2Lds<NW, NXMAX> L;
-  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+  for (int base = 0; base < nitems; base += gridDim.x) {
+    const int n = min((int)gridDim.x, nitems - base);
+    if ((int)blockIdx.x >= n) break;
+    const int w = base + psk::XcdTile(blockIdx.x, n);
     const Item it = Uniform(items[w]);
     const typename P::Args a1 = Uniform(in_args[it.grid]);
     const typename P::Args a2 = Uniform(out_args[it.grid]);
@@ -621,7 +630,7 @@ std::vector<Config> Candidates(const Batch &b) {
   }
   // Packed tiles (FP32 batches).
   for (int nw : {8, 16})
-    for (int zc : {16, 32, 64}) {
+    for (int zc : {12, 16, 24}) {
       Config c{0, nw, zc, 2, 3};
       if (b.Supports(c)) out.push_back(c);
     }
