@@ -98,17 +98,24 @@ inline hipError_t LaunchRing(const typename P::Args &a, const Box &box, hipStrea
   return LaunchZMarch3D<P, (P::kDirectGrids <= 2 && kRing2 <= 64 * 1024)>(a, box, 64, true, s, parity);
 }
 
-// One step of a star plan: the vectorised skeleton when rows are whole
-// 16-byte vectors, else the LDS ring.
+// One step of a star plan: the packed-row tile for whole FP32 grids whose
+// rows do not fill 256-point waves, the vectorised skeleton when rows are
+// whole 16-byte vectors, else the stride skeleton or the LDS ring.
 // (nx, ny, nz: extents of the staged grid.)
 template <class P>
 inline hipError_t LaunchStarStep(const typename P::Args &a, const Box &box, int nx, int ny, int nz, hipStream_t s) {
+  const bool whole = box.lo[0] == 0 && box.hi[0] == nx && box.lo[1] == 0 && box.hi[1] == ny && box.lo[2] == 0 &&
+                     box.hi[2] == nz;
+  // Whole-grid steps of FP32 grids whose rows do not fill whole 256-point
+  // star waves: the packed-row tile.
+  if (whole && nx % 256 != 0) {
+    const hipError_t e = LaunchPackStepGrid<P>(a, nx, ny, nz, s);
+    if (e != hipErrorNotSupported) return e;
+  }
   if (StarSupported(nx)) return LaunchZMarchStar<P, 8>(a, box, 128, true, s);
   // Whole-grid steps with rows of up to 256 points: the stride skeleton
   // (several times the LDS ring on small and mid-size grids).
-  if (nx <= 256 && box.lo[0] == 0 && box.hi[0] == nx && box.lo[1] == 0 && box.hi[1] == ny && box.lo[2] == 0 &&
-      box.hi[2] == nz)
-    return LaunchStrideStepGrid<P>(a, nx, ny, nz, s);
+  if (nx <= 256 && whole) return LaunchStrideStepGrid<P>(a, nx, ny, nz, s);
   return LaunchRing<P, 3>(a, box, s);
 }
 

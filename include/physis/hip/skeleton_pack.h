@@ -290,6 +290,168 @@ __device__ __forceinline__ void PackTB2Tile(const typename Plan::Args &a1, const
   if (k + 1 <= ze) step(k + 1, C1{});
 }
 
+// One map on the packed tile: out = body(in) on rows [ya, ya+R) (those
+// inside the grid), planes [zb, ze), of an FP32 grid (4 <= nx <= NXMAX):
+// the level-1 half of PackTB2Tile, whose values at in-grid points are the
+// map's, with every row an output row.  The LDS layout is PackTB2Tile's u0.
+template <int NW, int NXMAX>
+struct PackStepLds {
+  static constexpr int kS0 = PackTB2Lds<NW, NXMAX>::kS0;
+  float u0[2][4 * kS0];
+};
+
+template <class Plan, int NW, int NXMAX>
+__device__ __forceinline__ void PackStepTile(const typename Plan::Args &a, int nx, int ny, int nz, int ya, int zb,
+                                             int ze, PackStepLds<NW, NXMAX> &L) {
+  static_assert(std::is_same<typename Plan::Storage, Native<float>>::value, "FP32 grids");
+  constexpr bool W = Plan::kWrap;
+  constexpr int kT = 64 * NW;
+  constexpr int kS0 = PackStepLds<NW, NXMAX>::kS0;
+  nx = __builtin_amdgcn_readfirstlane(nx);
+  ny = __builtin_amdgcn_readfirstlane(ny);
+  nz = __builtin_amdgcn_readfirstlane(nz);
+  ya = __builtin_amdgcn_readfirstlane(ya);
+  const int P = nx * ny;
+  const int64_t plane = P;
+  const int V = (nx + 3) >> 2, R = kT / V;
+  const float *in = Plan::stage_ptr(a);
+  float *out = Plan::out_ptr(a);
+  const int lane = threadIdx.x;
+  int r = lane / V;
+  const int vi = lane - r * V;
+  const bool shadow = r >= R;  // lanes past the rows shadow the last row
+  r = min(r, R - 1);
+  const int xs = min(4 * vi, nx - 4);
+  const int y = ya + r;
+  const int yf = FixIndex<W>(y, ny);
+  const int goff = (yf * nx + xs) * 4;
+  const int soff = !shadow && y < ny ? goff : kPackNoStore;
+  const int q0 = (r + 1) * V + vi;
+  const bool last = vi == V - 1;
+  int wj, wd, ej, ed;
+  if (vi == 0) {
+    wj = W ? 3 : 0;
+    wd = W ? V - 1 : 0;
+  } else {
+    wj = last ? nx - 4 * V + 3 : 3;
+    wd = -1;
+  }
+  if (last) {
+    ej = W ? 0 : 3;
+    ed = W ? -vi : 0;
+  } else {
+    ej = vi == V - 2 ? 4 * V - nx : 0;
+    ed = 1;
+  }
+  const int aw0 = wj * kS0 + q0 + wd, ae0 = ej * kS0 + q0 + ed;
+  constexpr int kHN = (2 * NXMAX + kT - 1) / kT;
+  int hoff[kHN], ha[kHN], hb[kHN];
+  {
+    const int hy_lo = FixIndex<W>(ya - 1, ny), hy_hi = FixIndex<W>(ya + R, ny);
+#pragma unroll
+    for (int j = 0; j < kHN; ++j) {
+      int t = lane + j * kT;
+      t = t < 2 * nx ? t : t % (2 * nx);
+      const bool hi = t >= nx;
+      const int x = hi ? t - nx : t;
+      hoff[j] = ((hi ? hy_hi : hy_lo) * nx + x) * 4;
+      const int base = hi ? (R + 1) * V : 0;
+      const int lastpos = base + V - 1, lastslot = x - (nx - 4);
+      hb[j] = lastslot >= 0 ? lastslot * kS0 + lastpos : (x & 3) * kS0 + base + (x >> 2);
+      ha[j] = (x >> 2) < V - 1 ? (x & 3) * kS0 + base + (x >> 2) : hb[j];
+    }
+  }
+  float hraw[kHN];
+  auto ldh = [&](int zf) {
+    const __amdgpu_buffer_rsrc_t rs = PackPlane(in, (int64_t)zf * plane, P);
+#pragma unroll
+    for (int j = 0; j < kHN; ++j)
+      hraw[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, hoff[j], 0, 0));
+  };
+  auto ld = [&](int zf, float (&v)[4]) { PackLoad(PackPlane(in, (int64_t)zf * plane, P), goff, v); };
+
+  float Q0[3][4];
+  // Loads in flight at the loop head, oldest first: the halo rows of plane
+  // z, then plane z + 1.
+  ld(FixIndex<W>(zb - 1, nz), Q0[0]);
+  ld(FixIndex<W>(zb, nz), Q0[1]);
+  __builtin_amdgcn_sched_barrier(0);
+  ldh(FixIndex<W>(zb, nz));
+  __builtin_amdgcn_sched_barrier(0);
+  ld(FixIndex<W>(zb + 1, nz), Q0[2]);
+  __builtin_amdgcn_sched_barrier(0);
+  // Wrapped/clamped indices of planes z + 1 and z + 2.
+  int z1 = FixIndex<W>(zb + 1, nz), z2 = FixIndex<W>(zb + 2, nz);
+  // Step z with role rotation rq = (z - zb) % 3: planes z-1, z, z+1 in
+  // Q0[rq], Q0[rq+1], Q0[rq+2]; plane z+2 is loaded into Q0[rq] once the
+  // point is computed.
+  auto step = [&](int z, auto rc) {
+    constexpr int rq = decltype(rc)::value;
+    constexpr int j0 = rq, j1 = (rq + 1) % 3, j2 = (rq + 2) % 3;
+    float *R0 = L.u0[z & 1];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) R0[j * kS0 + q0] = Q0[j1][j];
+#pragma unroll
+    for (int j = 0; j < kHN; ++j) {
+      R0[ha[j]] = hraw[j];
+      R0[hb[j]] = hraw[j];
+    }
+    // z1 is an in-grid index for every z: at the chunk's end the load is
+    // unused.
+    ldh(z1);
+    __syncthreads();
+    float nv[4], sv[4], o[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      nv[j] = R0[j * kS0 + q0 - V];
+      sv[j] = R0[j * kS0 + q0 + V];
+    }
+    const float wl = R0[aw0], er = R0[ae0];
+    const float (&b)[4] = Q0[j0];
+    const float (&c)[4] = Q0[j1];
+    const float (&t)[4] = Q0[j2];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      StarCtx cx;
+      cx.c = c[j];
+      cx.w = j > 0 ? c[j - 1] : wl;
+      cx.e = j < 3 ? c[j + 1] : er;
+      cx.n = nv[j];
+      cx.s = sv[j];
+      cx.b = b[j];
+      cx.t = t[j];
+      o[j] = ComputeAt<Plan, true, true, true>(cx, a, xs + j, y, z);
+    }
+    ld(z2, Q0[j0]);
+    PackStore(PackPlane(out, (int64_t)z * plane, P), soff, o);
+    z1 = z2;
+    z2 = NextPlane<W>(z2, nz);
+  };
+  using C0 = std::integral_constant<int, 0>;
+  using C1 = std::integral_constant<int, 1>;
+  using C2 = std::integral_constant<int, 2>;
+  int z = zb;
+  for (; z + 3 <= ze; z += 3) {
+    step(z, C0{});
+    step(z + 1, C1{});
+    step(z + 2, C2{});
+  }
+  if (z < ze) step(z, C0{});
+  if (z + 1 < ze) step(z + 1, C1{});
+}
+
+// One map over one whole FP32 grid on the packed tile: workgroup
+// (blockIdx.x, blockIdx.y) = (tile of PackRows rows, z chunk); tiles are
+// renumbered per XCD.
+template <class Plan, int NW, int NXMAX>
+__global__ void __launch_bounds__(64 * NW, 8)
+PackStepGrid(typename Plan::Args a, int nx, int ny, int nz, int zchunk) {
+  __shared__ PackStepLds<NW, NXMAX> L;
+  const int ty = XcdTile(blockIdx.x, gridDim.x);
+  const int zb = (int)blockIdx.y * zchunk;
+  PackStepTile<Plan, NW, NXMAX>(a, nx, ny, nz, ty * PackRows(nx, NW), zb, min(zb + zchunk, nz), L);
+}
+
 // Two fused maps over one whole FP32 grid on the packed tile: workgroup
 // (blockIdx.x, blockIdx.y) = (tile of PackRows - 2 output rows, z chunk);
 // tiles are renumbered per XCD.
@@ -366,6 +528,29 @@ inline hipError_t LaunchPackTB2Grid(const typename Plan::Args &a1, const typenam
       PSK_PACK(8, 256)
     }
 #undef PSK_PACK
+    return hipGetLastError();
+  }
+}
+
+// Host launch of PackStepGrid for an FP32 plan and rows of 4 to 512 points
+// (8 waves per workgroup up to 256 points, else 16; z chunk by
+// PackZChunk); hipErrorNotSupported for other grids and storage kinds.
+template <class Plan>
+inline hipError_t LaunchPackStepGrid(const typename Plan::Args &a, int nx, int ny, int nz, hipStream_t stream) {
+  if constexpr (!std::is_same<typename Plan::Storage, Native<float>>::value) {
+    return hipErrorNotSupported;
+  } else {
+    if (nx < 4 || nx > 512 || ny < 1 || nz < 1) return hipErrorNotSupported;
+    const bool wide = nx > 256;
+    const int nw = wide ? 16 : 8;
+    const int rows = PackRows(nx, nw);
+    const int tiles_y = (ny + rows - 1) / rows;
+    const int zchunk = PackZChunk(tiles_y, nz, nw);
+    const dim3 grid(tiles_y, (nz + zchunk - 1) / zchunk, 1), block(64 * nw, 1, 1);
+    if (wide)
+      hipLaunchKernelGGL((PackStepGrid<Plan, 16, 512>), grid, block, 0, stream, a, nx, ny, nz, zchunk);
+    else
+      hipLaunchKernelGGL((PackStepGrid<Plan, 8, 256>), grid, block, 0, stream, a, nx, ny, nz, zchunk);
     return hipGetLastError();
   }
 }

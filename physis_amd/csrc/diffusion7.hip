@@ -294,6 +294,21 @@ template <class S, bool kPeriodic>
 hipError_t LaunchOne(const Variant &v, const void *in, void *out, const int *dim, const psk::Box &box,
                      const Coef7 &k, hipStream_t s) {
   using st = typename S::stored_t;
+  if (v.family == 3) {
+    // Whole-grid steps of FP32 grids whose rows do not fill whole 256-point
+    // star waves: the packed-row tile.
+    bool whole = true;
+    for (int d = 0; d < 3; ++d) whole = whole && box.lo[d] == 0 && box.hi[d] == dim[d];
+    if (whole && dim[0] % 256 != 0) {
+      typename DiffusionPlan<S, kPeriodic>::Args a;
+      a.g1 = static_cast<const st *>(in);
+      a.g2 = static_cast<st *>(out);
+      for (int d = 0; d < 3; ++d) a.dim[d] = dim[d];
+      a.k = k;
+      const hipError_t e = psk::LaunchPackStepGrid<DiffusionPlan<S, kPeriodic>>(a, dim[0], dim[1], dim[2], s);
+      if (e != hipErrorNotSupported) return e;
+    }
+  }
   if (v.family == 3 && !psk::StarSupported(dim[0])) {
     // Rows that are not whole 16-byte vectors: whole-grid steps with rows up
     // to 256 points run on the stride skeleton (1.3-5.6x the LDS ring on

@@ -92,13 +92,14 @@ def fused_variants(rt):
 
 
 @pytest.mark.parametrize("boundary", [0, 1])
-@pytest.mark.parametrize("shape", [(64, 48, 40), (37, 29, 23), (130, 17, 19), (512, 9, 7)])
+@pytest.mark.parametrize("shape", [(64, 48, 40), (37, 29, 23), (130, 17, 19), (383, 6, 9), (512, 9, 7)])
 def test_every_step_count_leaves_both_grids_as_the_reference(rt, shape, boundary):
     """The default run fuses every pair of steps, the last pass also storing
     its intermediate step, along a path of three buffers that depends on the
     parity of the pair count; odd counts run one step first.  For each count
-    from 0 to 9 and 100/101, on rows of 64, 37 and 130 points (packed-row
-    tile) and of 512 points (vectorised tile), both grids equal the
+    from 0 to 9 and 100/101, on rows of 64, 37, 130 and 383 points
+    (packed-row tiles: fused pairs and single steps) and of 512 points
+    (vectorised tiles), both grids equal the
     oracle bitwise: g1 and g2 hold the last two time levels exactly as the
     reference's ping-pong leaves them."""
     field = O.uniform_field(shape, seed=11)
