@@ -98,16 +98,16 @@ inline hipError_t LaunchRing(const typename P::Args &a, const Box &box, hipStrea
   return LaunchZMarch3D<P, (P::kDirectGrids <= 2 && kRing2 <= 64 * 1024)>(a, box, 64, true, s, parity);
 }
 
-// One step of a star plan: the packed-row tile for whole FP32 grids whose
-// rows do not fill 256-point waves, the vectorised skeleton when rows are
+// One step of a star plan: the packed-row tile for whole FP32-valued grids
+// whose rows do not fill 256-point waves, the vectorised skeleton when rows are
 // whole 16-byte vectors, else the stride skeleton or the LDS ring.
 // (nx, ny, nz: extents of the staged grid.)
 template <class P>
 inline hipError_t LaunchStarStep(const typename P::Args &a, const Box &box, int nx, int ny, int nz, hipStream_t s) {
   const bool whole = box.lo[0] == 0 && box.hi[0] == nx && box.lo[1] == 0 && box.hi[1] == ny && box.lo[2] == 0 &&
                      box.hi[2] == nz;
-  // Whole-grid steps of FP32 grids whose rows do not fill whole 256-point
-  // star waves: the packed-row tile.
+  // Whole-grid steps of FP32-valued grids whose rows do not fill whole
+  // 256-point star waves: the packed-row tile.
   if (whole && nx % 256 != 0) {
     const hipError_t e = LaunchPackStepGrid<P>(a, nx, ny, nz, s);
     if (e != hipErrorNotSupported) return e;
@@ -129,10 +129,10 @@ inline bool FusedPairSupported(int nx, int ny, int nz) {
 
 // Two fused steps src -> dst for a star plan.  mid: also store the
 // intermediate step through out_ptr(a1).  Returns the tile that ran: 3 the
-// packed-row tile (FP32 grids with rows of 4 to 512 points, unless they fill
-// whole 256-point star waves), 1 the vectorised tile (rows of whole 16-byte
-// vectors), 2 the stride tile (other rows of 16-bit or double grids), 0 none
-// (the row width is not supported).
+// packed-row tile (FP32 values stored in 32 or 16 bits, rows of 4 to 512
+// points, unless they fill whole 256-point star waves), 1 the vectorised
+// tile (rows of whole 16-byte vectors), 2 the stride tile (other rows), 0
+// none (the row width is not supported).
 template <class P>
 inline int LaunchFusedPair(const typename P::Args &a1, const typename P::Args &a2, int nx, int ny, int nz,
                            hipStream_t s, bool mid) {

@@ -1,20 +1,22 @@
-// skeleton_pack.h -- two fused star-stencil steps on rows of any length,
-// packed across lanes (FP32 grids).
+// skeleton_pack.h -- star-stencil steps (two fused, or one) on rows of any
+// length, packed across lanes (FP32 values stored in 32 or 16 bits).
 //
 // The row-tile skeletons give a wave whole rows of a fixed width: a row of
 // nx points occupies 64*NP lane slots (stride tiles) or 256 (star tiles), so
 // rows whose length is not a multiple of that width leave lanes idle, and
 // the stride tiles move one element per lane and instruction.  Here a row
 // takes V = ceil(nx/4) lanes of four consecutive points each (one 16-byte
-// access), and a workgroup of T lanes holds R = floor(T/V) consecutive rows
-// of one z plane, packed lane after lane: lane L holds row L / V, vector
-// vi = L % V, points xs = min(4 vi, nx - 4) .. xs + 3.  The last vector of a
-// row is aligned to the row's end, so it overlaps its neighbour when nx is
-// not a multiple of four: both lanes compute the overlapped points from the
-// same values and write the same bits.  Every access stays inside its row;
-// 16-byte accesses need only 4-byte alignment on this target, so rows need
-// not start on 16 bytes.  Lanes past R*V shadow a lane of the last row (same
-// loads, same LDS words, no stores).
+// access, 8 bytes for 16-bit storage), and a workgroup of T lanes holds
+// R = floor(T/V) consecutive rows of one z plane, packed lane after lane:
+// lane L holds row L / V, vector vi = L % V, points xs = min(4 vi, nx - 4)
+// .. xs + 3.  The last vector of a row is aligned to the row's end, so it
+// overlaps its neighbour when nx is not a multiple of four: both lanes
+// compute the overlapped points from the same values and write the same
+// bits.  Every access stays inside its row; vector accesses need only
+// element alignment on this target (the runtime runs in its unaligned access
+// mode, which the compiler's own code for unaligned copies relies on), so
+// rows need not start on 16 bytes.  Lanes past R*V shadow a lane of the last
+// row (same loads, same LDS words, no stores).
 //
 // LDS holds a plane slot-major: point j of every vector in array j, at the
 // vector's position (row-major over rows and vectors).  A lane's y
@@ -73,23 +75,56 @@ __host__ __device__ inline bool PackSupported(int nx, int nw, int nxmax) {
   return nx >= 4 && nx <= nxmax && PackRows(nx, nw) >= 3;
 }
 
-// One plane of an FP32 grid as a buffer resource of P elements (the same for
-// the whole wave: broadcast from the first lane).
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t PackPlane(const float *grid, int64_t zoff, int P) {
-  return __builtin_amdgcn_make_buffer_rsrc(BufRow<Native<float>>::Uniform(grid + zoff), (short)0,
-                                           __builtin_amdgcn_readfirstlane(P * 4), kBufferWord3);
+// Storage kinds the packed tiles serve: FP32 values stored in 32 or 16 bits.
+template <class S>
+constexpr bool kPackStorage = std::is_same<typename S::value_t, float>::value;
+
+// One plane of a grid as a buffer resource of P elements (the same for the
+// whole wave: broadcast from the first lane).
+template <class st>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t PackPlane(const st *grid, int64_t zoff, int P) {
+  const uint64_t a = reinterpret_cast<uint64_t>(grid + zoff);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(((uint64_t)hi << 32) | lo), (short)0,
+                                           __builtin_amdgcn_readfirstlane(P * (int)sizeof(st)), kBufferWord3);
 }
 
+// Four consecutive points (16 bytes, or 8 for 16-bit storage, at any
+// element-aligned offset) as FP32.
+template <class S>
 __device__ __forceinline__ void PackLoad(__amdgpu_buffer_rsrc_t r, int byte_off, float (&v)[4]) {
-  const v4f_t q = __builtin_bit_cast(v4f_t, __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0));
-  v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+  if constexpr (sizeof(typename S::stored_t) == 4) {
+    const v4f_t q = __builtin_bit_cast(v4f_t, __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0));
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+  } else {
+    Widen4<S>(__builtin_bit_cast(v2u_t, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0)), v);
+  }
 }
 
-// Non-temporal 16-byte store; dwords past the resource's range are dropped.
+// One point as FP32.
+template <class S>
+__device__ __forceinline__ float PackLoad1(__amdgpu_buffer_rsrc_t r, int byte_off) {
+  if constexpr (sizeof(typename S::stored_t) == 4)
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+  else
+    return S::widen(__builtin_amdgcn_raw_buffer_load_b16(r, byte_off, 0, 0));
+}
+
+// Four points, non-temporal (16-bit storage: rounded to nearest even in
+// pairs); dwords past the resource's range are dropped.
+template <class S>
 __device__ __forceinline__ void PackStore(__amdgpu_buffer_rsrc_t r, int byte_off, const float (&v)[4]) {
   typedef uint32_t v4u_t __attribute__((ext_vector_type(4)));
-  const v4f_t q = {v[0], v[1], v[2], v[3]};
-  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u_t, q), r, byte_off, 0, 2);
+  if constexpr (sizeof(typename S::stored_t) == 4) {
+    const v4f_t q = {v[0], v[1], v[2], v[3]};
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u_t, q), r, byte_off, 0, 2);
+  } else {
+    v2u_t q;
+    q.x = S::narrow2(v[0], v[1]);
+    q.y = S::narrow2(v[2], v[3]);
+    __builtin_amdgcn_raw_buffer_store_b64(q, r, byte_off, 0, 2);
+  }
 }
 
 // A byte offset past any plane's range (planes here hold < 2^31 bytes): a
@@ -104,7 +139,9 @@ constexpr int kPackNoStore = 0x7ffffff0;
 template <class Plan, int NW, int NXMAX, bool kMid = false>
 __device__ __forceinline__ void PackTB2Tile(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx,
                                             int ny, int nz, int ya, int zb, int ze, PackTB2Lds<NW, NXMAX> &L) {
-  static_assert(std::is_same<typename Plan::Storage, Native<float>>::value, "FP32 grids");
+  using S = typename Plan::Storage;
+  using st = typename S::stored_t;
+  static_assert(kPackStorage<S>, "FP32 values");
   constexpr bool W = Plan::kWrap;
   constexpr int kT = 64 * NW;
   constexpr int kS0 = PackTB2Lds<NW, NXMAX>::kS0, kS1 = PackTB2Lds<NW, NXMAX>::kS1;
@@ -115,8 +152,8 @@ __device__ __forceinline__ void PackTB2Tile(const typename Plan::Args &a1, const
   const int P = nx * ny;
   const int64_t plane = P;
   const int V = (nx + 3) >> 2, R = kT / V;
-  const float *in = Plan::stage_ptr(a1);
-  float *out = Plan::out_ptr(a2);
+  const st *in = Plan::stage_ptr(a1);
+  st *out = Plan::out_ptr(a2);
   const int lane = threadIdx.x;
   int r = lane / V;
   const int vi = lane - r * V;
@@ -124,7 +161,7 @@ __device__ __forceinline__ void PackTB2Tile(const typename Plan::Args &a1, const
   const int xs = min(4 * vi, nx - 4);
   const int y = ya + r;
   const int yf = FixIndex<W>(y, ny);
-  const int goff = (yf * nx + xs) * 4;  // byte offset in a plane
+  const int goff = (yf * nx + xs) * (int)sizeof(st);  // byte offset in a plane
   const bool out_row = r >= 1 && r <= R - 2 && y < ny;
   const int soff = out_row ? goff : kPackNoStore;
   // Positions: u0 rows at (r + 1) V (the halo rows at 0 and (R + 1) V), u1
@@ -170,7 +207,7 @@ __device__ __forceinline__ void PackTB2Tile(const typename Plan::Args &a1, const
       t = t < 2 * nx ? t : t % (2 * nx);
       const bool hi = t >= nx;
       const int x = hi ? t - nx : t;
-      hoff[j] = ((hi ? hy_hi : hy_lo) * nx + x) * 4;
+      hoff[j] = ((hi ? hy_hi : hy_lo) * nx + x) * (int)sizeof(st);
       const int base = hi ? (R + 1) * V : 0;
       const int lastpos = base + V - 1, lastslot = x - (nx - 4);
       hb[j] = lastslot >= 0 ? lastslot * kS0 + lastpos : (x & 3) * kS0 + base + (x >> 2);
@@ -182,7 +219,7 @@ __device__ __forceinline__ void PackTB2Tile(const typename Plan::Args &a1, const
     const __amdgpu_buffer_rsrc_t rs = PackPlane(in, (int64_t)zf * plane, P);
 #pragma unroll
     for (int j = 0; j < kHN; ++j)
-      hraw[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, hoff[j], 0, 0));
+      hraw[j] = PackLoad1<S>(rs, hoff[j]);
   };
   auto puth = [&](float *R0) {
 #pragma unroll
@@ -191,7 +228,7 @@ __device__ __forceinline__ void PackTB2Tile(const typename Plan::Args &a1, const
       R0[hb[j]] = hraw[j];
     }
   };
-  auto ld = [&](int zf, float (&v)[4]) { PackLoad(PackPlane(in, (int64_t)zf * plane, P), goff, v); };
+  auto ld = [&](int zf, float (&v)[4]) { PackLoad<S>(PackPlane(in, (int64_t)zf * plane, P), goff, v); };
   // The lane's four points into the slot arrays (stride S) at position q.
   auto put = [&](float *Rb, int q, int S, const float (&v)[4]) {
 #pragma unroll
@@ -221,6 +258,9 @@ __device__ __forceinline__ void PackTB2Tile(const typename Plan::Args &a1, const
       cx.t = t[j];
       u[j] = ComputeAt<Plan, true, true, true>(cx, a1, xs + j, yf, kf);
     }
+    // 16-bit grids: the intermediate level holds what a separate step would
+    // store and load back.
+    RoundTripN<S>(u);
   };
   // u2 of the lane's points at plane k - 1 from u1 planes b, c, t.
   auto level2 = [&](const float *R1, int k, const float (&b)[4], const float (&c)[4], const float (&t)[4]) {
@@ -240,7 +280,7 @@ __device__ __forceinline__ void PackTB2Tile(const typename Plan::Args &a1, const
       cx.t = t[j];
       o[j] = ComputeAt<Plan, true, true, false>(cx, a2, xs + j, y, k - 1);
     }
-    PackStore(PackPlane(out, (int64_t)(k - 1) * plane, P), soff, o);
+    PackStore<S>(PackPlane(out, (int64_t)(k - 1) * plane, P), soff, o);
   };
 
   float Q0[3][4], Q1[3][4] = {};
@@ -269,7 +309,7 @@ __device__ __forceinline__ void PackTB2Tile(const typename Plan::Args &a1, const
     __syncthreads();
     level1(R0, z0, Q0[j0], Q0[j1], Q0[j2], Q1[j2]);
     if constexpr (kMid) {
-      if (k >= zb && k < ze) PackStore(PackPlane(Plan::out_ptr(a1), (int64_t)k * plane, P), soff, Q1[j2]);
+      if (k >= zb && k < ze) PackStore<S>(PackPlane(Plan::out_ptr(a1), (int64_t)k * plane, P), soff, Q1[j2]);
     }
     ld(z2, Q0[j0]);
     if (k - 1 >= zb) level2(R1, k, Q1[j0], Q1[j1], Q1[j2]);
@@ -303,7 +343,9 @@ struct PackStepLds {
 template <class Plan, int NW, int NXMAX>
 __device__ __forceinline__ void PackStepTile(const typename Plan::Args &a, int nx, int ny, int nz, int ya, int zb,
                                              int ze, PackStepLds<NW, NXMAX> &L) {
-  static_assert(std::is_same<typename Plan::Storage, Native<float>>::value, "FP32 grids");
+  using S = typename Plan::Storage;
+  using st = typename S::stored_t;
+  static_assert(kPackStorage<S>, "FP32 values");
   constexpr bool W = Plan::kWrap;
   constexpr int kT = 64 * NW;
   constexpr int kS0 = PackStepLds<NW, NXMAX>::kS0;
@@ -314,8 +356,8 @@ __device__ __forceinline__ void PackStepTile(const typename Plan::Args &a, int n
   const int P = nx * ny;
   const int64_t plane = P;
   const int V = (nx + 3) >> 2, R = kT / V;
-  const float *in = Plan::stage_ptr(a);
-  float *out = Plan::out_ptr(a);
+  const st *in = Plan::stage_ptr(a);
+  st *out = Plan::out_ptr(a);
   const int lane = threadIdx.x;
   int r = lane / V;
   const int vi = lane - r * V;
@@ -324,7 +366,7 @@ __device__ __forceinline__ void PackStepTile(const typename Plan::Args &a, int n
   const int xs = min(4 * vi, nx - 4);
   const int y = ya + r;
   const int yf = FixIndex<W>(y, ny);
-  const int goff = (yf * nx + xs) * 4;
+  const int goff = (yf * nx + xs) * (int)sizeof(st);
   const int soff = !shadow && y < ny ? goff : kPackNoStore;
   const int q0 = (r + 1) * V + vi;
   const bool last = vi == V - 1;
@@ -354,7 +396,7 @@ __device__ __forceinline__ void PackStepTile(const typename Plan::Args &a, int n
       t = t < 2 * nx ? t : t % (2 * nx);
       const bool hi = t >= nx;
       const int x = hi ? t - nx : t;
-      hoff[j] = ((hi ? hy_hi : hy_lo) * nx + x) * 4;
+      hoff[j] = ((hi ? hy_hi : hy_lo) * nx + x) * (int)sizeof(st);
       const int base = hi ? (R + 1) * V : 0;
       const int lastpos = base + V - 1, lastslot = x - (nx - 4);
       hb[j] = lastslot >= 0 ? lastslot * kS0 + lastpos : (x & 3) * kS0 + base + (x >> 2);
@@ -366,9 +408,9 @@ __device__ __forceinline__ void PackStepTile(const typename Plan::Args &a, int n
     const __amdgpu_buffer_rsrc_t rs = PackPlane(in, (int64_t)zf * plane, P);
 #pragma unroll
     for (int j = 0; j < kHN; ++j)
-      hraw[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, hoff[j], 0, 0));
+      hraw[j] = PackLoad1<S>(rs, hoff[j]);
   };
-  auto ld = [&](int zf, float (&v)[4]) { PackLoad(PackPlane(in, (int64_t)zf * plane, P), goff, v); };
+  auto ld = [&](int zf, float (&v)[4]) { PackLoad<S>(PackPlane(in, (int64_t)zf * plane, P), goff, v); };
 
   float Q0[3][4];
   // Loads in flight at the loop head, oldest first: the halo rows of plane
@@ -423,7 +465,7 @@ __device__ __forceinline__ void PackStepTile(const typename Plan::Args &a, int n
       o[j] = ComputeAt<Plan, true, true, true>(cx, a, xs + j, y, z);
     }
     ld(z2, Q0[j0]);
-    PackStore(PackPlane(out, (int64_t)z * plane, P), soff, o);
+    PackStore<S>(PackPlane(out, (int64_t)z * plane, P), soff, o);
     z1 = z2;
     z2 = NextPlane<W>(z2, nz);
   };
@@ -499,14 +541,15 @@ inline int PackZChunk(int tiles_y, int nz, int nw) {
   return best;
 }
 
-// Host launch of PackTB2Grid for an FP32 plan and rows of 4 to 512 points
+// Host launch of PackTB2Grid for a plan of FP32 values (stored in 32 or 16
+// bits) and rows of 4 to 512 points
 // (8 waves per workgroup up to 256 points, else 16), z chunk by PackZChunk;
 // hipErrorNotSupported for other grids and storage kinds.
 // mid: also store the intermediate level through out_ptr(a1).
 template <class Plan>
 inline hipError_t LaunchPackTB2Grid(const typename Plan::Args &a1, const typename Plan::Args &a2, int nx, int ny,
                                     int nz, hipStream_t stream, bool mid = false) {
-  if constexpr (!std::is_same<typename Plan::Storage, Native<float>>::value) {
+  if constexpr (!kPackStorage<typename Plan::Storage>) {
     return hipErrorNotSupported;
   } else {
     if (nx < 4 || nx > 512 || ny < 1 || nz < 1) return hipErrorNotSupported;
@@ -532,12 +575,13 @@ inline hipError_t LaunchPackTB2Grid(const typename Plan::Args &a1, const typenam
   }
 }
 
-// Host launch of PackStepGrid for an FP32 plan and rows of 4 to 512 points
+// Host launch of PackStepGrid for a plan of FP32 values (stored in 32 or 16
+// bits) and rows of 4 to 512 points
 // (8 waves per workgroup up to 256 points, else 16; z chunk by
 // PackZChunk); hipErrorNotSupported for other grids and storage kinds.
 template <class Plan>
 inline hipError_t LaunchPackStepGrid(const typename Plan::Args &a, int nx, int ny, int nz, hipStream_t stream) {
-  if constexpr (!std::is_same<typename Plan::Storage, Native<float>>::value) {
+  if constexpr (!kPackStorage<typename Plan::Storage>) {
     return hipErrorNotSupported;
   } else {
     if (nx < 4 || nx > 512 || ny < 1 || nz < 1) return hipErrorNotSupported;

@@ -149,7 +149,7 @@ BatchStarTB2(const typename P::Args *in_args, const typename P::Args *out_args, 
   }
 }
 
-// Packed mode (FP32): every grid's fused pairs on the packed-row tile
+// Packed mode (FP32 and BF16 storage): every grid's fused pairs on the packed-row tile
 // (psk::PackTB2Tile: rows of any length, several short rows per wave), all
 // grids in one work list and one launch; rows of up to NXMAX points.
 //
@@ -190,8 +190,8 @@ struct Config {
   // (their kernels keep full occupancy at 16 rows, which halves the share
   // of halo rows).
   // 3: packed: the fused pairs of every grid on the packed-row tile in one
-  // launch (`by` = waves per workgroup; FP32 batches of rows of 4 to 512
-  // points), single steps as in mode 1.
+  // launch (`by` = waves per workgroup; rows of 4 to 512 points), single
+  // steps as in mode 1.
   int mixed = 0;
   bool operator<(const Config &o) const {
     return std::tie(np, by, zchunk, degree, mixed) < std::tie(o.np, o.by, o.zchunk, o.degree, o.mixed);
@@ -267,19 +267,25 @@ KernelFn TB2Kernel(int storage, bool per, int np, int by, bool mid = false) {
              : TB2KernelFor<psk::Native<float>, false>(np, by, mid);
 }
 
-// Packed fused kernel (FP32 only): nw waves, rows up to nxmax points.
-KernelFn PackKernel(int storage, bool per, int nw, int nxmax, bool mid) {
-  if (storage != PS_HIP_STORAGE_NATIVE) return nullptr;
-#define PSH_K(NW, NXMAX)                                                                                       \
-  if (nw == NW && nxmax == NXMAX) {                                                                            \
-    if (per)                                                                                                   \
-      return mid ? reinterpret_cast<KernelFn>(&BatchPackTB2<BatchPlan<psk::Native<float>, true>, NW, NXMAX, true>) \
-                 : reinterpret_cast<KernelFn>(&BatchPackTB2<BatchPlan<psk::Native<float>, true>, NW, NXMAX, false>); \
-    return mid ? reinterpret_cast<KernelFn>(&BatchPackTB2<BatchPlan<psk::Native<float>, false>, NW, NXMAX, true>)  \
-               : reinterpret_cast<KernelFn>(&BatchPackTB2<BatchPlan<psk::Native<float>, false>, NW, NXMAX, false>); \
-  }
+// Packed fused kernel (FP32 and BF16 storage): nw waves, rows up to nxmax
+// points.
+template <class S, bool kPer>
+KernelFn PackKernelT(int nw, int nxmax, bool mid) {
+  using P = BatchPlan<S, kPer>;
+#define PSH_K(NW, NXMAX)                                                                          \
+  if (nw == NW && nxmax == NXMAX)                                                                 \
+    return mid ? reinterpret_cast<KernelFn>(&BatchPackTB2<P, NW, NXMAX, true>)                    \
+               : reinterpret_cast<KernelFn>(&BatchPackTB2<P, NW, NXMAX, false>);
   PSH_K(8, 256) PSH_K(16, 256) PSH_K(16, 512)
 #undef PSH_K
+  return nullptr;
+}
+KernelFn PackKernel(int storage, bool per, int nw, int nxmax, bool mid) {
+  if (storage == PS_HIP_STORAGE_NATIVE)
+    return per ? PackKernelT<psk::Native<float>, true>(nw, nxmax, mid)
+               : PackKernelT<psk::Native<float>, false>(nw, nxmax, mid);
+  if (storage == PS_HIP_STORAGE_BF16)
+    return per ? PackKernelT<psk::BF16, true>(nw, nxmax, mid) : PackKernelT<psk::BF16, false>(nw, nxmax, mid);
   return nullptr;
 }
 
@@ -628,7 +634,7 @@ std::vector<Config> Candidates(const Batch &b) {
       if (b.Supports(c)) out.push_back(c);
     }
   }
-  // Packed tiles (FP32 batches).
+  // Packed tiles.
   for (int nw : {8, 16})
     for (int zc : {12, 16, 24}) {
       Config c{0, nw, zc, 2, 3};
@@ -646,8 +652,8 @@ std::vector<Config> Candidates(const Batch &b) {
   return out;
 }
 
-// Before tuning: the packed mode where it serves the batch (FP32 rows of 4
-// to 512 points), else one width for all grids.
+// Before tuning: the packed mode where it serves the batch (rows of 4 to
+// 512 points), else one width for all grids.
 Config DefaultConfig(const Batch &b) {
   for (int nw : {8, 16}) {
     const Config c{0, nw, 16, 2, 3};
@@ -751,8 +757,8 @@ int PSHipBatchRun(void *h, int steps, int use_graph) {
 // Sets the launch configuration explicitly (np, by, zchunk, degree); np = 0
 // selects the mixed mode (each grid in its class, classes on their own
 // streams), np = -1 the mixed mode with 16-row tiles for the 1- and 2-point
-// classes, np = -2 the packed mode with `by` waves per workgroup (FP32
-// batches; fused pairs only, degree 2).
+// classes, np = -2 the packed mode with `by` waves per workgroup (fused
+// pairs only, degree 2).
 int PSHipBatchConfigure(void *h, int np, int by, int zchunk, int degree) {
   if (!h) return PS_HIP_ERR_INVALID;
   Batch *b = static_cast<Batch *>(h);

@@ -227,9 +227,9 @@ hipError_t LaunchPair(const Variant &v, const void *src, void *dst, void *mid, c
   return hipErrorInvalidValue;
 }
 
-// A fused pair for whole-grid runs whose rows are not whole 16-byte vectors
-// (rows of any length up to 512 points): the packed-row tile for FP32 grids,
-// the stride tile for 16-bit storage.
+// A fused pair for whole-grid runs whose rows do not fill whole 256-point
+// star waves (rows of any length up to 512 points): the packed-row tile
+// (rows of at least 4 points), else the stride tile.
 template <class S, bool kPeriodic>
 hipError_t LaunchStridePair(const void *src, void *dst, void *mid, const int *dim, const Coef7 &k, hipStream_t s) {
   using P = DiffusionPlan<S, kPeriodic>;
@@ -295,8 +295,8 @@ hipError_t LaunchOne(const Variant &v, const void *in, void *out, const int *dim
                      const Coef7 &k, hipStream_t s) {
   using st = typename S::stored_t;
   if (v.family == 3) {
-    // Whole-grid steps of FP32 grids whose rows do not fill whole 256-point
-    // star waves: the packed-row tile.
+    // Whole-grid steps whose rows do not fill whole 256-point star waves:
+    // the packed-row tile.
     bool whole = true;
     for (int d = 0; d < 3; ++d) whole = whole && box.lo[d] == 0 && box.hi[d] == dim[d];
     if (whole && dim[0] % 256 != 0) {
@@ -419,15 +419,12 @@ int PSHipDiffusion7Run(__PSGrid *g1, __PSGrid *g2, const __PSDomain *dom, const 
   const bool periodic = boundary == PS_HIP_PERIODIC;
   if (v.family == 4) {
     const Variant &one = kVariants[kStarVariant - 1];
-    // The default variant fuses whole-grid runs of FP32 grids on the
-    // packed-row tile unless the rows fill whole 256-point star waves (rows of
-    // 4 to 512 points; the star tile leaves lanes idle at other widths), and
-    // rows that are not whole 16-byte vectors of 16-bit grids on the stride
-    // skeleton (rows up to 512 points).
+    // The default variant fuses whole-grid runs on the packed-row tile
+    // unless the rows fill whole 256-point star waves (rows of 4 to 512
+    // points; the star tile leaves lanes idle at other widths).
     bool whole = true;
     for (int d = 0; d < 3; ++d) whole = whole && box.lo[d] == 0 && box.hi[d] == g1->dim[d];
-    const bool pack_first = variant == 0 && whole && storage == PS_HIP_STORAGE_NATIVE && g1->dim[0] % 256 != 0 &&
-                            g1->dim[0] >= 4 && g1->dim[0] <= 512;
+    const bool pack_first = variant == 0 && whole && g1->dim[0] % 256 != 0 && g1->dim[0] >= 4 && g1->dim[0] <= 512;
     const bool star_pair = !pack_first && PairApplies(v, g1->dim, box);
     const bool stride_pair = !star_pair && variant == 0 && whole && g1->dim[0] <= 512;
     auto pair = [&](const void *src, void *dst, void *mid) {

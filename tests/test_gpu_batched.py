@@ -180,6 +180,40 @@ def test_packed_batch_bitwise_vs_oracle(rt, boundary, cfg, shapes, steps, graph)
         g2.free()
 
 
+@pytest.mark.parametrize("cfg,shapes", [((-2, 8, 16, 2), PACKED), ((-2, 16, 12, 2), PACKED_WIDE)])
+@pytest.mark.parametrize("boundary", [0, 1])
+@pytest.mark.parametrize("steps", [6, 9])
+def test_bf16_packed_batch_equals_single_grid_runs(rt, cfg, shapes, boundary, steps):
+    """BF16 storage on the packed mode: each grid ends bitwise equal to the
+    same grid advanced alone by the plain single-grid run (both round to
+    BF16 at every step and evaluate the body with the same fused
+    multiply-adds), for rows of 4 to 512 points."""
+    rng = np.random.default_rng(17)
+    batch_pairs, solo_pairs, coefs = [], [], []
+    for i, shp in enumerate(shapes):
+        f = O.uniform_field(shp, seed=900 + i)
+        nb = rng.uniform(0.02, 0.15, 6).astype(np.float32)
+        coefs.append([float(v) for v in nb] + [float(np.float32(1.0) - np.float32(nb.sum()))])
+        for pairs in (batch_pairs, solo_pairs):
+            g1, g2 = rt.Grid(shp, storage=rt.STORAGE_BF16), rt.Grid(shp, storage=rt.STORAGE_BF16)
+            g1.copyin(f)
+            g2.copyin(np.zeros_like(f))
+            pairs.append((g1, g2))
+    batch = rt.Batch(batch_pairs, coefs, boundary)
+    batch.configure(*cfg)
+    print("config", batch.config_name(), "steps", steps, flush=True)
+    batch.run(steps, graph=True)
+    for (s1, s2), c in zip(solo_pairs, coefs):
+        rt.diffusion7_run(s1, s2, c, boundary, steps)
+    for (b1, b2), (s1, s2) in zip(batch_pairs, solo_pairs):
+        assert np.array_equal(b1.copyout(), s1.copyout()), "grid %s" % (b1.shape,)
+        assert np.array_equal(b2.copyout(), s2.copyout()), "grid %s" % (b2.shape,)
+    batch.destroy()
+    for g1, g2 in batch_pairs + solo_pairs:
+        g1.free()
+        g2.free()
+
+
 def test_packed_mode_rejects_rows_it_does_not_serve(rt):
     # Rows shorter than 4 points, rows over 256 points for 8 waves per
     # workgroup, and one step per launch are not served by the packed mode.
