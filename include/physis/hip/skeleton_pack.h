@@ -127,6 +127,72 @@ __device__ __forceinline__ void PackStore(__amdgpu_buffer_rsrc_t r, int byte_off
   }
 }
 
+// Where lane `lane` of a packed workgroup of 64*nw lanes works on rows of
+// nx points (V = ceil(nx/4) lanes per row, R = floor(64 nw / V) rows):
+// row r (a lane past the R rows shadows lane (R-1, vi)), vector vi, first
+// point xs; and the LDS words of the x neighbours across its vector's ends,
+// each as (slot array, position offset from the vector's own position):
+// west of point xs (wj, wd) and east of point xs + 3 (ej, ed) -- the
+// neighbouring vector's adjacent point, or at a row end the point itself
+// (clamped) or the row's other end (wrap).
+struct PackLane {
+  int r, vi, xs, shadow;
+  int wj, wd, ej, ed;
+};
+__host__ __device__ inline PackLane PackLaneOf(int lane, int nx, int nw, bool wrap) {
+  const int V = (nx + 3) >> 2, R = 64 * nw / V;
+  PackLane p;
+  int r = lane / V;
+  p.vi = lane - r * V;
+  p.shadow = r >= R;
+  p.r = r < R ? r : R - 1;
+  p.xs = 4 * p.vi < nx - 4 ? 4 * p.vi : nx - 4;
+  const bool last = p.vi == V - 1;
+  if (p.vi == 0) {
+    p.wj = wrap ? 3 : 0;
+    p.wd = wrap ? V - 1 : 0;  // wrap: the row's last point; clamped: itself
+  } else {
+    p.wj = last ? nx - 4 * V + 3 : 3;  // point xs - 1 in vector vi - 1
+    p.wd = -1;
+  }
+  if (last) {
+    p.ej = wrap ? 0 : 3;
+    p.ed = wrap ? -p.vi : 0;  // wrap: the row's first point; clamped: itself
+  } else {
+    p.ej = p.vi == V - 2 ? 4 * V - nx : 0;  // point xs + 4 in vector vi + 1
+    p.ed = 1;
+  }
+  return p;
+}
+
+// The two LDS places (slot array, position) of point x of a halo row whose
+// vectors start at position `base` (rows of nx points, V vectors): its
+// vector's slot, and where a row's last vector (aligned to the row's end)
+// holds it too; the same place twice if the point is in one vector only.
+struct PackHalo {
+  int ja, pa, jb, pb;
+};
+__host__ __device__ inline PackHalo PackHaloOf(int x, int nx, int base) {
+  const int V = (nx + 3) >> 2;
+  PackHalo h;
+  const int lastslot = x - (nx - 4);
+  if (lastslot >= 0) {
+    h.jb = lastslot;
+    h.pb = base + V - 1;
+  } else {
+    h.jb = x & 3;
+    h.pb = base + (x >> 2);
+  }
+  if ((x >> 2) < V - 1) {
+    h.ja = x & 3;
+    h.pa = base + (x >> 2);
+  } else {
+    h.ja = h.jb;
+    h.pa = h.pb;
+  }
+  return h;
+}
+
 // A byte offset past any plane's range (planes here hold < 2^31 bytes): a
 // lane that writes nothing stores there, and the range check drops it, so
 // the store needs no branch.
@@ -155,10 +221,8 @@ __device__ __forceinline__ void PackTB2Tile(const typename Plan::Args &a1, const
   const st *in = Plan::stage_ptr(a1);
   st *out = Plan::out_ptr(a2);
   const int lane = threadIdx.x;
-  int r = lane / V;
-  const int vi = lane - r * V;
-  r = min(r, R - 1);  // lanes past the rows shadow the last row
-  const int xs = min(4 * vi, nx - 4);
+  const PackLane pl = PackLaneOf(lane, nx, NW, W);
+  const int r = pl.r, vi = pl.vi, xs = pl.xs;
   const int y = ya + r;
   const int yf = FixIndex<W>(y, ny);
   const int goff = (yf * nx + xs) * (int)sizeof(st);  // byte offset in a plane
@@ -171,24 +235,7 @@ __device__ __forceinline__ void PackTB2Tile(const typename Plan::Args &a1, const
   // nothing read their own row too (in range).
   const int qn = (!out_row || (!W && y == 0)) ? q1 : q1 - V;
   const int qs = (!out_row || (!W && y == ny - 1)) ? q1 : q1 + V;
-  // The x neighbours across the vector's ends, as (array, position offset):
-  // west of point xs, east of point xs + 3.
-  const bool last = vi == V - 1;
-  int wj, wd, ej, ed;
-  if (vi == 0) {
-    wj = W ? 3 : 0;
-    wd = W ? V - 1 : 0;  // periodic: the row's last point; clamped: itself
-  } else {
-    wj = last ? nx - 4 * V + 3 : 3;  // point xs - 1 in vector vi - 1
-    wd = -1;
-  }
-  if (last) {
-    ej = W ? 0 : 3;
-    ed = W ? -vi : 0;  // periodic: the row's first point; clamped: itself
-  } else {
-    ej = vi == V - 2 ? 4 * V - nx : 0;  // point xs + 4 in vector vi + 1
-    ed = 1;
-  }
+  const int wj = pl.wj, wd = pl.wd, ej = pl.ej, ed = pl.ed;
   const int aw0 = wj * kS0 + q0 + wd, ae0 = ej * kS0 + q0 + ed;
   const int aw1 = wj * kS1 + q1 + wd, ae1 = ej * kS1 + q1 + ed;
 
@@ -208,10 +255,9 @@ __device__ __forceinline__ void PackTB2Tile(const typename Plan::Args &a1, const
       const bool hi = t >= nx;
       const int x = hi ? t - nx : t;
       hoff[j] = ((hi ? hy_hi : hy_lo) * nx + x) * (int)sizeof(st);
-      const int base = hi ? (R + 1) * V : 0;
-      const int lastpos = base + V - 1, lastslot = x - (nx - 4);
-      hb[j] = lastslot >= 0 ? lastslot * kS0 + lastpos : (x & 3) * kS0 + base + (x >> 2);
-      ha[j] = (x >> 2) < V - 1 ? (x & 3) * kS0 + base + (x >> 2) : hb[j];
+      const PackHalo h = PackHaloOf(x, nx, hi ? (R + 1) * V : 0);
+      ha[j] = h.ja * kS0 + h.pa;
+      hb[j] = h.jb * kS0 + h.pb;
     }
   }
   float hraw[kHN];
@@ -359,32 +405,14 @@ __device__ __forceinline__ void PackStepTile(const typename Plan::Args &a, int n
   const st *in = Plan::stage_ptr(a);
   st *out = Plan::out_ptr(a);
   const int lane = threadIdx.x;
-  int r = lane / V;
-  const int vi = lane - r * V;
-  const bool shadow = r >= R;  // lanes past the rows shadow the last row
-  r = min(r, R - 1);
-  const int xs = min(4 * vi, nx - 4);
+  const PackLane pl = PackLaneOf(lane, nx, NW, W);
+  const int r = pl.r, vi = pl.vi, xs = pl.xs;
   const int y = ya + r;
   const int yf = FixIndex<W>(y, ny);
   const int goff = (yf * nx + xs) * (int)sizeof(st);
-  const int soff = !shadow && y < ny ? goff : kPackNoStore;
+  const int soff = !pl.shadow && y < ny ? goff : kPackNoStore;
   const int q0 = (r + 1) * V + vi;
-  const bool last = vi == V - 1;
-  int wj, wd, ej, ed;
-  if (vi == 0) {
-    wj = W ? 3 : 0;
-    wd = W ? V - 1 : 0;
-  } else {
-    wj = last ? nx - 4 * V + 3 : 3;
-    wd = -1;
-  }
-  if (last) {
-    ej = W ? 0 : 3;
-    ed = W ? -vi : 0;
-  } else {
-    ej = vi == V - 2 ? 4 * V - nx : 0;
-    ed = 1;
-  }
+  const int wj = pl.wj, wd = pl.wd, ej = pl.ej, ed = pl.ed;
   const int aw0 = wj * kS0 + q0 + wd, ae0 = ej * kS0 + q0 + ed;
   constexpr int kHN = (2 * NXMAX + kT - 1) / kT;
   int hoff[kHN], ha[kHN], hb[kHN];
@@ -397,10 +425,9 @@ __device__ __forceinline__ void PackStepTile(const typename Plan::Args &a, int n
       const bool hi = t >= nx;
       const int x = hi ? t - nx : t;
       hoff[j] = ((hi ? hy_hi : hy_lo) * nx + x) * (int)sizeof(st);
-      const int base = hi ? (R + 1) * V : 0;
-      const int lastpos = base + V - 1, lastslot = x - (nx - 4);
-      hb[j] = lastslot >= 0 ? lastslot * kS0 + lastpos : (x & 3) * kS0 + base + (x >> 2);
-      ha[j] = (x >> 2) < V - 1 ? (x & 3) * kS0 + base + (x >> 2) : hb[j];
+      const PackHalo h = PackHaloOf(x, nx, hi ? (R + 1) * V : 0);
+      ha[j] = h.ja * kS0 + h.pa;
+      hb[j] = h.jb * kS0 + h.pb;
     }
   }
   float hraw[kHN];

@@ -121,6 +121,13 @@ extern int PSHipDiffusion7Run(__PSGrid *g1, __PSGrid *g2,
 /* Number of kernel variants and a short name for each (1-based ids). */
 extern int PSHipDiffusion7Variants(void);
 extern const char *PSHipDiffusion7VariantName(int variant);
+/* Diagnostic (host only, no reference counterpart): the packed-row tiles'
+ * lane and halo geometry for rows of nx points and 64*nw-lane workgroups.
+ * lane_out: 8 ints per lane (row, vector, first point, shadow flag, and the
+ * slot and position offset of the x neighbours across the vector's ends);
+ * halo_out: 4 ints per point of a halo row (two slot/position places).
+ * Returns the rows per workgroup. */
+extern int PSHipPackGeometry(int nx, int nw, int periodic, int *lane_out, int *halo_out);
 
 /* Himeno 19-point Jacobi sweep (reference examples/himeno/himenobmtxpa_physis.c:331-361).
  * grids[] = {a0,a1,a2,a3,b0,b1,b2,c0,c1,c2,bnd,wrk1}; p0/p1 ping-pong.

@@ -390,6 +390,30 @@ extern "C" {
 
 int PSHipDiffusion7Variants(void) { return kNumVariants; }
 
+// The packed-row tiles' geometry for rows of nx points and workgroups of
+// 64*nw lanes (psk::PackLaneOf, psk::PackHaloOf): lane_out[8*lane ..] =
+// r, vi, xs, shadow, wj, wd, ej, ed for every lane; halo_out[4*x ..] = the
+// two (slot, position) places of point x of a halo row whose vectors start
+// at position 0.  Returns the rows per workgroup, or PS_HIP_ERR_INVALID.
+int PSHipPackGeometry(int nx, int nw, int periodic, int *lane_out, int *halo_out) {
+  if (nx < 4 || nw < 1 || nw > 16 || !lane_out || !halo_out) return PS_HIP_ERR_INVALID;
+  const int R = psk::PackRows(nx, nw);
+  if (R < 1) return PS_HIP_ERR_INVALID;
+  for (int lane = 0; lane < 64 * nw; ++lane) {
+    const psk::PackLane p = psk::PackLaneOf(lane, nx, nw, periodic != 0);
+    const int v[8] = {p.r, p.vi, p.xs, p.shadow, p.wj, p.wd, p.ej, p.ed};
+    for (int k = 0; k < 8; ++k) lane_out[8 * lane + k] = v[k];
+  }
+  for (int x = 0; x < nx; ++x) {
+    const psk::PackHalo h = psk::PackHaloOf(x, nx, 0);
+    halo_out[4 * x] = h.ja;
+    halo_out[4 * x + 1] = h.pa;
+    halo_out[4 * x + 2] = h.jb;
+    halo_out[4 * x + 3] = h.pb;
+  }
+  return R;
+}
+
 const char *PSHipDiffusion7VariantName(int variant) {
   if (variant < 1 || variant > kNumVariants) return nullptr;
   return kVariants[variant - 1].name;
