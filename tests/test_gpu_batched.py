@@ -62,10 +62,11 @@ def test_batch_bitwise_vs_oracle(rt, boundary, cfg, graph):
         g2.free()
 
 
-@pytest.mark.parametrize("cfg", [(2, 8, 16, 2), (0, 8, 16, 2)])
+@pytest.mark.parametrize("cfg", [(2, 8, 16, 2), (0, 8, 16, 2), (-2, 8, 12, 2)])
 def test_graph_replays_with_eager_work_between(rt, cfg):
     steps = 6
-    pairs, fields, coefs, batch = make_batch(rt, NARROW[:6] + (SHAPES[-2:] if cfg[0] == 0 else []), seed=9, boundary=0)
+    shapes = (PACKED[:6] if cfg[0] == -2 else NARROW[:6] + (SHAPES[-2:] if cfg[0] == 0 else []))
+    pairs, fields, coefs, batch = make_batch(rt, shapes, seed=9, boundary=0)
     batch.configure(*cfg)
     for rep in range(1, 4):
         batch.run(steps, graph=True)
