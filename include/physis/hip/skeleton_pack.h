@@ -75,6 +75,12 @@ __host__ __device__ inline bool PackSupported(int nx, int nw, int nxmax) {
   return nx >= 4 && nx <= nxmax && PackRows(nx, nw) >= 3;
 }
 
+// Whether a plane of nx x ny elements of elem bytes fits the tiles' 32-bit
+// byte offsets, with room for the out-of-range store offset (kPackNoStore).
+__host__ __device__ inline bool PackPlaneFits(int nx, int ny, int elem) {
+  return (int64_t)nx * ny * elem < 0x7ffffff0LL;
+}
+
 // Storage kinds the packed tiles serve: FP32 values stored in 32 or 16 bits.
 template <class S>
 constexpr bool kPackStorage = std::is_same<typename S::value_t, float>::value;
@@ -534,15 +540,15 @@ PackTB2Grid(typename Plan::Args a1, typename Plan::Args a2, int nx, int ny, int 
   PackTB2Tile<Plan, NW, NXMAX, kMid>(a1, a2, nx, ny, nz, ty * rows_out - 1, zb, min(zb + zchunk, nz), L);
 }
 
-// Compute units of the current device (queried once).
+// Compute units of the current device (queried once per device).
 inline int PackDeviceCUs() {
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-      cus = 256;
-  }
+  static int cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (dev >= 0 && dev < 64 && cache[dev] > 0) return cache[dev];
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+  if (dev >= 0 && dev < 64) cache[dev] = cus;
   return cus;
 }
 
@@ -580,6 +586,7 @@ inline hipError_t LaunchPackTB2Grid(const typename Plan::Args &a1, const typenam
     return hipErrorNotSupported;
   } else {
     if (nx < 4 || nx > 512 || ny < 1 || nz < 1) return hipErrorNotSupported;
+    if (!PackPlaneFits(nx, ny, (int)sizeof(typename Plan::Storage::stored_t))) return hipErrorNotSupported;
     const bool wide = nx > 256;
     const int nw = wide ? 16 : 8;
     if (!PackSupported(nx, nw, wide ? 512 : 256)) return hipErrorNotSupported;
@@ -612,6 +619,7 @@ inline hipError_t LaunchPackStepGrid(const typename Plan::Args &a, int nx, int n
     return hipErrorNotSupported;
   } else {
     if (nx < 4 || nx > 512 || ny < 1 || nz < 1) return hipErrorNotSupported;
+    if (!PackPlaneFits(nx, ny, (int)sizeof(typename Plan::Storage::stored_t))) return hipErrorNotSupported;
     const bool wide = nx > 256;
     const int nw = wide ? 16 : 8;
     const int rows = PackRows(nx, nw);

@@ -406,7 +406,9 @@ struct Batch {
       if (c.degree != 2 || c.zchunk <= 0 || PackKernel(storage, periodic, c.by, nxmax, false) == nullptr)
         return false;
       for (int i = 0; i < Count(); ++i)
-        if (!psk::PackSupported(a[i]->dim[0], c.by, nxmax)) return false;
+        if (!psk::PackSupported(a[i]->dim[0], c.by, nxmax) ||
+            !psk::PackPlaneFits(a[i]->dim[0], a[i]->dim[1], Rec(a[i])->storage == PS_HIP_STORAGE_NATIVE ? 4 : 2))
+          return false;
       return Supports(SingleSteps(c));
     }
     if (c.mixed) {
