@@ -113,6 +113,21 @@ def test_every_step_count_leaves_both_grids_as_the_reference(rt, shape, boundary
         assert np.array_equal(got2, want2), "g2, %d steps" % steps
 
 
+@pytest.mark.parametrize("boundary", [0, 1])
+@pytest.mark.parametrize("shape,steps", [((257, 193, 97), 12), ((383, 64, 130), 13), ((300, 120, 70), 10),
+                                         ((510, 40, 33), 9)])
+def test_large_packed_grids_bitwise(rt, shape, steps, boundary):
+    """The packed-row tiles at sizes with many row tiles and z chunks per
+    launch, both workgroup widths (8 waves up to 256 points, 16 above),
+    ragged and aligned rows, even and odd step counts: both grids equal the
+    oracle bitwise."""
+    field = O.uniform_field(shape, seed=5 + shape[0])
+    print("shape", shape, "steps", steps, "boundary", boundary, flush=True)
+    want1, want2 = O.diffusion7(field, K_ASYM, periodic=bool(boundary), steps=steps)
+    got1, got2 = run_hip(rt, field, K_ASYM, boundary, steps)
+    assert np.array_equal(got1, want1) and np.array_equal(got2, want2)
+
+
 def test_512cubed_six_steps_bitwise(rt):
     """Full size: the default kernel and the fused two-step kernels, 6 steps
     (three fused pairs, the last also storing its intermediate step)."""
