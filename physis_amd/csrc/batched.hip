@@ -153,24 +153,35 @@ BatchStarTB2(const typename P::Args *in_args, const typename P::Args *out_args, 
 // (psk::PackTB2Tile: rows of any length, several short rows per wave), all
 // grids in one work list and one launch; rows of up to NXMAX points.
 //
-// The workgroups take the list in rounds of gridDim.x items; within a round
-// each XCD (the hardware deals workgroups to the eight XCDs round robin)
-// takes a contiguous run (psk::XcdTile), so the items that share halo rows
-// and planes (adjacent z chunks and row tiles of one grid, adjacent in the
-// list) run at the same time on one XCD and meet in its L2.
+// The workgroups take items from the list through a queue: the first thread
+// of a workgroup that is free claims the next item with an atomic add on
+// queue[0], so a workgroup that drew short items takes more of them (the
+// list is ordered longest first).  The last workgroup to find the queue
+// empty (queue[1] counts them) resets both words for the next launch, so a
+// captured graph replays without a memset.
 template <class P, int NW, int NXMAX, bool kMid>
 __global__ void __launch_bounds__(64 * NW, 8)
-BatchPackTB2(const typename P::Args *in_args, const typename P::Args *out_args, const Item *items, int nitems) {
+BatchPackTB2(const typename P::Args *in_args, const typename P::Args *out_args, const Item *items, int nitems,
+             int *queue) {
   __shared__ psk::PackTB2Lds<NW, NXMAX> L;
-  for (int base = 0; base < nitems; base += gridDim.x) {
-    const int n = min((int)gridDim.x, nitems - base);
-    if ((int)blockIdx.x >= n) break;
-    const int w = base + psk::XcdTile(blockIdx.x, n);
+  __shared__ int claim;
+  for (;;) {
+    if (threadIdx.x == 0) claim = atomicAdd(&queue[0], 1);
+    __syncthreads();
+    const int w = Uni(claim);
+    if (w >= nitems) break;
     const Item it = Uniform(items[w]);
     const typename P::Args a1 = Uniform(in_args[it.grid]);
     const typename P::Args a2 = Uniform(out_args[it.grid]);
     psk::PackTB2Tile<P, NW, NXMAX, kMid>(a1, a2, a1.dim[0], a1.dim[1], a1.dim[2], it.y0, it.zb, it.ze, L);
-    __syncthreads();
+    __syncthreads();  // LDS and `claim` are reused by the next item
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&queue[1], 1) == (int)gridDim.x - 1) {
+      atomicExch(&queue[0], 0);
+      atomicExch(&queue[1], 0);
+    }
   }
 }
 
@@ -332,6 +343,7 @@ struct Batch {
   // Per-config work lists and arg tables, and cached graphs per (config, steps).
   struct Plan {
     DevArray<Item> items;
+    DevArray<int> queue;  // packed mode: next item, workgroups done (zero between launches)
     int nitems = 0;
     int blocks = 0;
   };
@@ -443,6 +455,11 @@ struct Batch {
         for (int ty = 0; ty * rows_out < ny; ++ty)
           for (int zb = 0; zb < nz; zb += zc) items.push_back(Item{g, ty * rows_out - 1, zb, std::min(zb + zc, nz)});
       }
+      // Longest items (most planes) first, in list order among equals, so
+      // the queue ends on short items.
+      std::stable_sort(items.begin(), items.end(),
+                       [](const Item &x, const Item &y) { return x.ze - x.zb > y.ze - y.zb; });
+      p.queue.Assign(std::vector<int>{0, 0}, s);
     }
     for (int g : (c.mixed == 3 ? std::vector<int>() : order)) {
       if (c.mixed && cls[g] != c.np) continue;  // another class
@@ -543,7 +560,8 @@ struct Batch {
       const void *ti = mid ? t_in_mid.p : t_in[src].p, *to = t_out[dst].p;
       const Item *iv = p.items.p;
       int n = p.nitems;
-      void *args[] = {&ti, &to, &iv, &n};
+      int *q = p.queue.p;
+      void *args[] = {&ti, &to, &iv, &n, &q};
       e = hipLaunchKernel(k, dim3(p.blocks), dim3(64 * c.by), args, 0, s);
     }
     if (odd && e == hipSuccess) e = Enqueue(SingleSteps(c), 1, s);
