@@ -12,6 +12,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "physis/physis_hip.h"
 #include "physis/physis_hip_ext.h"
 #include "physis/runtime.h"
@@ -27,10 +30,22 @@ namespace psk {
 // A grid as a kernel sees it: device pointer and extents (unused axes 1).
 template <class S>
 struct GridRef {
+  using value_t = typename S::value_t;
   typename S::stored_t *p;
   int dim[3];
-  __device__ __forceinline__ typename S::value_t load(int64_t off) const { return S::load(p + off); }
-  __device__ __forceinline__ void store(int64_t off, typename S::value_t v) const { S::store(p + off, v); }
+  __device__ __forceinline__ value_t load(int64_t off) const { return S::load(p + off); }
+  __device__ __forceinline__ void store(int64_t off, value_t v) const { S::store(p + off, v); }
+  // Element idx of a grid below 4 GiB: the uniform base plus a 32-bit byte
+  // offset (one VGPR per access instead of a 64-bit address).
+  __device__ __forceinline__ value_t load_u(uint32_t idx) const {
+    return S::load(reinterpret_cast<const typename S::stored_t *>(reinterpret_cast<const char *>(p) +
+                                                                  idx * (uint32_t)sizeof(typename S::stored_t)));
+  }
+  __device__ __forceinline__ void store_u(uint32_t idx, value_t v) const {
+    S::store(reinterpret_cast<typename S::stored_t *>(reinterpret_cast<char *>(p) +
+                                                      idx * (uint32_t)sizeof(typename S::stored_t)),
+             v);
+  }
 };
 
 template <class S>
@@ -61,6 +76,39 @@ __device__ __forceinline__ int64_t OffP3(const R &g, int i, int j, int k) {
          (int64_t)((k + g.dim[2]) % g.dim[2]) * g.dim[0] * g.dim[1];
 }
 
+// Accesses of rank-3 grids in a 3-D plan.  A plan instantiated with kU (its
+// launch checked UniformExtents) indexes every grid by ref's extents in 32
+// bits, so the grids read at one point share one element index, computed
+// once; otherwise each grid's own extents give a 64-bit offset (Off3).
+template <bool kU, class R, class Ref>
+__device__ __forceinline__ typename R::value_t Get3(const R &g, const Ref &ref, int i, int j, int k) {
+  if constexpr (kU) {
+    const uint32_t nx = (uint32_t)ref.dim[0], nxy = nx * (uint32_t)ref.dim[1];
+    return g.load_u((uint32_t)i + (uint32_t)j * nx + (uint32_t)k * nxy);
+  } else {
+    return g.load(Off3(g, i, j, k));
+  }
+}
+template <bool kU, class R, class Ref>
+__device__ __forceinline__ void Put3(const R &g, const Ref &ref, int i, int j, int k, typename R::value_t v) {
+  if constexpr (kU) {
+    const uint32_t nx = (uint32_t)ref.dim[0], nxy = nx * (uint32_t)ref.dim[1];
+    g.store_u((uint32_t)i + (uint32_t)j * nx + (uint32_t)k * nxy, v);
+  } else {
+    g.store(Off3(g, i, j, k), v);
+  }
+}
+
+// Whether every grid has ref's extents and fits 32-bit byte offsets.
+template <class Ref, class... R>
+inline bool UniformExtents(const Ref &ref, const R &...g) {
+  auto fits = [&](const auto &x) {
+    return x.dim[0] == ref.dim[0] && x.dim[1] == ref.dim[1] && x.dim[2] == ref.dim[2] &&
+           (uint64_t)x.dim[0] * (uint64_t)x.dim[1] * (uint64_t)x.dim[2] * sizeof(*x.p) <= 0xffffffffull;
+  };
+  return fits(ref) && (fits(g) && ...);
+}
+
 inline void CheckLaunch(hipError_t e, const char *kernel) {
   if (e != hipSuccess) {
     fprintf(stderr, "physis-hip: launch of %s failed: %s\n", kernel, hipGetErrorString(e));
@@ -89,13 +137,63 @@ inline int FloatStorage(const __PSGrid *const *grids, int n) {
 // 3-D maps of large boxes may run two tile rows per thread (LaunchZMarch3D);
 // plans with many grids read outside the ring, or whose two-row ring would
 // not fit 64 KiB of LDS, keep one row per thread.
+// Bytes of an element of a plan's staged grid (4 when it stages none).
+template <class P, class = void>
+struct PlanElemBytes {
+  static constexpr int value = 4;
+};
+template <class P>
+struct PlanElemBytes<P, std::void_t<typename P::Storage>> {
+  static constexpr int value = (int)sizeof(typename P::Storage::stored_t);
+};
+
+// 3-D plans that read four or more grids straight from HBM at each point
+// (the Himeno Jacobi reads twelve): tiles of 256 threads, the widest of
+// 256, 128 and 64 points that covers the box's rows with at most 10% of the
+// lanes idle (each step then reads longer runs of every grid), and the
+// longest z chunk of 4, 2 planes (4-byte elements) or 16, 8, 4, 2 (2-byte)
+// that still gives 1024 workgroups.  The same rule as the built-in Himeno
+// launch (himeno19.hip AutoTile), which timed it against 35 tiles.
+template <class P>
+inline hipError_t LaunchZMarchWide(const typename P::Args &a, const Box &box, hipStream_t s, int parity) {
+  const int ex = std::max(box.hi[0] - box.lo[0], 1);
+  int bx = 64;
+  for (int w : {256, 128}) {
+    if (10 * ex >= 9 * ((ex + w - 1) / w * w)) {
+      bx = w;
+      break;
+    }
+  }
+  static const int kChunks4[] = {4, 2}, kChunks2[] = {16, 8, 4, 2};
+  const bool narrow = PlanElemBytes<P>::value == 2;
+  const int *chunks = narrow ? kChunks2 : kChunks4;
+  const int n = narrow ? 4 : 2;
+  int zc = chunks[n - 1];
+  for (int i = 0; i < n; ++i) {
+    const int blocks = bx == 256 ? ZMarchBlocks<256, 1>(box, chunks[i])
+                       : bx == 128 ? ZMarchBlocks<128, 2>(box, chunks[i])
+                                   : ZMarchBlocks<64, 4>(box, chunks[i]);
+    if (blocks >= 1024) {
+      zc = chunks[i];
+      break;
+    }
+  }
+  if (bx == 256) return LaunchZMarch<P, 256, 1>(a, box, zc, true, s, parity);
+  if (bx == 128) return LaunchZMarch<P, 128, 2>(a, box, zc, true, s, parity);
+  return LaunchZMarch<P, 64, 4>(a, box, zc, true, s, parity);
+}
+
 template <class P, int RANK>
 inline hipError_t LaunchRing(const typename P::Args &a, const Box &box, hipStream_t s, int parity = -1) {
   if (RANK == 1) return LaunchZMarch<P, 64, 1>(a, box, 0, true, s, parity);
   if (RANK == 2) return LaunchZMarch<P, 64, 4>(a, box, 0, true, s, parity);
-  constexpr size_t kRing2 = (size_t)(P::NS > 0 ? P::NS : 1) * (2 * P::RZ + 2) * (64 + 2 * P::RX) *
-                            (16 + 2 * P::RY) * sizeof(typename P::SV);
-  return LaunchZMarch3D<P, (P::kDirectGrids <= 2 && kRing2 <= 64 * 1024)>(a, box, 64, true, s, parity);
+  if constexpr (RANK == 3 && P::kDirectGrids >= 4) {
+    return LaunchZMarchWide<P>(a, box, s, parity);
+  } else {
+    constexpr size_t kRing2 = (size_t)(P::NS > 0 ? P::NS : 1) * (2 * P::RZ + 2) * (64 + 2 * P::RX) *
+                              (16 + 2 * P::RY) * sizeof(typename P::SV);
+    return LaunchZMarch3D<P, (P::kDirectGrids <= 2 && kRing2 <= 64 * 1024)>(a, box, 64, true, s, parity);
+  }
 }
 
 // One step of a star plan: the packed-row tile for whole FP32-valued grids

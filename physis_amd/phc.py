@@ -37,13 +37,14 @@ def translate(src, out_hip, config=None, defines=()):
     return out_hip
 
 
-def compile_hip(hip, exe, extra_c=(), extra_cxx=(), cxx_flags=()):
+def compile_hip(hip, exe, extra_c=(), extra_cxx=(), cxx_flags=(), defines=()):
     """hipcc the generated file and link it with libphysis_rt_hip; extra_c
     are plain C files of the program (a module's host side), compiled with
     the system C compiler and linked in, as the reference's test driver does
     for its module tests (run_system_tests.sh.cmake:605-631); extra_cxx are
     plain C++ files (a benchmark harness), compiled with the system C++
-    compiler and cxx_flags."""
+    compiler and cxx_flags; defines (NAME or NAME=V) also apply to the
+    headers the generated file includes."""
     out_dir = os.path.dirname(os.path.abspath(exe))
     rel = os.path.relpath(LIB_DIR, out_dir)
     rpath = "$ORIGIN/" + rel if os.path.abspath(out_dir).startswith(HERE) else LIB_DIR
@@ -56,7 +57,7 @@ def compile_hip(hip, exe, extra_c=(), extra_cxx=(), cxx_flags=()):
         obj = exe + "." + os.path.splitext(os.path.basename(c))[0] + ".o"
         subprocess.run(["g++", "-O3", "-ffp-contract=off"] + list(cxx_flags) + ["-c", c, "-o", obj], check=True)
         objs.append(obj)
-    cmd = [HIPCC] + HIP_FLAGS + ["-I", INCLUDE, "-x", "hip", hip, "-x", "none"] + objs + \
+    cmd = [HIPCC] + HIP_FLAGS + ["-D" + d for d in defines] + ["-I", INCLUDE, "-x", "hip", hip, "-x", "none"] + objs + \
           ["-L", LIB_DIR, "-lphysis_rt_hip", "-Wl,-rpath," + rpath, "-o", exe]
     subprocess.run(cmd, check=True)
     for obj in objs:

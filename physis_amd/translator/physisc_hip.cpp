@@ -568,6 +568,21 @@ struct Kernel {
   int clamp_ifs = 0;    // clamped-face reads of the staged grid (ClampIf)
 };
 
+// The grid whose extents index every rank-3 grid of a 3-D kernel launched
+// with uniform extents (psk::Get3/Put3 with __kU): the staged grid, else the
+// first rank-3 grid of primitive type; "" when the kernel has none (or is
+// not 3-D), and then its accesses keep per-grid offsets.
+std::string UniformRef(const Kernel &k) {
+  if (k.rank != 3) return "";
+  if (!k.staged.empty()) {
+    const GridType &sg = k.gtype.at(k.staged);
+    if (sg.rank == 3 && sg.utype.empty()) return k.staged;
+  }
+  for (const Param &p : k.all)
+    if (p.is_grid && p.gt.rank == 3 && p.gt.utype.empty()) return p.name;
+  return "";
+}
+
 // Parses "x", "x + 3", "x - 1", "2 + x", "(x-1)" against index name `name`.
 bool RelativeIndex(const std::vector<Tok> &t, size_t b, size_t e, const std::string &name, int *off) {
   while (e - b >= 2 && t[b].s == "(" && MatchClose(t, b) == e - 1) {
@@ -893,6 +908,7 @@ std::string Translator::RewriteBody(const Kernel &k, size_t b, size_t e, bool co
     if (!out.empty() && IsIdentChar(out.back()) && !s.empty() && IsIdentChar(s[0])) out += ' ';
     out += s;
   };
+  const std::string uref = UniformRef(k);
   // Offset of the map's current point in grid g.
   auto center = [&](const std::string &g, const GridType &gt) {
     std::string at = "psk::Off" + std::to_string(gt.rank) + "(" + g;
@@ -983,8 +999,14 @@ std::string Translator::RewriteBody(const Kernel &k, size_t b, size_t e, bool co
         r << "(" << g << ".dim[" << a[0] << "])";
       } else if (tk.s == "PSGridEmit") {
         const std::string at = center(g, gt);
-        if (compute_form) r << "return (" << gt.elem << ")(" << a[0] << ")";
-        else r << g << ".store(" << at << ", (" << gt.elem << ")(" << a[0] << "))";
+        if (compute_form) {
+          r << "return (" << gt.elem << ")(" << a[0] << ")";
+        } else if (!uref.empty() && gt.rank == 3) {
+          r << "psk::Put3<__kU>(" << g << ", __ref, " << k.idx[0] << ", " << k.idx[1] << ", " << k.idx[2] << ", ("
+            << gt.elem << ")(" << a[0] << "))";
+        } else {
+          r << g << ".store(" << at << ", (" << gt.elem << ")(" << a[0] << "))";
+        }
       } else {
         Get acc;
         acc.relative = gt.rank <= k.rank;
@@ -993,6 +1015,10 @@ std::string Translator::RewriteBody(const Kernel &k, size_t b, size_t e, bool co
         if (g == k.staged && acc.relative) {
           r << "((" << gt.elem << ")__ctx.at(0, " << acc.off[0] << ", " << acc.off[1] << ", " << acc.off[2]
             << "))";
+        } else if (!uref.empty() && gt.rank == 3 && tk.s != "PSGridGetPeriodic") {
+          r << "psk::Get3<__kU>(" << g << ", __ref";
+          for (auto &s : a) r << ", " << s;
+          r << ")";
         } else {
           r << g << ".load(psk::Off" << (tk.s == "PSGridGetPeriodic" ? "P" : "") << gt.rank << "(" << g;
           for (auto &s : a) r << ", " << s;
@@ -1029,7 +1055,10 @@ std::string Translator::EmitPlan(const Kernel &k) {
   o << "// Plan for kernel " << f.name << " (rank " << k.rank << ", staged grid: "
     << (k.staged.empty() ? "none" : k.staged) << ", radius " << k.rx << "/" << k.ry << "/" << k.rz
     << (k.star ? ", star form" : "") << (k.fusable ? ", fusable pairs" : "") << ")\n";
-  o << "template <class SF>\nstruct " << P << " {\n";
+  // __kU: launched with uniform extents (every rank-3 grid has the extents
+  // of __ref and fits 32-bit byte offsets): accesses at a point share one
+  // element index (psk::Get3/Put3).
+  o << "template <class SF, bool __kU = false>\nstruct " << P << " {\n";
   o << "  using SV = " << sv << ";\n";
   o << "  static constexpr int NS = " << (k.staged.empty() ? 0 : 1) << ", RX = " << k.rx << ", RY = " << k.ry
     << ", RZ = " << k.rz << ";\n";
@@ -1074,6 +1103,8 @@ std::string Translator::EmitPlan(const Kernel &k) {
       else q << "    " << p.type << " " << p.name << " = __a." << p.name << ";\n";
     }
     for (const Param &p : k.all) q << "    (void)" << p.name << ";\n";
+    const std::string uref = UniformRef(k);
+    if (!uref.empty()) q << "    const auto &__ref = __a." << uref << ";\n    (void)__ref;\n";
     return q.str();
   };
   std::string params = "const C &__ctx, const Args &__a";
@@ -1124,8 +1155,8 @@ std::string Translator::EmitMapAndLaunch(const Kernel &k) {
   }
   o << "  return s;\n}\n\n";
   // Typed args.
-  o << "template <class SF>\nstatic typename " << P << "<SF>::Args __PSArgs_" << f.name << "(const struct " << S
-    << " *s) {\n  typename " << P << "<SF>::Args a;\n";
+  o << "template <class SF, bool __kU = false>\nstatic typename " << P << "<SF, __kU>::Args __PSArgs_" << f.name
+    << "(const struct " << S << " *s) {\n  typename " << P << "<SF, __kU>::Args a;\n";
   for (const Param &p : k.all) {
     if (p.is_grid && !p.gt.utype.empty())
       o << "  a." << p.name << " = __PSUMakeRef_" << p.gt.utype << "(s->" << p.name << ");\n";
@@ -1144,6 +1175,14 @@ std::string Translator::EmitMapAndLaunch(const Kernel &k) {
     o << "  if (parity < 0) {\n    psk::CheckLaunch(psk::LaunchStarStep<" << P << "<SF>>(a, box, a." << k.staged
       << ".dim[0], a." << k.staged << ".dim[1], a." << k.staged << ".dim[2], st), \"" << f.name
       << "\");\n    return;\n  }\n";
+  const std::string uref = UniformRef(k);
+  if (!uref.empty()) {
+    o << "  if (psk::UniformExtents(a." << uref;
+    for (const Param &p : k.all)
+      if (p.is_grid && p.gt.rank == 3 && p.gt.utype.empty() && p.name != uref) o << ", a." << p.name;
+    o << ")) {\n    psk::CheckLaunch(psk::LaunchRing<" << P << "<SF, true>, 3>(__PSArgs_" << f.name
+      << "<SF, true>(s), box, st, parity), \"" << f.name << "\");\n    return;\n  }\n";
+  }
   o << "  psk::CheckLaunch(psk::LaunchRing<" << P << "<SF>, " << k.rank << ">(a, box, st, parity), \"" << f.name
     << "\");\n";
   o << "}\n\n";

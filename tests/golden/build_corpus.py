@@ -11,8 +11,9 @@ main) is linked with its plain C driver test_*.module_base.c.  The
 reference's diffusion benchmark (examples/diffusion-benchmark) is built the
 same way: its Physis program translated, its C++ harness compiled for the
 Physis variant and linked in (tests/_corpus_bin/diffusion_benchmark_physis).
-The DSL programs of this repository under tests/golden/programs are built
-into the same directory.
+The reference's Himeno program is built the same way (himeno_physis, see
+build_himeno).  The DSL programs of this repository under
+tests/golden/programs are built into the same directory.
 """
 import json
 import os
@@ -56,6 +57,29 @@ def build_benchmark(phc):
         phc.build(os.path.join(BENCH_DIR, "diffusion3d_physis.c"), DROPIN_EXE, config=cfg,
                   extra_cxx=[os.path.join(HERE, "dropin_timing.cc")], cxx_flags=["-w"])
         os.remove(DROPIN_EXE + ".hip")
+        return True
+    except subprocess.CalledProcessError:
+        return False
+
+
+# The reference's Himeno program (examples/himeno/himenobmtxpa_physis.c, the
+# 19-point Jacobi of the Himeno workload), translated unchanged with its
+# ENABLE_DUMP block on (it writes p after the four rehearsal sweeps to
+# himeno.physis.dat) and its wall clock replaced by himeno_clock.c, so that
+# its one-minute measured loop runs 18 sweeps.
+HIMENO_SRC = "/root/reference/examples/himeno/himenobmtxpa_physis.c"
+HIMENO_EXE = os.path.join(OUT, "himeno_physis")
+
+
+def build_himeno(phc):
+    if not os.path.exists(HIMENO_SRC):
+        return True
+    defines = ["ENABLE_DUMP", "gettimeofday=ps_himeno_clock"]
+    hip = HIMENO_EXE + ".hip"
+    try:
+        phc.translate(HIMENO_SRC, hip, None, defines)
+        phc.compile_hip(hip, HIMENO_EXE, extra_c=[os.path.join(HERE, "himeno_clock.c")], defines=defines)
+        os.remove(hip)
         return True
     except subprocess.CalledProcessError:
         return False
@@ -114,6 +138,7 @@ def main(jobs=8):
     built = [s for s, ok in results if ok]
     failed = [s for s, ok in results if not ok]
     (built if build_benchmark(phc) else failed).append("diffusion_benchmark_physis")
+    (built if build_himeno(phc) else failed).append("himeno_physis")
     with open(os.path.join(OUT, "manifest.json"), "w") as f:
         json.dump({"built": built, "failed": failed}, f, indent=1)
     print("built %d, failed %d %s" % (len(built), len(failed), failed), file=sys.stderr)

@@ -173,3 +173,36 @@ def test_kernel_reading_its_output_grid_runs_unfused_and_matches(tmp_path, shape
     got = np.fromfile(dst, np.float32).reshape(2, nz, ny, nx)
     want_u, want_v = wave_oracle(u, v, 0.1, it)
     assert np.array_equal(got[0], want_u) and np.array_equal(got[1], want_v)
+
+
+@pytest.mark.parametrize("size,dims", [("XS", (64, 32, 32)), ("S", (128, 64, 64))])
+def test_reference_himeno_program_translated(tmp_path, size, dims):
+    """The reference's Himeno program (examples/himeno/himenobmtxpa_physis.c,
+    the 19-point Jacobi of the Himeno workload), translated unchanged by
+    physisc-hip (tests/golden/build_corpus.py): its grids, initialisation
+    (p = (k/(nz-1))^2, a = c = bnd = 1, a3 = 1/6, b = wrk1 = 0,
+    himenobmtxpa_physis.c:120-151, 297-329), its ping-pong PSStencilRun of
+    jacobi_kernel over the interior domain (:331-393) and its own dump of p
+    after the four rehearsal sweeps (ENABLE_DUMP, :177-194, "%f" per point).  Every dumped
+    line equals the oracle's FP32 value printed the same way, and the
+    measured loop (18 sweeps under the stand-in clock) completes."""
+    import numpy as np
+    from oracle import oracle as O
+    nx, ny, nz = dims
+    print("himeno", size, flush=True)
+    out = subprocess.run([exe("himeno_physis"), size], capture_output=True, timeout=120, cwd=tmp_path)
+    assert out.returncode == 0, out.stderr.decode()[-2000:]
+    text = out.stdout.decode()
+    assert "mimax = %d mjmax = %d mkmax = %d" % dims in text
+    assert "Loop executed for 18 times" in text
+    got = (tmp_path / "himeno.physis.dat").read_text().split("\n")[:-1]
+    assert len(got) == nx * ny * nz
+    k = np.arange(nz, dtype=np.int64)
+    col = (k * k).astype(np.float32) / np.float32((nz - 1) * (nz - 1))
+    p = np.ascontiguousarray(np.broadcast_to(col[:, None, None], (nz, ny, nx)), dtype=np.float32)
+    vals = [1.0, 1.0, 1.0, 1.0 / 6.0, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, 1.0, 0.0]
+    coefs = [np.full((nz, ny, nx), np.float32(v), np.float32) for v in vals]
+    want, _, _ = O.himeno(p, coefs, np.float32(0.8), 4)
+    want_lines = ["%f" % v for v in want.ravel().tolist()]
+    bad = [i for i, (g, w) in enumerate(zip(got, want_lines)) if g != w]
+    assert not bad, "%d points differ, first at %d: %s vs %s" % (len(bad), bad[0], got[bad[0]], want_lines[bad[0]])

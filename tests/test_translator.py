@@ -177,3 +177,55 @@ def test_fusable_check_rejects_grids_aliasing_the_swapped_pair(tmp_path):
     fus = hip[hip.find("static int __PSFusable_wave"):]
     fus = fus[:fus.find("\n}\n")]
     assert "a->g == a->a || a->g == a->b" in fus
+
+
+COEF_KERNEL = """
+#include "physis/physis.h"
+static void relax(const int x, const int y, const int z, PSGrid3DFloat p, PSGrid3DFloat q, PSGrid3DFloat a,
+                  PSGrid3DFloat b, PSGrid3DFloat c, PSGrid3DFloat d, PSGrid2DFloat m) {
+  float s = PSGridGet(a, x, y, z) * PSGridGet(p, x + 1, y, z) + PSGridGet(b, x, y, z) * PSGridGet(p, x, y - 1, z)
+            + PSGridGet(c, x, y, z + 1) * PSGridGet(p, x, y, z) + PSGridGet(d, x, y, z) * PSGridGet(m, x, y)
+            + PSGridGetPeriodic(a, x - 1, y, z);
+  PSGridEmit(q, s);
+  return;
+}
+int main(int argc, char *argv[]) {
+  PSInit(&argc, &argv, 3, 16, 16, 16);
+  PSGrid3DFloat p = PSGrid3DFloatNew(16, 16, 16);
+  PSGrid3DFloat q = PSGrid3DFloatNew(16, 16, 16);
+  PSGrid3DFloat a = PSGrid3DFloatNew(16, 16, 16);
+  PSGrid3DFloat b = PSGrid3DFloatNew(16, 16, 16);
+  PSGrid3DFloat c = PSGrid3DFloatNew(16, 16, 17);
+  PSGrid3DFloat d = PSGrid3DFloatNew(16, 16, 16);
+  PSGrid2DFloat m = PSGrid2DFloatNew(16, 16);
+  PSDomain3D dom = PSDomain3DNew(1, 15, 1, 15, 1, 15);
+  PSStencilRun(PSStencilMap(relax, dom, p, q, a, b, c, d, m), 1);
+  PSFinalize();
+  return 0;
+}
+"""
+
+
+def test_plan_reading_many_grids_has_a_uniform_extents_variant(tmp_path):
+    """A 3-D kernel reading grids at the point itself gets a second plan
+    instantiation (__kU) in which every rank-3 access indexes by the staged
+    grid's extents in 32 bits (psk::Get3/Put3), launched only when
+    psk::UniformExtents holds for every rank-3 grid of the map (here c has
+    other extents, so that map keeps the per-grid path at run time).  The
+    2-D grid and the periodic read keep their own offsets; the staged grid
+    is read from the LDS ring."""
+    hip = translate_text(tmp_path, COEF_KERNEL)
+    start = hip.find("struct __PSPlan_relax")
+    plan = hip[start:hip.find("\n};\n", start)]
+    assert "template <class SF, bool __kU = false>\nstruct __PSPlan_relax" in hip
+    for g, at in (("a", "x, y, z"), ("b", "x, y, z"), ("c", "x, y, z + 1"), ("d", "x, y, z")):
+        assert "psk::Get3<__kU>(%s, __ref, %s)" % (g, at) in plan, g
+    assert "psk::Put3<__kU>(q, __ref, x, y, z," in plan
+    assert "m.load(psk::Off2(m, x, y))" in plan
+    assert "a.load(psk::OffP3(a, x - 1, y, z))" in plan
+    assert "const auto &__ref = __a.p;" in plan
+    assert "__ctx.at(0, 1, 0, 0)" in plan
+    launch = hip[hip.find("static void __PSLaunchT_relax"):]
+    assert "psk::UniformExtents(a.p, a.q, a.a, a.b, a.c, a.d))" in launch
+    assert "LaunchRing<__PSPlan_relax<SF, true>, 3>(__PSArgs_relax<SF, true>(s)" in launch
+    assert "LaunchRing<__PSPlan_relax<SF>, 3>(a, box, st, parity)" in launch
