@@ -179,7 +179,10 @@ extern int PSHipBatchCreateEx(void **out, __PSGrid *const *a, __PSGrid *const *b
 extern int PSHipBatchDestroy(void *batch);
 /* steps steps of every grid (A->B, B->A, ...); use_graph: capture and replay. */
 extern int PSHipBatchRun(void *batch, int steps, int use_graph);
-/* The same on a caller stream (a hipStream_t), without a graph. */
+/* The same on a caller stream (a hipStream_t), without a graph.  Runs of one
+ * batch must not overlap in time (they share its grids, third buffers and
+ * the device queue its packed launches take work items from): queue them on
+ * one stream, or order them with events. */
 extern int PSHipBatchRunOn(void *batch, int steps, void *stream);
 /* Launch shape: points per lane np (row width 64*np), rows per workgroup by,
  * planes per work item zchunk, temporal degree 1 or 2.  np = 0: mixed mode,
