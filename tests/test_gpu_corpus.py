@@ -206,3 +206,54 @@ def test_reference_himeno_program_translated(tmp_path, size, dims):
     want_lines = ["%f" % v for v in want.ravel().tolist()]
     bad = [i for i, (g, w) in enumerate(zip(got, want_lines)) if g != w]
     assert not bad, "%d points differ, first at %d: %s vs %s" % (len(bad), bad[0], got[bad[0]], want_lines[bad[0]])
+
+
+def coef_extents_oracle(p, a, b, c, w, iters):
+    """FP32 evaluation of tests/golden/programs/coef_extents.c in its own
+    order: ((a*p(x+1) + b*p(y-1)) + c(x, y, z+1)*p) + w*a(x-1) over the
+    interior, maps p -> q then q -> p (arrays indexed [z, y, x]; c may have
+    more rows than p)."""
+    import numpy as np
+    w = np.float32(w)
+    nz, ny, nx = p.shape
+    I = (slice(1, nz - 1), slice(1, ny - 1), slice(1, nx - 1))
+    c1 = c[2:nz, 1:ny - 1, 1:nx - 1]  # c(x, y, z + 1) on the interior
+
+    def relax(src, dst):
+        out = dst.copy()
+        v = a[I] * src[1:-1, 1:-1, 2:] + b[I] * src[1:-1, :-2, 1:-1]
+        v = v + c1 * src[I]
+        v = v + w * a[1:-1, 1:-1, :-2]
+        out[I] = v
+        return out
+
+    q = p.copy()
+    for _ in range(iters):
+        q = relax(p, q)
+        p = relax(q, p)
+    return p, q
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_translated_plan_with_direct_grids_both_extents_paths(tmp_path, mode):
+    """A map reading three grids besides the staged one (one of them at
+    z + 1 and one through the periodic form), run as a ping-pong over the
+    interior: with every grid of one shape the launch takes the uniform-
+    extents plan, with grid c one row taller (MODE 1) the per-grid one; both
+    equal an FP32 evaluation in the program's order bitwise."""
+    import numpy as np
+    from oracle import oracle as O
+    nx, ny, nz, it = 70, 23, 19, 3
+    p = O.uniform_field((nx, ny, nz), seed=11)
+    a = O.uniform_field((nx, ny, nz), seed=12)
+    b = O.uniform_field((nx, ny, nz), seed=13)
+    c = O.uniform_field((nx, ny + mode, nz), seed=14)
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    np.concatenate([p.ravel(), a.ravel(), b.ravel(), c.ravel()]).astype(np.float32).tofile(src)
+    print("coef_extents mode", mode, flush=True)
+    out = subprocess.run([exe("coef_extents"), str(nx), str(ny), str(nz), str(mode), str(it), str(src), str(dst)],
+                         capture_output=True, timeout=120)
+    assert out.returncode == 0, out.stderr.decode()[-2000:]
+    got = np.fromfile(dst, np.float32).reshape(2, nz, ny, nx)
+    want_p, want_q = coef_extents_oracle(p, a, b, c, 0.25, it)
+    assert np.array_equal(got[0], want_p) and np.array_equal(got[1], want_q)
