@@ -21,6 +21,7 @@
 #include "physis/hip/skeleton.h"
 #include "physis/hip/skeleton_star.h"
 #include "physis/hip/skeleton_pack.h"
+#include "physis/hip/skeleton_point_tb.h"
 #include "physis/hip/skeleton_star_tb.h"
 #include "physis/hip/skeleton_stride.h"
 #include "physis/hip/storage.h"

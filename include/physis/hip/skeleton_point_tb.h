@@ -43,6 +43,7 @@
 
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 #include "physis/hip/skeleton.h"
 #include "physis/hip/storage.h"
@@ -345,4 +346,51 @@ inline int TBSchedule(int sweeps, int T, int *out, int max) {
   return n;
 }
 
+// `sweeps` sweeps of a point plan without reduction (kReduce false) from u_0
+// in A over the interior of a grid of a.dim, leaving A and B as single
+// sweeps leave them (u_n in A for even n, else in B; u_{n-1} in the other):
+// passes of two sweeps through the third buffer X while at least four
+// sweeps remain (TBSchedule), then one sweep per pass, on 64 x 8 tiles (the
+// fused Himeno shape fastest for FP32 storage on 257x129x129 and 384^3).  a.c, a.sc and a.dim are the caller's; the rest is
+// filled here.
+template <class Plan, class S>
+inline hipError_t RunPointTB(PointTBArgs<typename Plan::Scalars> a, void *A, void *B, void *X, int sweeps,
+                             hipStream_t stream) {
+  static_assert(!Plan::kReduce, "plans with a reduction need the partial sums of their own runner");
+  const int ix = a.dim[0] - 2, iy = a.dim[1] - 2, iz = a.dim[2] - 2;
+  if (ix <= 0 || iy <= 0 || iz <= 0 || sweeps <= 0) return hipSuccess;
+  std::vector<int> passes(4 * ((size_t)sweeps + 1));
+  const int n = TBSchedule(sweeps, 2, passes.data(), sweeps + 1);
+  if (n < 0) return hipErrorInvalidValue;
+  void *buf[3] = {A, B, X};
+  a.shell[0] = A;
+  a.shell[1] = B;
+  a.partials = nullptr;
+  a.prev_partials = nullptr;
+  a.prev_levels = a.prev_blocks = 0;
+  a.prev_out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    const int T = passes[4 * i + 1];
+    a.k0 = passes[4 * i];
+    a.in = buf[passes[4 * i + 2]];
+    a.out = buf[passes[4 * i + 3]];
+    const int ox = 64 - 2 * (T - 1), oy = 8 - 2 * (T - 1);
+    a.tiles_x = (ix + ox - 1) / ox;
+    const int tiles_y = (iy + oy - 1) / oy;
+    // Fused passes take 32-plane chunks; the one-sweep passes at the end the
+    // longest of 16, 8, 4, 2 planes that still gives 1024 workgroups.
+    int zc = 32;
+    if (T == 1)
+      for (zc = 16; zc > 2 && a.tiles_x * tiles_y * ((iz + zc - 1) / zc) < 1024; zc /= 2) {
+      }
+    a.zchunk = iz < zc ? iz : zc;
+    const int chunks = (iz + a.zchunk - 1) / a.zchunk;
+    const void *k = T == 2 ? reinterpret_cast<const void *>(&PointTB<Plan, S, 2, 8, 1, 1>)
+                           : reinterpret_cast<const void *>(&PointTB<Plan, S, 1, 8, 1, 1>);
+    void *args[] = {&a};
+    const hipError_t e = hipLaunchKernel(k, dim3(a.tiles_x * tiles_y, chunks), dim3(64, 8), args, 0, stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
 }  // namespace psk

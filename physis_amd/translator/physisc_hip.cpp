@@ -563,6 +563,14 @@ struct Kernel {
   // reads of its output grid (the first map's input) would see neither the
   // value the separate steps see nor a finished buffer.
   bool fusable = false;
+  // Point form (psk::PointTB, skeleton_point_tb.h): a staged float grid read
+  // within radius 1 on every axis (diagonals included), the other grids read
+  // at the point itself only (point_grids, in parameter order), one float
+  // grid emitted as the last statement, the index names used only inside
+  // grid reads.  Two maps of such a kernel over the interior of the staged
+  // grid run two sweeps per HBM pass.
+  bool point = false;
+  std::vector<std::string> point_grids;
   // Locals assigned once from PSGridDim(staged grid, d): name -> d.
   std::map<std::string, int> dimvar;
   int clamp_ifs = 0;    // clamped-face reads of the staged grid (ClampIf)
@@ -572,6 +580,13 @@ struct Kernel {
 // with uniform extents (psk::Get3/Put3 with __kU): the staged grid, else the
 // first rank-3 grid of primitive type; "" when the kernel has none (or is
 // not 3-D), and then its accesses keep per-grid offsets.
+// A parameter type without a leading "const" (a struct member assigned
+// after construction).
+std::string StripConst(std::string t) {
+  while (t.compare(0, 6, "const ") == 0) t = t.substr(6);
+  return t;
+}
+
 std::string UniformRef(const Kernel &k) {
   if (k.rank != 3) return "";
   if (!k.staged.empty()) {
@@ -625,6 +640,12 @@ class Translator {
   void CollectGridVars();
   void AnalyzeKernel(Kernel &k);
   std::string RewriteBody(const Kernel &k, size_t b, size_t e, bool compute_form, bool exact = false);
+  // Whether the body's one PSGridEmit sits at its top level, followed only
+  // by an optional "return;", with no return before it.
+  bool EmitIsTail(const Func &f) const;
+  // Set while RewriteBody renders a point-form update(): staged-grid reads
+  // become __PSat(dx, dy, dz) and point-grid reads __PSpt(n).
+  bool point_mode_ = false;
   int ClampIf(const Kernel &k, size_t i, size_t e) const;
   std::string EmitPlan(const Kernel &k);
   std::string EmitMapAndLaunch(const Kernel &k);
@@ -779,33 +800,7 @@ void Translator::AnalyzeKernel(Kernel &k) {
       int nz = (g.off[0] != 0) + (g.off[1] != 0) + (g.off[2] != 0);
       if (nz > 1 || std::abs(g.off[0]) > 1 || std::abs(g.off[1]) > 1 || std::abs(g.off[2]) > 1) in_star = false;
     }
-    // The emit statement must sit at the top level of the body, followed
-    // only by an optional "return;".
-    size_t emit_at = 0;
-    int depth = 0;
-    bool top = false;
-    for (size_t i = f.lbrace + 1; i < f.rbrace; ++i) {
-      const std::string &s = t_[i].s;
-      if (s == "{") ++depth;
-      else if (s == "}") --depth;
-      else if (s == "PSGridEmit") {
-        emit_at = i;
-        top = depth == 0;
-      } else if (s == "return" && i + 1 < f.rbrace && t_[i + 1].s != ";") {
-        top = false;  // value returns are not part of the DSL
-      }
-    }
-    bool tail_ok = false;
-    if (top && emit_at) {
-      size_t close = MatchClose(t_, emit_at + 1);
-      size_t q = close + 1;
-      if (q < f.rbrace && t_[q].s == ";") ++q;
-      if (q < f.rbrace && t_[q].s == "return" && q + 1 < f.rbrace && t_[q + 1].s == ";") q += 2;
-      tail_ok = q == f.rbrace;
-      // no early "return;" before the emit
-      for (size_t i = f.lbrace + 1; i < emit_at; ++i)
-        if (t_[i].s == "return") tail_ok = false;
-    }
+    const bool tail_ok = EmitIsTail(f);
     const GridType &og = k.gtype[*k.emitted.begin()];
     if (in_star && tail_ok && (k.rx | k.ry | k.rz) <= 1 && og.elem == "float" && og.rank == 3) {
       k.star = true;
@@ -813,6 +808,43 @@ void Translator::AnalyzeKernel(Kernel &k) {
       k.fusable = true;
       for (const Get &g : k.gets)
         if (g.grid == k.out_grid) k.fusable = false;
+    }
+  }
+  if (!k.star && k.rank == 3 && !k.staged.empty() && k.gtype[k.staged].elem == "float" &&
+      k.gtype[k.staged].utype.empty() && k.gtype[k.staged].rank == 3 && !any_abs[k.staged] && !k.wrap &&
+      (k.rx | k.ry | k.rz) <= 1 && k.emitted.size() == 1 && k.emit_count == 1 && EmitIsTail(f)) {
+    const std::string out = *k.emitted.begin();
+    const GridType &og = k.gtype[out];
+    bool ok = out != k.staged && og.elem == "float" && og.rank == 3 && og.utype.empty();
+    std::set<std::string> point;
+    for (const Get &g : k.gets) {
+      if (g.grid == out || g.periodic) ok = false;
+      if (g.grid == k.staged) continue;
+      const GridType &gt = k.gtype[g.grid];
+      if (!g.relative || g.off[0] || g.off[1] || g.off[2] || gt.rank != 3 || gt.elem != "float" ||
+          !gt.utype.empty())
+        ok = false;
+      point.insert(g.grid);
+    }
+    // The index names may appear only inside grid reads (update() has no
+    // point coordinates), and PSGridDim not at all.
+    for (size_t i = f.lbrace + 1; i < f.rbrace && ok; ++i) {
+      const std::string &s = t_[i].s;
+      if ((s == "PSGridGet" || s == "PSGridGetPeriodic") && i + 1 < f.rbrace && t_[i + 1].s == "(") {
+        i = MatchClose(t_, i + 1);
+        continue;
+      }
+      if (s == "PSGridDim") ok = false;
+      for (const std::string &n : k.idx)
+        if (t_[i].kind == Tok::kIdent && s == n) ok = false;
+    }
+    for (const Param &p : k.all)
+      if (p.is_grid && point.count(p.name)) k.point_grids.push_back(p.name);
+    if (ok && !k.point_grids.empty() && (int)k.point_grids.size() <= 16) {
+      k.point = true;
+      k.out_grid = out;
+    } else {
+      k.point_grids.clear();
     }
   }
   if (k.star && !k.wrap) {
@@ -899,6 +931,32 @@ int Translator::ClampIf(const Kernel &k, size_t i, size_t e) const {
   std::string v2;
   if (!assign(q + 1, v, nb, &v2)) return -1;
   return d;
+}
+
+bool Translator::EmitIsTail(const Func &f) const {
+  size_t emit_at = 0;
+  int depth = 0;
+  bool top = false;
+  for (size_t i = f.lbrace + 1; i < f.rbrace; ++i) {
+    const std::string &s = t_[i].s;
+    if (s == "{") ++depth;
+    else if (s == "}") --depth;
+    else if (s == "PSGridEmit") {
+      emit_at = i;
+      top = depth == 0;
+    } else if (s == "return" && i + 1 < f.rbrace && t_[i + 1].s != ";") {
+      top = false;  // value returns are not part of the DSL
+    }
+  }
+  if (!top || !emit_at) return false;
+  size_t close = MatchClose(t_, emit_at + 1);
+  size_t q = close + 1;
+  if (q < f.rbrace && t_[q].s == ";") ++q;
+  if (q < f.rbrace && t_[q].s == "return" && q + 1 < f.rbrace && t_[q + 1].s == ";") q += 2;
+  if (q != f.rbrace) return false;
+  for (size_t i = f.lbrace + 1; i < emit_at; ++i)  // no early "return;"
+    if (t_[i].s == "return") return false;
+  return true;
 }
 
 std::string Translator::RewriteBody(const Kernel &k, size_t b, size_t e, bool compute_form, bool exact) {
@@ -1012,7 +1070,14 @@ std::string Translator::RewriteBody(const Kernel &k, size_t b, size_t e, bool co
         acc.relative = gt.rank <= k.rank;
         for (int d = 0; d < gt.rank && acc.relative; ++d)
           acc.relative = RelativeIndex(t_, args[d + 1].first, args[d + 1].second, k.idx[d], &acc.off[d]);
-        if (g == k.staged && acc.relative) {
+        if (point_mode_) {
+          if (g == k.staged) {
+            r << "((float)__PSat(" << acc.off[0] << ", " << acc.off[1] << ", " << acc.off[2] << "))";
+          } else {
+            const auto it = std::find(k.point_grids.begin(), k.point_grids.end(), g);
+            r << "((float)__PSpt(" << (it - k.point_grids.begin()) << "))";
+          }
+        } else if (g == k.staged && acc.relative) {
           r << "((" << gt.elem << ")__ctx.at(0, " << acc.off[0] << ", " << acc.off[1] << ", " << acc.off[2]
             << "))";
         } else if (!uref.empty() && gt.rank == 3 && tk.s != "PSGridGetPeriodic") {
@@ -1128,6 +1193,30 @@ std::string Translator::EmitPlan(const Kernel &k) {
       << params << ") {\n";
     o << "    (void)__ctx;\n" << prologue();
     o << "    {\n" << RewriteBody(k, f.lbrace + 1, f.rbrace, true, true) << "    }\n  }\n";
+  }
+  if (k.point) {
+    // psk::PointTB's plan interface: the point grids in parameter order, the
+    // scalars, and the body as update(): __PSpt(n) is point grid n at the
+    // point, __PSat(dx, dy, dz) the staged grid at an offset; the body is
+    // evaluated as written whatever the storage (as point() evaluates it).
+    o << "  static constexpr int NG = " << k.point_grids.size() << ";\n";
+    o << "  struct Scalars {\n";
+    int nsc = 0;
+    for (const Param &p : k.all)
+      if (!p.is_grid) {
+        o << "    " << StripConst(p.type) << " " << p.name << ";\n";
+        ++nsc;
+      }
+    if (!nsc) o << "    int __none;\n";
+    o << "  };\n";
+    o << "  template <bool __kFma, class __G, class __PF>\n"
+      << "  __device__ static float update(const __G &__PSpt, const __PF &__PSat, const Scalars &__s, float *) {\n";
+    for (const Param &p : k.all)
+      if (!p.is_grid) o << "    " << p.type << " " << p.name << " = __s." << p.name << ";\n    (void)" << p.name << ";\n";
+    o << "    (void)__s;\n";
+    point_mode_ = true;
+    o << "    {\n" << RewriteBody(k, f.lbrace + 1, f.rbrace, true) << "    }\n  }\n";
+    point_mode_ = false;
   }
   o << "};\n\n";
   return o.str();
@@ -1249,6 +1338,62 @@ std::string Translator::EmitMapAndLaunch(const Kernel &k) {
     if (cfg_.error_check) o << "  __PSCheckDeviceError(\"" << f.name << "\");\n";
     o << "}\n\n";
   }
+  if (k.point) {
+    // Two maps of a point-form kernel that differ only by swapping the staged
+    // and the emitted grid, over the interior of the staged grid, FP32
+    // storage, every grid of one shape below 4 GiB: two sweeps per HBM pass
+    // (psk::RunPointTB), leaving both grids as the separate sweeps do.
+    const std::string &in = k.staged, &out = k.out_grid;
+    o << "static int __PSPointFusable_" << f.name << "(const struct " << S << " *a, const struct " << S
+      << " *b) {\n";
+    o << "  if (a->" << in << " != b->" << out << " || a->" << out << " != b->" << in << " || a->" << in
+      << " == a->" << out << ") return 0;\n";
+    for (const Param &p : k.all) {
+      if (p.name == in || p.name == out) continue;
+      if (p.is_grid) {
+        o << "  if (a->" << p.name << " != b->" << p.name << ") return 0;\n";
+        o << "  if (a->" << p.name << " == a->" << in << " || a->" << p.name << " == a->" << out << ") return 0;\n";
+      } else {
+        o << "  if (memcmp(&a->" << p.name << ", &b->" << p.name << ", sizeof(a->" << p.name << ")) != 0) return 0;\n";
+      }
+    }
+    o << "  if (__PSStorage_" << f.name << "(a) != PS_HIP_STORAGE_NATIVE) return 0;\n";
+    o << "  const __PSGrid *g = a->" << in << ";\n";
+    o << "  for (int d = 0; d < 3; ++d) {\n"
+      << "    if (a->dom.local_min[d] != 1 || a->dom.local_max[d] != g->dim[d] - 1) return 0;\n"
+      << "    if (b->dom.local_min[d] != 1 || b->dom.local_max[d] != g->dim[d] - 1) return 0;\n  }\n";
+    o << "  const __PSGrid *gs[] = {";
+    bool first = true;
+    for (const Param &p : k.all)
+      if (p.is_grid) {
+        o << (first ? "" : ", ") << "a->" << p.name;
+        first = false;
+      }
+    o << "};\n";
+    o << "  for (const __PSGrid *x : gs) {\n"
+      << "    if (x->num_dims != 3 || x->dim[0] != g->dim[0] || x->dim[1] != g->dim[1] || x->dim[2] != g->dim[2]) "
+         "return 0;\n"
+      << "    if ((uint64_t)x->num_elms * sizeof(float) > 0xffffffffull) return 0;\n  }\n";
+    o << "  return 1;\n}\n\n";
+    o << "static void __PSRunPointTB_" << f.name << "(const struct " << S << " *a, int iter) {\n";
+    o << "  using SF = psk::Native<float>;\n";
+    o << "  psk::PointTBArgs<typename " << P << "<SF>::Scalars> t{};\n";
+    o << "  const __PSGrid *g = a->" << in << ";\n";
+    o << "  for (int d = 0; d < 3; ++d) t.dim[d] = g->dim[d];\n";
+    for (size_t n = 0; n < k.point_grids.size(); ++n)
+      o << "  t.c[" << n << "] = a->" << k.point_grids[n] << "->p0;\n";
+    for (const Param &p : k.all)
+      if (!p.is_grid) o << "  t.sc." << p.name << " = a->" << p.name << ";\n";
+    o << "  void *X = __PSHipAuxBuffer((size_t)g->num_elms * sizeof(float));\n";
+    o << "  if (!X) {\n    fprintf(stderr, \"physis-hip: no memory for the fused-sweep buffer\\n\");\n"
+      << "    PSAbort(1);\n  }\n";
+    o << "  psk::CheckLaunch(psk::RunPointTB<" << P << "<SF>, SF>(t, a->" << in << "->p0, a->" << out
+      << "->p0, X, 2 * iter, (hipStream_t)__PSGetStream()), \"" << f.name << "\");\n";
+    o << "  if (__ps_trace) fprintf(__ps_trace, \"Physis: fused sweeps: %d in passes of two on the point tile\\n\", "
+         "2 * iter);\n";
+    if (cfg_.error_check) o << "  __PSCheckDeviceError(\"" << f.name << "\");\n";
+    o << "}\n\n";
+  }
   return o.str();
 }
 
@@ -1266,10 +1411,16 @@ std::string Translator::EmitRun(const std::vector<std::pair<std::string, std::st
   }
   bool fusable = cfg_.fuse && maps.size() == 2 && maps[0].first == maps[1].first &&
                  kernels_.at(maps[0].first).fusable && modes[0] == 0 && modes[1] == 0;
+  bool point = cfg_.fuse && maps.size() == 2 && maps[0].first == maps[1].first && kernels_.at(maps[0].first).point &&
+               modes[0] == 0 && modes[1] == 0;
   if (fusable) {
     const std::string &kn = maps[0].first;
     o << "  if (iter > 1 && __PSFusable_" << kn << "(&s0, &s1)) {\n";
     o << "    __PSRunPingPong_" << kn << "(&s0, &s1, iter);\n  } else {\n";
+  } else if (point) {
+    const std::string &kn = maps[0].first;
+    o << "  if (iter > 1 && __PSPointFusable_" << kn << "(&s0, &s1)) {\n";
+    o << "    __PSRunPointTB_" << kn << "(&s0, iter);\n  } else {\n";
   }
   o << "  for (int i = 0; i < iter; ++i) {\n";
   // A red-black map runs its red then its black pass (reference
@@ -1284,7 +1435,7 @@ std::string Translator::EmitRun(const std::vector<std::pair<std::string, std::st
     }
   }
   o << "  }\n";
-  if (fusable) o << "  }\n";
+  if (fusable || point) o << "  }\n";
   if (cfg_.trace) {
     o << "  (void)hipStreamSynchronize((hipStream_t)__PSGetStream());\n";
     o << "  float f = __PSStopwatchStop(&__st);\n  __PSTraceStencilPost(f);\n  return f;\n";

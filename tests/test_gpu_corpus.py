@@ -257,3 +257,57 @@ def test_translated_plan_with_direct_grids_both_extents_paths(tmp_path, mode):
     got = np.fromfile(dst, np.float32).reshape(2, nz, ny, nx)
     want_p, want_q = coef_extents_oracle(p, a, b, c, 0.25, it)
     assert np.array_equal(got[0], want_p) and np.array_equal(got[1], want_q)
+
+
+def point_relax_oracle(p, q, a, b, c, w, iters):
+    """FP32 evaluation of tests/golden/programs/point_relax.c in its own
+    order over the interior: s = (a*(p(x+1) + p(x-1)) + b*(p(y+1,z-1) -
+    p(y-1,z+1))) + c*p(x+1,y+1,z+1); v = p + w*(s - p); maps p -> q then
+    q -> p (arrays indexed [z, y, x])."""
+    import numpy as np
+    w = np.float32(w)
+    nz, ny, nx = p.shape
+    I = (slice(1, nz - 1), slice(1, ny - 1), slice(1, nx - 1))
+
+    def at(g, dx, dy, dz):
+        return g[1 + dz:nz - 1 + dz, 1 + dy:ny - 1 + dy, 1 + dx:nx - 1 + dx]
+
+    def relax(src, dst):
+        out = dst.copy()
+        s = a[I] * (at(src, 1, 0, 0) + at(src, -1, 0, 0))
+        s = s + b[I] * (at(src, 0, 1, -1) - at(src, 0, -1, 1))
+        s = s + c[I] * at(src, 1, 1, 1)
+        out[I] = src[I] + w * (s - src[I])
+        return out
+
+    for _ in range(iters):
+        q = relax(p, q)
+        p = relax(q, p)
+    return p, q
+
+
+@pytest.mark.parametrize("shape,iters", [((70, 40, 23), 1), ((70, 40, 23), 2), ((70, 40, 23), 3),
+                                         ((129, 17, 9), 5), ((19, 7, 5), 4), ((3, 3, 3), 2), ((200, 61, 45), 6)])
+def test_point_form_map_fuses_two_sweeps_bitwise(tmp_path, shape, iters):
+    """A ping-pong of a point-form kernel (staged grid within radius 1 with
+    diagonals, three coefficient grids at the point, the interior domain)
+    runs two sweeps per pass from two iterations on (the trace says so),
+    one sweep per launch at one; both grids equal an FP32 evaluation in
+    the program's order bitwise (q starts different from p, shell
+    included, so a pass that took the wrong iterate's shell would show)."""
+    import numpy as np
+    from oracle import oracle as O
+    nx, ny, nz = shape
+    p = O.uniform_field(shape, seed=21)
+    q = O.uniform_field(shape, seed=22, lo=-1.0, hi=1.0)
+    a, b, c = (O.uniform_field(shape, seed=s, lo=0.1, hi=0.4) for s in (23, 24, 25))
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    np.concatenate([x.ravel() for x in (p, q, a, b, c)]).astype(np.float32).tofile(src)
+    print("point_relax", shape, "iter", iters, flush=True)
+    out = subprocess.run([exe("point_relax"), str(nx), str(ny), str(nz), str(iters), str(src), str(dst),
+                          "--physis-trace"], capture_output=True, timeout=120)
+    assert out.returncode == 0, out.stderr.decode()[-2000:]
+    assert ("fused sweeps: %d" % (2 * iters) in out.stderr.decode()) == (iters > 1)
+    got = np.fromfile(dst, np.float32).reshape(2, nz, ny, nx)
+    want_p, want_q = point_relax_oracle(p, q, a, b, c, 0.7, iters)
+    assert np.array_equal(got[0], want_p) and np.array_equal(got[1], want_q)

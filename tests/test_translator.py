@@ -229,3 +229,50 @@ def test_plan_reading_many_grids_has_a_uniform_extents_variant(tmp_path):
     assert "psk::UniformExtents(a.p, a.q, a.a, a.b, a.c, a.d))" in launch
     assert "LaunchRing<__PSPlan_relax<SF, true>, 3>(__PSArgs_relax<SF, true>(s)" in launch
     assert "LaunchRing<__PSPlan_relax<SF>, 3>(a, box, st, parity)" in launch
+
+
+POINT_KERNEL = """
+#include "physis/physis.h"
+static void relax(const int x, const int y, const int z, PSGrid3DFloat p, PSGrid3DFloat q, PSGrid3DFloat a,
+                  PSGrid3DFloat b, float w) {
+  float s = PSGridGet(a, x, y, z) * (PSGridGet(p, x + 1, y - 1, z) + PSGridGet(p, x, y, z - 1))
+            + PSGridGet(b, %(bidx)s) * PSGridGet(p, x, y, z);
+  PSGridEmit(q, %(emit)s);
+  return;
+}
+int main(int argc, char *argv[]) {
+  PSInit(&argc, &argv, 3, 16, 16, 16);
+  PSGrid3DFloat p = PSGrid3DFloatNew(16, 16, 16);
+  PSGrid3DFloat q = PSGrid3DFloatNew(16, 16, 16);
+  PSGrid3DFloat a = PSGrid3DFloatNew(16, 16, 16);
+  PSGrid3DFloat b = PSGrid3DFloatNew(16, 16, 16);
+  PSDomain3D d = PSDomain3DNew(1, 15, 1, 15, 1, 15);
+  PSStencilRun(PSStencilMap(relax, d, p, q, a, b, 0.5f), PSStencilMap(relax, d, q, p, a, b, 0.5f), 4);
+  PSFinalize();
+  return 0;
+}
+"""
+
+
+def test_point_form_kernel_gets_update_and_fused_run(tmp_path):
+    """A kernel reading its staged grid within radius 1 (diagonals included)
+    and other grids only at the point gets update() for psk::PointTB: the
+    staged reads become __PSat(dx, dy, dz), the point grids __PSpt(n) in
+    parameter order, the emit a return; the run tries the fused sweeps
+    first."""
+    hip = translate_text(tmp_path, POINT_KERNEL % {"bidx": "x, y, z", "emit": "s"})
+    upd = hip[hip.find("static float update("):hip.find("\n  }\n", hip.find("static float update("))]
+    assert "((float)__PSpt(0)) * (((float)__PSat(1, -1, 0)) +((float)__PSat(0, 0, -1)))" in upd
+    assert "((float)__PSpt(1)) *((float)__PSat(0, 0, 0))" in upd
+    assert "return (float)(s);" in upd and "float w = __s.w;" in upd
+    assert "static constexpr int NG = 2;" in hip
+    assert "__PSPointFusable_relax(&s0, &s1)" in hip and "__PSRunPointTB_relax(&s0, iter)" in hip
+    assert "a->dom.local_min[d] != 1 || a->dom.local_max[d] != g->dim[d] - 1" in hip
+
+
+@pytest.mark.parametrize("bidx,emit", [("x, y, z + 1", "s"), ("x, y, z", "s + (float)x")])
+def test_kernels_outside_the_point_form_run_unfused(tmp_path, bidx, emit):
+    """A second grid read off the point, or an index name used outside a
+    grid read (update() has no coordinates), keeps the per-sweep path."""
+    hip = translate_text(tmp_path, POINT_KERNEL % {"bidx": bidx, "emit": emit})
+    assert "update(" not in hip and "__PSPointFusable_" not in hip
