@@ -175,7 +175,7 @@ def test_kernel_reading_its_output_grid_runs_unfused_and_matches(tmp_path, shape
     assert np.array_equal(got[0], want_u) and np.array_equal(got[1], want_v)
 
 
-@pytest.mark.parametrize("size,dims", [("XS", (64, 32, 32)), ("S", (128, 64, 64))])
+@pytest.mark.parametrize("size,dims", [("XS", (64, 32, 32)), ("S", (128, 64, 64)), ("M", (256, 128, 128))])
 def test_reference_himeno_program_translated(tmp_path, size, dims):
     """The reference's Himeno program (examples/himeno/himenobmtxpa_physis.c,
     the 19-point Jacobi of the Himeno workload), translated unchanged by
