@@ -373,14 +373,13 @@ def storage16_bench(rt, n, steps=200):
 HIMENO_FUSED_SHAPES = [(2, 8, 1, 8), (2, 8, 1, 16), (2, 8, 1, 32), (2, 16, 1, 32), (3, 16, 1, 32)]
 
 
-def himeno_bench(rt, sweeps=400, seed=300):
-    """Himeno workload: 19-point Jacobi on 257x129x129, coefficients
+def himeno_bench(rt, sweeps=400, seed=300, shape=(257, 129, 129)):
+    """Himeno workload: 19-point Jacobi on 257x129x129 (or `shape`), coefficients
     a0..a2, b0..b2, c0..c2 uniform [0.9, 1.1], a3 uniform [0.15, 0.18], bnd 1,
     wrk1 0, p = (z/(nz-1))^2, omega 0.8, 400 sweeps with the FP64 sum of ss^2
     per sweep; FP32 and BF16 storage, one sweep per pass and the fastest
     fused shape (several sweeps per HBM pass).  Bytes per sweep follow the
     reference's GetThroughput (himenobmtxpa_physis.c:418-432)."""
-    shape = (257, 129, 129)
     nx, ny, nz = shape
     interior = (nx - 2) * (ny - 2) * (nz - 2)
     whole = nx * ny * nz
@@ -565,6 +564,9 @@ def main():
         batched = batched_bench(rt)
         print("batched: %s" % json.dumps(batched), file=sys.stderr, flush=True)
     himeno = himeno_bench(rt)
+    # The same legs on 384^3 (40 sweeps): a grid past the 256 MB Infinity
+    # Cache, where two sweeps per pass pay for BF16 storage too.
+    himeno["large_384"] = himeno_bench(rt, sweeps=40, shape=(384, 384, 384))
     print("himeno: %s" % json.dumps(himeno), file=sys.stderr, flush=True)
     served = None
     if not args.no_served:
