@@ -439,6 +439,39 @@ def himeno_bench(rt, sweeps=400, seed=300, shape=(257, 129, 129)):
     return out
 
 
+def himeno_dropin_bench(sweeps=400, shape=(257, 129, 129)):
+    """The drop-in path on the reference's Himeno program
+    (examples/himeno/himenobmtxpa_physis.c) translated by physisc-hip with
+    TRACE_KERNEL, its jacobi driven by tests/golden/himeno_timing.cc with the
+    program's own initial values (a warm run of 4 sweeps, then `sweeps`); the
+    time is the one its traced PSStencilRun reports.  The program's jacobi
+    computes no residual (its gosa reduction is commented out,
+    himenobmtxpa_physis.c:387-391)."""
+    exe = os.path.join(ROOT, "tests", "_corpus_bin", "himeno_dropin_timing")
+    if not os.path.exists(exe):
+        return None
+    nx, ny, nz = shape
+    interior = (nx - 2) * (ny - 2) * (nz - 2)
+    whole = nx * ny * nz
+    out = {"workload": "reference examples/himeno/himenobmtxpa_physis.c translated by physisc-hip (TRACE_KERNEL), "
+                       "jacobi on %dx%dx%d, %d sweeps, program's own initial values" % (nx, ny, nz, sweeps)}
+    for name, extra, es in (("fp32", [], 4), ("bf16", ["bf16"], 2)):
+        try:
+            err = subprocess.run([exe, str(nx), str(ny), str(nz), str(sweeps)] + extra + ["--physis-trace"],
+                                 capture_output=True, text=True, timeout=120, check=True).stderr
+        except (subprocess.SubprocessError, OSError) as e:
+            print("Himeno drop-in timing failed: %s" % e, file=sys.stderr)
+            return None
+        times = re.findall(r"Stencil finished \(time: ([0-9.]+)\)", err)
+        if len(times) < 2:
+            return None
+        ms = float(times[-1])
+        out[name] = {"ms_per_sweep": round(ms / sweeps, 4), "glups": round(interior * sweeps / (ms / 1e3) / 1e9, 1),
+                     "gbs": round(es * (12 * interior + whole + interior) * sweeps / (ms / 1e3) / 1e9, 1),
+                     "fused": "fused sweeps" in err}
+    return out
+
+
 def reference_benchmark(n, steps):
     """The reference's diffusion benchmark (examples/diffusion-benchmark):
     its Physis program translated by physisc-hip and linked with its own
@@ -499,7 +532,9 @@ def main():
     from physis_amd import runtime as rt
     rt.init(["bench"])
     if args.himeno_only or args.only == "himeno":
-        print(json.dumps(himeno_bench(rt)), flush=True)
+        himeno = himeno_bench(rt)
+        himeno["dropin_program"] = himeno_dropin_bench()
+        print(json.dumps(himeno), flush=True)
         return
     if args.only:
         leg = {"served": lambda: served_bench(rt, verbose=True), "batched": lambda: batched_bench(rt),
@@ -567,6 +602,8 @@ def main():
     # The same legs on 384^3 (40 sweeps): a grid past the 256 MB Infinity
     # Cache, where two sweeps per pass pay for BF16 storage too.
     himeno["large_384"] = himeno_bench(rt, sweeps=40, shape=(384, 384, 384))
+    # The reference's own Himeno program through the drop-in path.
+    himeno["dropin_program"] = himeno_dropin_bench()
     print("himeno: %s" % json.dumps(himeno), file=sys.stderr, flush=True)
     served = None
     if not args.no_served:

@@ -11,8 +11,8 @@ main) is linked with its plain C driver test_*.module_base.c.  The
 reference's diffusion benchmark (examples/diffusion-benchmark) is built the
 same way: its Physis program translated, its C++ harness compiled for the
 Physis variant and linked in (tests/_corpus_bin/diffusion_benchmark_physis).
-The reference's Himeno program is built the same way (himeno_physis, see
-build_himeno).  The DSL programs of this repository under
+The reference's Himeno program is built the same way (himeno_physis, and
+himeno_dropin_timing for the benchmark, see build_himeno).  The DSL programs of this repository under
 tests/golden/programs are built into the same directory.
 """
 import json
@@ -69,6 +69,9 @@ def build_benchmark(phc):
 # its one-minute measured loop runs 18 sweeps.
 HIMENO_SRC = "/root/reference/examples/himeno/himenobmtxpa_physis.c"
 HIMENO_EXE = os.path.join(OUT, "himeno_physis")
+# The same program translated with TRACE_KERNEL and its main renamed, driven
+# by a timing harness of this repository (tests/golden/himeno_timing.cc).
+HIMENO_TIMING_EXE = os.path.join(OUT, "himeno_dropin_timing")
 
 
 def build_himeno(phc):
@@ -79,6 +82,14 @@ def build_himeno(phc):
     try:
         phc.translate(HIMENO_SRC, hip, None, defines)
         phc.compile_hip(hip, HIMENO_EXE, extra_c=[os.path.join(HERE, "himeno_clock.c")], defines=defines)
+        os.remove(hip)
+        cfg = os.path.join(OUT, "trace_kernel.cfg")
+        with open(cfg, "w") as f:
+            f.write("TRACE_KERNEL = true\n")
+        hip = HIMENO_TIMING_EXE + ".hip"
+        phc.translate(HIMENO_SRC, hip, cfg, ["main=himeno_program_main"])
+        phc.compile_hip(hip, HIMENO_TIMING_EXE, extra_cxx=[os.path.join(HERE, "himeno_timing.cc")],
+                        cxx_flags=["-I" + os.path.join(ROOT, "include"), "-w"])
         os.remove(hip)
         return True
     except subprocess.CalledProcessError:

@@ -11,6 +11,7 @@ user-defined point type checks) must exit 0.
 import hashlib
 import json
 import os
+import re
 import subprocess
 
 import pytest
@@ -206,6 +207,56 @@ def test_reference_himeno_program_translated(tmp_path, size, dims):
     want_lines = ["%f" % v for v in want.ravel().tolist()]
     bad = [i for i, (g, w) in enumerate(zip(got, want_lines)) if g != w]
     assert not bad, "%d points differ, first at %d: %s vs %s" % (len(bad), bad[0], got[bad[0]], want_lines[bad[0]])
+
+
+def bf16_round(x):
+    """FP32 values rounded to the nearest BF16 (ties to even), kept as FP32."""
+    import numpy as np
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.parametrize("dims,sweeps,storage", [((65, 33, 33), 8, "fp32"), ((66, 34, 17), 6, "fp32"),
+                                                 ((65, 33, 33), 8, "bf16")])
+def test_reference_himeno_jacobi_timing_harness(dims, sweeps, storage):
+    """The benchmark's drop-in Himeno leg: the reference's Himeno program
+    translated with TRACE_KERNEL and driven through its own mat_set,
+    mat_set_init and jacobi (tests/golden/himeno_timing.cc, 4 then `sweeps`
+    sweeps).  FP32 takes the fused point-form run; BF16 storage runs one map
+    per launch.  The FP64 sum of p in memory order equals, exactly, that of
+    the oracle's FP32 field after the same sweeps (for BF16: the oracle's
+    sweep on the BF16 values, its result rounded to BF16 after each sweep)."""
+    import numpy as np
+    from oracle import oracle as O
+    nx, ny, nz = dims
+    print("himeno timing harness", dims, sweeps, storage, flush=True)
+    args = [exe("himeno_dropin_timing"), str(nx), str(ny), str(nz), str(sweeps)]
+    out = subprocess.run(args + ([storage] if storage == "bf16" else []) + ["--physis-trace"],
+                         capture_output=True, timeout=120)
+    assert out.returncode == 0, out.stderr.decode()[-2000:]
+    err = out.stderr.decode()
+    assert len(re.findall(r"Stencil finished \(time: ([0-9.]+)\)", err)) == 2
+    assert ("fused sweeps" in err) == (storage == "fp32")
+    got = float(re.search(r"p0 sum: (\S+)", err).group(1))
+    k = np.arange(nz, dtype=np.int64)
+    col = (k * k).astype(np.float32) / np.float32((nz - 1) * (nz - 1))
+    p = np.ascontiguousarray(np.broadcast_to(col[:, None, None], (nz, ny, nx)), dtype=np.float32)
+    vals = [1.0, 1.0, 1.0, 1.0 / 6.0, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, 1.0, 0.0]
+    coefs = [np.full((nz, ny, nx), np.float32(v), np.float32) for v in vals]
+    if storage == "fp32":
+        want, _, _ = O.himeno(p, coefs, np.float32(0.8), 4 + sweeps)
+    else:
+        coefs = [bf16_round(c) for c in coefs]
+        src, dst = bf16_round(p), bf16_round(p)
+        for _ in range(4 + sweeps):
+            _, nxt, _ = O.himeno(src, coefs, np.float32(0.8), 1, p1_init=dst)
+            src, dst = bf16_round(nxt), src
+        want = src
+    total = 0.0
+    for v in want.ravel().tolist():
+        total += v
+    assert got == total
 
 
 def coef_extents_oracle(p, a, b, c, w, iters):
