@@ -25,6 +25,7 @@
 #include "physis/hip/skeleton_star_tb.h"
 #include "physis/hip/skeleton_stride.h"
 #include "physis/hip/storage.h"
+#include "physis/hip/wave_reduce.h"
 
 namespace psk {
 
@@ -300,6 +301,76 @@ inline void RunPingPong(typename P::Args ab, typename P::Args ba, int iter, int 
     CheckLaunch(LaunchStarStep<P>(ab, box, nx, ny, nz, s), "step");
     CheckLaunch(LaunchStarStep<P>(ba, box, nx, ny, nz, s), "step");
   }
+}
+
+// ---- kernel reductions: PSReduce(&v, op, kernel, dom, args...) -------------
+// (docs/programming.md "Reduction" of the reference; its translators abort on
+// it, reference_translator.cc:1357-1360.)  RP::value(a, x, y, z) is the
+// reduction kernel's body evaluated at a point of the domain.  The domain is
+// cut into items of up to kReduceMapRow consecutive x points of one row;
+// workgroup b takes items b, b + nb, ... in order, each thread folding the
+// values of its points in FP64 (exact for MAX and MIN), and writes one
+// partial; the runtime then folds the partials in index order.  The split
+// depends only on the domain's extents, so a domain reduces to the same bits
+// on every call.
+constexpr int kReduceMapThreads = 256;
+constexpr int kReduceMapRow = 4 * kReduceMapThreads;
+constexpr int kReduceMapMaxGroups = 16384;
+
+template <class Op> struct ReduceIdentity;
+template <> struct ReduceIdentity<OpSum> { static constexpr double v = 0.0; };
+template <> struct ReduceIdentity<OpProd> { static constexpr double v = 1.0; };
+template <> struct ReduceIdentity<OpMax> { static constexpr double v = -__builtin_inf(); };
+template <> struct ReduceIdentity<OpMin> { static constexpr double v = __builtin_inf(); };
+
+template <class RP, class Op>
+__global__ void __launch_bounds__(kReduceMapThreads)
+ReduceMapKernel(typename RP::Args a, Box box, int row_items, int64_t items, double *partials) {
+  __shared__ double lds[kReduceMapThreads / 64];
+  Op op;
+  const int w1 = box.hi[1] - box.lo[1];
+  double acc = ReduceIdentity<Op>::v;
+  for (int64_t j = blockIdx.x; j < items; j += gridDim.x) {
+    const int64_t row = j / row_items;
+    const int xc = (int)(j - row * row_items);
+    const int y = box.lo[1] + (int)(row % w1), z = box.lo[2] + (int)(row / w1);
+    const int x0 = box.lo[0] + xc * kReduceMapRow + (int)threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < kReduceMapRow / kReduceMapThreads; ++m) {
+      const int x = x0 + m * kReduceMapThreads;
+      if (x < box.hi[0]) acc = op(acc, (double)RP::value(a, x, y, z));
+    }
+  }
+  const double r = block_reduce<kReduceMapThreads>(acc, op, lds, (int)threadIdx.x);
+  if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// Reduces RP's values over the box with op; the FP64 result goes to *out
+// (the operator's identity for an empty box).  Waits for the result.
+template <class RP>
+inline hipError_t RunReduceMap(const typename RP::Args &a, const Box &box, enum PSReduceOp op, double *out,
+                               hipStream_t s) {
+  const int64_t w0 = box.hi[0] - box.lo[0], w1 = box.hi[1] - box.lo[1], w2 = box.hi[2] - box.lo[2];
+  if (w0 <= 0 || w1 <= 0 || w2 <= 0) {
+    *out = op == PS_SUM ? 0.0 : op == PS_PROD ? 1.0 : op == PS_MAX ? -__builtin_inf() : __builtin_inf();
+    return op >= PS_MAX && op <= PS_PROD ? hipSuccess : hipErrorInvalidValue;
+  }
+  const int row_items = (int)((w0 + kReduceMapRow - 1) / kReduceMapRow);
+  const int64_t items = (int64_t)row_items * w1 * w2;
+  const int nb = (int)std::min<int64_t>(items, kReduceMapMaxGroups);
+  double *partials = __PSHipReduceMapPartials(nb);
+  if (!partials) return hipErrorOutOfMemory;
+  const dim3 grid(nb), block(kReduceMapThreads);
+  switch (op) {
+    case PS_SUM: hipLaunchKernelGGL((ReduceMapKernel<RP, OpSum>), grid, block, 0, s, a, box, row_items, items, partials); break;
+    case PS_PROD: hipLaunchKernelGGL((ReduceMapKernel<RP, OpProd>), grid, block, 0, s, a, box, row_items, items, partials); break;
+    case PS_MAX: hipLaunchKernelGGL((ReduceMapKernel<RP, OpMax>), grid, block, 0, s, a, box, row_items, items, partials); break;
+    case PS_MIN: hipLaunchKernelGGL((ReduceMapKernel<RP, OpMin>), grid, block, 0, s, a, box, row_items, items, partials); break;
+    default: return hipErrorInvalidValue;
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return __PSHipReduceMapFinish(out, op, partials, nb) == PS_HIP_OK ? hipSuccess : hipErrorUnknown;
 }
 
 }  // namespace psk

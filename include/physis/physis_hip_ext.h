@@ -88,6 +88,17 @@ extern int PSHipReduceGridNoAlloc(double *out, enum PSReduceOp op, __PSGrid *g);
  * buffer).  NULL if it cannot be allocated. */
 extern void *__PSHipAuxBuffer(size_t bytes);
 
+/* Kernel reductions, PSReduce(&v, op, kernel, dom, ...) (documented in the
+ * reference's docs/programming.md "Reduction" but not implemented there,
+ * reference_translator.cc:1357-1360), as physisc-hip lowers them
+ * (physis/hip/generated.h RunReduceMap): the reduction kernel's pass writes
+ * one FP64 partial per workgroup into __PSHipReduceMapPartials(n) (n <= 16384;
+ * NULL outside that range), then __PSHipReduceMapFinish folds them in index
+ * order with op on the runtime stream, waits, and stores the FP64 result in
+ * *out (a PSHipStatus). */
+extern double *__PSHipReduceMapPartials(int n);
+extern int __PSHipReduceMapFinish(double *out, enum PSReduceOp op, const double *partials, int n);
+
 /* Blocks until the runtime stream is idle; returns a PSHipStatus. */
 extern int PSHipSynchronize(void);
 /* Device bytes held by live grids. */

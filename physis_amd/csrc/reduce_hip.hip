@@ -285,6 +285,38 @@ int PSHipReduceGridNoAlloc(double *out, enum PSReduceOp op, __PSGrid *g) {
   return hipStreamSynchronize(s) == hipSuccess ? PS_HIP_OK : PS_HIP_ERR_DEVICE;
 }
 
+// Partials of a kernel reduction (PSReduce(&v, op, kernel, dom, ...), lowered
+// by physisc-hip: include/physis/hip/generated.h RunReduceMap): n FP64 values
+// in the runtime's scratch buffer, used in stream order.
+double *__PSHipReduceMapPartials(int n) {
+  if (n < 1 || n > kMaxBlocks) return nullptr;
+  return static_cast<double *>(Runtime::Get().Scratch((size_t)n * sizeof(double)));
+}
+
+// Folds the n partials a kernel reduction queued on the runtime stream, in
+// index order (the pass 2 of PSHipReduceGrid), waits for it and stores the
+// FP64 result in *out.
+int __PSHipReduceMapFinish(double *out, enum PSReduceOp op, const double *partials, int n) {
+  if (!out || !partials || n < 1 || n > kMaxBlocks) return PS_HIP_ERR_INVALID;
+  Runtime &rt = Runtime::Get();
+  hipStream_t s = rt.stream();
+  double *host = static_cast<double *>(rt.PinnedScratch(sizeof(double)));
+  double *host_dev = nullptr;
+  if (hipHostGetDevicePointer(reinterpret_cast<void **>(&host_dev), host, 0) != hipSuccess)
+    return PS_HIP_ERR_DEVICE;
+  switch (op) {
+    case PS_SUM: hipLaunchKernelGGL((reduce_pass2<psk::OpSum>), dim3(1), dim3(kThreads2), 0, s, partials, n, host_dev); break;
+    case PS_PROD: hipLaunchKernelGGL((reduce_pass2<psk::OpProd>), dim3(1), dim3(kThreads2), 0, s, partials, n, host_dev); break;
+    case PS_MAX: hipLaunchKernelGGL((reduce_pass2<psk::OpMax>), dim3(1), dim3(kThreads2), 0, s, partials, n, host_dev); break;
+    case PS_MIN: hipLaunchKernelGGL((reduce_pass2<psk::OpMin>), dim3(1), dim3(kThreads2), 0, s, partials, n, host_dev); break;
+    default: return PS_HIP_ERR_INVALID;
+  }
+  if (hipGetLastError() != hipSuccess) return PS_HIP_ERR_DEVICE;
+  if (hipStreamSynchronize(s) != hipSuccess) return PS_HIP_ERR_DEVICE;
+  *out = *static_cast<volatile double *>(host);
+  return PS_HIP_OK;
+}
+
 // PSReduce(&v, op, g) on a float grid (reference libphysis_rt_ref.cc:150-153).
 void __PSReduceGridFloat(void *buf, enum PSReduceOp op, __PSGrid *g) {
   if (op < PS_MAX || op > PS_PROD) PSAbort(1);  // invalid operator (runtime/reduce.h:49-51)

@@ -17,6 +17,14 @@
 //                                      defaulting to 1 (reference_translator.cc:1249-1283)
 //   PSReduce(&v, op, g)              -> __PSReduceGridFloat/Double(&v, op, g)
 //                                      (reference_translator.cc:1315-1351)
+//   PSReduce(&v, op, k, dom, args..) -> __PSReduceMap_k(&v, op, dom, args..):
+//                                      the reduction kernel k (returns float or
+//                                      double) evaluated at every point of dom
+//                                      and folded in FP64 (psk::RunReduceMap);
+//                                      documented in the reference
+//                                      (docs/programming.md, "Reduction") but
+//                                      not implemented by its translators
+//                                      (reference_translator.cc:1357-1360)
 //   kernel k                         -> template <class SF> struct __PSPlan_k, a
 //                                      Plan for the skeletons in include/physis/hip
 //   PSGridGet(g, x+dx, y+dy, z+dz)   -> LDS/register tap for the staged grid,
@@ -39,8 +47,8 @@
 //   PSStencilMapRedBlack/Red/Black   -> the map run over the points of one or
 //                                      both colours (x+y+z+c even)
 //
-// Not supported (rejected with a message): kernel reductions, grids
-// captured from globals, nested structs inside point types.
+// Not supported (rejected with a message): grids captured from globals,
+// nested structs inside point types.
 //
 // Usage: physisc-hip [--config FILE] [-o OUT] [-I DIR]... [-D NAME[=V]]... INPUT.c
 // Config keys (key = value lines): TRACE_KERNEL, KERNEL_ERROR_CHECK (bools),
@@ -649,6 +657,11 @@ class Translator {
   int ClampIf(const Kernel &k, size_t i, size_t e) const;
   std::string EmitPlan(const Kernel &k);
   std::string EmitMapAndLaunch(const Kernel &k);
+  // Locals bound from a plan's Args at the top of a body.
+  std::string Prologue(const Kernel &k) const;
+  // A reduction kernel (the third argument of a PSReduce with a domain):
+  // its value plan and the host function the PSReduce call becomes.
+  std::string EmitReduceKernel(const Kernel &k);
   std::string RewriteHost(size_t b, size_t e);
   // modes[i]: 0 plain map, 1 red-black (red then black pass), 2 red only, 3 black only.
   std::string EmitRun(const std::vector<std::pair<std::string, std::string>> &maps, const std::vector<int> &modes,
@@ -659,6 +672,7 @@ class Translator {
   std::string input_;
   std::vector<Func> funcs_;
   std::map<std::string, Kernel> kernels_;
+  std::map<std::string, Kernel> rkernels_;  // reduction kernels
   std::map<std::string, GridType> gridvars_;
   std::vector<std::string> runs_;  // emitted run functions
   int run_id_ = 0;
@@ -1112,6 +1126,18 @@ std::string Translator::RewriteBody(const Kernel &k, size_t b, size_t e, bool co
   return out;
 }
 
+std::string Translator::Prologue(const Kernel &k) const {
+  std::ostringstream q;
+  for (const Param &p : k.all) {
+    if (p.is_grid) q << "    const auto &" << p.name << " = __a." << p.name << ";\n";
+    else q << "    " << p.type << " " << p.name << " = __a." << p.name << ";\n";
+  }
+  for (const Param &p : k.all) q << "    (void)" << p.name << ";\n";
+  const std::string uref = UniformRef(k);
+  if (!uref.empty()) q << "    const auto &__ref = __a." << uref << ";\n    (void)__ref;\n";
+  return q.str();
+}
+
 std::string Translator::EmitPlan(const Kernel &k) {
   const Func &f = *k.f;
   std::ostringstream o;
@@ -1161,17 +1187,7 @@ std::string Translator::EmitPlan(const Kernel &k) {
       << k.out_grid << ".p; }\n";
     o << "  static void set_out(Args &a, typename Storage::stored_t *p) { a." << k.out_grid << ".p = p; }\n";
   }
-  auto prologue = [&]() {
-    std::ostringstream q;
-    for (const Param &p : k.all) {
-      if (p.is_grid) q << "    const auto &" << p.name << " = __a." << p.name << ";\n";
-      else q << "    " << p.type << " " << p.name << " = __a." << p.name << ";\n";
-    }
-    for (const Param &p : k.all) q << "    (void)" << p.name << ";\n";
-    const std::string uref = UniformRef(k);
-    if (!uref.empty()) q << "    const auto &__ref = __a." << uref << ";\n    (void)__ref;\n";
-    return q.str();
-  };
+  auto prologue = [&]() { return Prologue(k); };
   std::string params = "const C &__ctx, const Args &__a";
   const char *dummies[3] = {"__i0", "__i1", "__i2"};
   for (int d = 0; d < 3; ++d) params += ", const int " + (d < k.rank ? k.idx[d] : std::string(dummies[d]));
@@ -1219,6 +1235,71 @@ std::string Translator::EmitPlan(const Kernel &k) {
     point_mode_ = false;
   }
   o << "};\n\n";
+  return o.str();
+}
+
+std::string Translator::EmitReduceKernel(const Kernel &k) {
+  const Func &f = *k.f;
+  const std::string T = t_[f.lparen - 2].s;
+  const std::string P = "__PSRPlan_" + f.name;
+  std::ostringstream o;
+  o << "// Reduction kernel " << f.name << " (rank " << k.rank << ", value " << T << ")\n";
+  o << "template <class SF, bool __kU = false>\nstruct " << P << " {\n";
+  o << "  struct Args {\n";
+  for (const Param &p : k.all) {
+    if (p.is_grid && !p.gt.utype.empty()) o << "    struct __PSURef_" << p.gt.utype << " " << p.name << ";\n";
+    else if (p.is_grid) o << "    psk::GridRef<" << StorageOf(p.gt) << "> " << p.name << ";\n";
+    else o << "    " << StripConst(p.type) << " " << p.name << ";\n";
+  }
+  o << "  };\n";
+  std::string params = "const Args &__a";
+  const char *dummies[3] = {"__i0", "__i1", "__i2"};
+  for (int d = 0; d < 3; ++d) params += ", const int " + (d < k.rank ? k.idx[d] : std::string(dummies[d]));
+  o << "  __device__ static " << T << " value(" << params << ") {\n" << Prologue(k);
+  o << "    {\n" << RewriteBody(k, f.lbrace + 1, f.rbrace, false) << "    }\n  }\n};\n\n";
+  std::string decl, names;
+  for (const Param &p : k.all) {
+    decl += std::string(", ") + (p.is_grid ? "__PSGrid *" : p.type + " ") + p.name;
+    names += ", " + p.name;
+  }
+  o << "template <class SF>\nstatic void __PSReduceMapT_" << f.name << "(" << T
+    << " *__out, enum PSReduceOp __op, __PSDomain __dom" << decl << ") {\n";
+  o << "  typename " << P << "<SF>::Args __a;\n";
+  for (const Param &p : k.all) {
+    if (p.is_grid && !p.gt.utype.empty())
+      o << "  __a." << p.name << " = __PSUMakeRef_" << p.gt.utype << "(" << p.name << ");\n";
+    else if (p.is_grid)
+      o << "  __a." << p.name << " = psk::MakeRef<" << (p.gt.elem == "float" ? "SF" : StorageOf(p.gt)) << ">("
+        << p.name << ");\n";
+    else
+      o << "  __a." << p.name << " = " << p.name << ";\n";
+  }
+  o << "  double __r = 0.0;\n";
+  o << "  psk::CheckLaunch(psk::RunReduceMap<" << P << "<SF>>(__a, psk::BoxFromDomain(__dom, " << k.rank
+    << "), __op, &__r, (hipStream_t)__PSGetStream()), \"" << f.name << "\");\n";
+  o << "  *__out = (" << T << ")__r;\n}\n\n";
+  std::vector<std::string> floats;
+  for (const Param &p : k.all)
+    if (p.is_grid && p.gt.elem == "float") floats.push_back(p.name);
+  o << "static void __PSReduceMap_" << f.name << "(" << T << " *__out, enum PSReduceOp __op, __PSDomain __dom"
+    << decl << ") {\n";
+  o << "  if (__op < PS_MAX || __op > PS_PROD) PSAbort(1);\n";
+  if (floats.empty()) {
+    o << "  __PSReduceMapT_" << f.name << "<psk::Native<float>>(__out, __op, __dom" << names << ");\n";
+  } else {
+    o << "  const __PSGrid *__g[] = {";
+    for (size_t i = 0; i < floats.size(); ++i) o << (i ? ", " : "") << floats[i];
+    o << "};\n  switch (psk::FloatStorage(__g, " << floats.size() << ")) {\n";
+    o << "    case PS_HIP_STORAGE_BF16: __PSReduceMapT_" << f.name << "<psk::BF16>(__out, __op, __dom" << names
+      << "); break;\n";
+    o << "    case PS_HIP_STORAGE_FP16: __PSReduceMapT_" << f.name << "<psk::FP16>(__out, __op, __dom" << names
+      << "); break;\n";
+    o << "    default: __PSReduceMapT_" << f.name << "<psk::Native<float>>(__out, __op, __dom" << names
+      << "); break;\n  }\n";
+  }
+  if (cfg_.trace)
+    o << "  if (__ps_trace) fprintf(__ps_trace, \"Physis: Reduce " << f.name << " finished\\n\");\n";
+  o << "}\n\n";
   return o.str();
 }
 
@@ -1507,7 +1588,23 @@ std::string Translator::RewriteHost(size_t b, size_t e) {
     if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" && tk.s == "PSReduce") {
       size_t close = MatchClose(t_, i + 1);
       auto args = SplitArgs(t_, i + 2, close);
-      if (args.size() != 3) Die("PSReduce with a kernel is not supported by this target");
+      if (args.size() >= 4) {
+        // PSReduce(&v, op, kernel, dom, args...): a reduction kernel.
+        const std::string kn = Text(t_, args[2].first, args[2].second);
+        if (!rkernels_.count(kn)) Die("PSReduce: '" + kn + "' is not a reduction kernel of this file");
+        const Kernel &rk = rkernels_.at(kn);
+        if (args.size() != 4 + rk.all.size())
+          Die("PSReduce: kernel " + kn + " takes " + std::to_string(rk.all.size()) + " arguments after the domain");
+        std::string call = "__PSReduceMap_" + kn + "(";
+        for (size_t q = 0; q < args.size(); ++q) {
+          if (q == 2) continue;
+          call += (q ? ", " : "") + RewriteHost(args[q].first, args[q].second);
+        }
+        add(call + ")");
+        i = close;
+        continue;
+      }
+      if (args.size() != 3) Die("PSReduce takes a result address, an operator and a grid (or a kernel and a domain)");
       std::string g = Text(t_, args[2].first, args[2].second);
       auto it = gridvars_.find(g);
       if (it == gridvars_.end()) Die("PSReduce: unknown grid '" + g + "'");
@@ -1591,6 +1688,31 @@ std::string Translator::Run(const std::vector<std::string> &includes) {
       }
     }
   }
+  // Reduction kernels: third argument of every PSReduce with a domain.
+  for (size_t i = 0; i + 2 < t_.size(); ++i) {
+    if (t_[i].kind != Tok::kIdent || t_[i].s != "PSReduce" || t_[i + 1].s != "(") continue;
+    const size_t close = MatchClose(t_, i + 1);
+    const auto args = SplitArgs(t_, i + 2, close);
+    if (args.size() < 4) continue;
+    const std::string kn = Text(t_, args[2].first, args[2].second);
+    if (rkernels_.count(kn)) continue;
+    auto it = byname.find(kn);
+    if (it == byname.end()) Die("reduction kernel '" + kn + "' is not defined in this file");
+    if (kernels_.count(kn)) Die("kernel " + kn + " is used both as a stencil and as a reduction kernel");
+    const Func &f = *it->second;
+    const std::string ret = f.lparen >= 2 ? t_[f.lparen - 2].s : "";
+    if (ret != "float" && ret != "double")
+      Die("reduction kernel " + kn + " must return float or double (returns '" + ret + "')");
+    Kernel k;
+    k.f = it->second;
+    AnalyzeKernel(k);
+    if (k.emit_count) Die("reduction kernel " + kn + " must return its value, not emit it");
+    // Every grid is read straight from HBM (no staged tile).
+    k.staged.clear();
+    k.rx = k.ry = k.rz = 0;
+    k.wrap = false;
+    rkernels_[kn] = k;
+  }
   // Emit: header, user code with kernels replaced by the generated code
   // (inserted where the first kernel was defined).
   std::ostringstream o;
@@ -1600,17 +1722,20 @@ std::string Translator::Run(const std::vector<std::string> &includes) {
   o << "#include \"physis/physis.h\"\n#include \"physis/hip/generated.h\"\n\n";
   size_t first_kernel = t_.size();
   std::set<size_t> kernel_starts;
-  for (auto &kv : kernels_) {
-    first_kernel = std::min(first_kernel, kv.second.f->begin);
-    kernel_starts.insert(kv.second.f->begin);
-  }
+  for (auto *ks : {&kernels_, &rkernels_})
+    for (auto &kv : *ks) {
+      first_kernel = std::min(first_kernel, kv.second.f->begin);
+      kernel_starts.insert(kv.second.f->begin);
+    }
   std::string generated;
   for (auto &kv : kernels_) generated += EmitPlan(kv.second) + EmitMapAndLaunch(kv.second);
+  for (auto &kv : rkernels_) generated += EmitReduceKernel(kv.second);
   // Host code, skipping kernel definitions (and their prototypes are harmless).
   std::string host_before, host_after;
   size_t pos = 0;
   std::vector<std::pair<size_t, size_t>> skip;
-  for (auto &kv : kernels_) skip.push_back({kv.second.f->begin, kv.second.f->rbrace + 1});
+  for (auto *ks : {&kernels_, &rkernels_})
+    for (auto &kv : *ks) skip.push_back({kv.second.f->begin, kv.second.f->rbrace + 1});
   std::sort(skip.begin(), skip.end());
   std::string *dst = &host_before;
   for (auto &sk : skip) {

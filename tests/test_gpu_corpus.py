@@ -362,3 +362,59 @@ def test_point_form_map_fuses_two_sweeps_bitwise(tmp_path, shape, iters):
     got = np.fromfile(dst, np.float32).reshape(2, nz, ny, nx)
     want_p, want_q = point_relax_oracle(p, q, a, b, c, 0.7, iters)
     assert np.array_equal(got[0], want_p) and np.array_equal(got[1], want_q)
+
+
+def reduce_kernel_oracle(f, bf16):
+    """Each reduction of tests/golden/programs/reduce_kernel.c restated with
+    numpy: the kernel's value at every point of its domain, evaluated in the
+    kernel's own order (FP32 for the float kernels, FP64 for the double
+    ones; the float grid rounded to BF16 first when it is stored so), then
+    folded in FP64 and converted to the kernel's type."""
+    import numpy as np
+    g = bf16_round(f) if bf16 else f.astype(np.float32)
+    h = 0.5 * f.astype(np.float64) + 0.25
+    d = g[:, :, 1:] - g[:, :, :-1]
+    sq = (np.float32(0.75) * d) * d
+    cv = g - np.float32(0.5) * (np.roll(g, 1, axis=1) + np.roll(g, -1, axis=1))
+    cv = np.abs(cv[1:-1])
+    out = {"sqdiff_sum": np.float32(np.sum(sq, dtype=np.float64)),
+           "curv_max": np.float32(cv.max()), "curv_min": np.float32(cv.min()),
+           "dsq_sum": float(np.sum(h * h + g.astype(np.float64))),
+           "dprod_prod": float(np.prod(1.0 + 1e-3 * h[0:2, 0:2, :])),
+           "row2_sum": np.float32(np.sum(np.float32(0.25) * f[0].astype(np.float32), dtype=np.float64)),
+           "empty_sum": np.float32(0.0), "empty_max": np.float32(-np.inf)}
+    return out
+
+
+@pytest.mark.parametrize("shape,bf16", [((67, 33, 21), 0), ((300, 17, 9), 1), ((1030, 5, 4), 0),
+                                        ((16, 300, 200), 0), ((5, 4, 3), 0)])
+def test_kernel_reductions_match_oracle(tmp_path, shape, bf16):
+    """PSReduce(&v, op, kernel, dom, args...) lowered by physisc-hip (the
+    interface the reference documents but does not implement): float SUM
+    (an x difference, squared and weighted), MAX and MIN (a periodic y
+    curvature over a z-interior), double SUM and PROD (a double grid mixed
+    with the float one), a 2-D kernel, and empty domains (the operator's
+    identity).  MAX and MIN equal the oracle exactly; the FP64-folded sums
+    and products agree to FP64 rounding (the float results to one FP32
+    ulp).  Rows longer than one workgroup item (1030 points) and more items
+    than workgroups (60000 rows) are covered."""
+    import numpy as np
+    from oracle import oracle as O
+    nx, ny, nz = shape
+    f = O.uniform_field(shape, seed=60 + nx)
+    src = tmp_path / "in.bin"
+    f.astype(np.float32).tofile(src)
+    print("reduce_kernel", shape, "bf16", bf16, flush=True)
+    out = subprocess.run([exe("reduce_kernel"), str(nx), str(ny), str(nz), str(src), str(bf16)],
+                         capture_output=True, timeout=120)
+    assert out.returncode == 0, out.stderr.decode()[-2000:]
+    got = dict(l.split() for l in out.stdout.decode().splitlines())
+    want = reduce_kernel_oracle(f, bf16)
+    assert set(got) == set(want)
+    for name in ("curv_max", "curv_min", "empty_sum", "empty_max"):
+        assert np.float32(float(got[name])) == want[name], (name, got[name], want[name])
+    for name in ("sqdiff_sum", "row2_sum"):
+        g32, w32 = np.float32(float(got[name])), want[name]
+        assert abs(float(g32) - float(w32)) <= 2.0 ** -23 * abs(float(w32)), (name, got[name], w32)
+    for name in ("dsq_sum", "dprod_prod"):
+        assert abs(float(got[name]) - want[name]) <= 1e-13 * abs(want[name]), (name, got[name], want[name])

@@ -276,3 +276,51 @@ def test_kernels_outside_the_point_form_run_unfused(tmp_path, bidx, emit):
     grid read (update() has no coordinates), keeps the per-sweep path."""
     hip = translate_text(tmp_path, POINT_KERNEL % {"bidx": bidx, "emit": emit})
     assert "update(" not in hip and "__PSPointFusable_" not in hip
+
+
+REDUCE_KERNEL = """
+#include "physis/physis.h"
+%(ret)s energy(const int x, const int y, const int z, PSGrid3DFloat g, PSGrid3DDouble h, float w) {
+  float d = PSGridGet(g, x + 1, y, z) - PSGridGet(g, x, y, z);
+  %(body)s
+}
+int main(int argc, char *argv[]) {
+  PSInit(&argc, &argv, 3, 16, 16, 16);
+  PSGrid3DFloat g = PSGrid3DFloatNew(16, 16, 16);
+  PSGrid3DDouble h = PSGrid3DDoubleNew(16, 16, 16);
+  %(ret)s v;
+  PSReduce(&v, PS_SUM, energy, PSDomain3DNew(0, 15, 0, 16, 0, 16), g, h, 0.5f);
+  PSFinalize();
+  return 0;
+}
+"""
+
+
+def test_reduction_kernel_lowered_to_a_value_plan(tmp_path):
+    """PSReduce(&v, op, kernel, dom, args...) becomes __PSReduceMap_<k>(&v,
+    op, dom, args...); the kernel becomes a plan whose value() is the body
+    with its grid reads lowered and its return kept (no staged tile), run
+    through psk::RunReduceMap with float grids dispatched by storage."""
+    hip = translate_text(tmp_path, REDUCE_KERNEL % {"ret": "float",
+                                                     "body": "return w * d * d + (float)PSGridGet(h, x, y, z);"})
+    assert "__PSReduceMap_energy(& v, PS_SUM, PSDomain3DNew(0, 15, 0, 16, 0, 16), g, h, 0.5f);" in hip
+    val = hip[hip.find("static float value("):hip.find("\n  }\n", hip.find("static float value("))]
+    assert "return w * d * d + (float)psk::Get3<__kU>(h, __ref, x, y, z);" in val and "__ctx" not in val
+    assert "psk::GridRef<psk::Native<double>> h;" in hip and "psk::GridRef<SF> g;" in hip
+    assert "psk::RunReduceMap<__PSRPlan_energy<SF>>(__a, psk::BoxFromDomain(__dom, 3), __op, &__r" in hip
+    assert "case PS_HIP_STORAGE_BF16: __PSReduceMapT_energy<psk::BF16>" in hip
+    # The kernel's definition does not reach the host code.
+    assert "energy(const int x" not in hip
+
+
+@pytest.mark.parametrize("ret,body,msg", [
+    ("int", "return (int)d;", "must return float or double"),
+    ("float", "PSGridEmit(g, d); return d;", "must return its value"),
+])
+def test_reduction_kernels_outside_the_interface_are_rejected(tmp_path, ret, body, msg):
+    if not os.path.exists(PHYSISC):
+        pytest.skip("physisc-hip is not built (make -C physis_amd/translator)")
+    src = tmp_path / "r.c"
+    src.write_text(REDUCE_KERNEL % {"ret": ret, "body": body})
+    out = subprocess.run([PHYSISC, "-o", str(tmp_path / "r.hip"), str(src)], capture_output=True, text=True)
+    assert out.returncode != 0 and msg in out.stderr
