@@ -327,7 +327,30 @@ def reduce_and_copy_bench(rt, n, reps=10):
     out["copyout_gbs_pageable"] = round(nbytes / cout / 1e9, 2)
     del host, back
     g.free()
+    out["kernel_reduce_sum"] = kernel_reduce_bench(n)
     return out
+
+
+def kernel_reduce_bench(n, reps=50):
+    """A kernel reduction through the drop-in path: tests/golden/programs/
+    reduce_timing.c translated by physisc-hip, PSReduce(&v, PS_SUM, sqdiff,
+    dom, g) with sqdiff = (g(x+1) - g(x))^2 over (n-1) x n x n points of an
+    n^3 FP32 grid, timed by the program over `reps` calls (each returns its
+    value to the host).  Bytes: one read of the grid per call."""
+    exe = os.path.join(ROOT, "tests", "_corpus_bin", "reduce_timing")
+    if not os.path.exists(exe):
+        return None
+    try:
+        out = subprocess.run([exe, str(n), str(reps)], capture_output=True, text=True, timeout=120,
+                             check=True).stdout
+    except (subprocess.SubprocessError, OSError) as e:
+        print("kernel reduction timing failed: %s" % e, file=sys.stderr)
+        return None
+    vals = dict(l.split() for l in out.splitlines() if l.strip())
+    ms = float(vals["reduce_ms"])
+    return {"workload": "PSReduce(&v, PS_SUM, sqdiff, dom, g), (g(x+1)-g(x))^2 over %dx%dx%d of a %d^3 FP32 grid, "
+                        "tests/golden/programs/reduce_timing.c" % (n - 1, n, n, n),
+            "us_per_call_wall": round(ms * 1e3, 1), "gbs_wall": round(4 * n ** 3 / (ms / 1e3) / 1e9, 1)}
 
 
 def diffusion_leg(rt, n, steps, warmup):

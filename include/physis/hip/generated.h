@@ -307,14 +307,14 @@ inline void RunPingPong(typename P::Args ab, typename P::Args ba, int iter, int 
 // (docs/programming.md "Reduction" of the reference; its translators abort on
 // it, reference_translator.cc:1357-1360.)  RP::value(a, x, y, z) is the
 // reduction kernel's body evaluated at a point of the domain.  The domain is
-// cut into items of up to kReduceMapRow consecutive x points of one row;
-// workgroup b takes items b, b + nb, ... in order, each thread folding the
-// values of its points in FP64 (exact for MAX and MIN), and writes one
-// partial; the runtime then folds the partials in index order.  The split
-// depends only on the domain's extents, so a domain reduces to the same bits
-// on every call.
+// cut into items of M x 256 consecutive x points of one row (M = 1, 2 or 4,
+// the fewest that cover the row's width up to 1024); workgroup b takes items
+// b, b + nb, ... in order, 4 / M at a time, each thread folding the values of
+// its points in FP64 (exact for MAX and MIN), and writes one partial; the
+// runtime then folds the partials in index order.  The split depends only on
+// the domain's extents, so a domain reduces to the same bits on every call.
 constexpr int kReduceMapThreads = 256;
-constexpr int kReduceMapRow = 4 * kReduceMapThreads;
+constexpr int kReduceMapPoints = 4;  // points per thread in flight
 constexpr int kReduceMapMaxGroups = 16384;
 
 template <class Op> struct ReduceIdentity;
@@ -323,26 +323,64 @@ template <> struct ReduceIdentity<OpProd> { static constexpr double v = 1.0; };
 template <> struct ReduceIdentity<OpMax> { static constexpr double v = -__builtin_inf(); };
 template <> struct ReduceIdentity<OpMin> { static constexpr double v = __builtin_inf(); };
 
-template <class RP, class Op>
+// I: the item index type (32 bits when the domain has fewer than 2^31
+// items, which spares 64-bit divisions).
+template <class RP, class Op, class I, int M>
 __global__ void __launch_bounds__(kReduceMapThreads)
 ReduceMapKernel(typename RP::Args a, Box box, int row_items, int64_t items, double *partials) {
+  constexpr int U = kReduceMapPoints / M;
   __shared__ double lds[kReduceMapThreads / 64];
   Op op;
-  const int w1 = box.hi[1] - box.lo[1];
+  const I w1 = (I)(box.hi[1] - box.lo[1]), ri = (I)row_items, n = (I)items, step = (I)gridDim.x;
   double acc = ReduceIdentity<Op>::v;
-  for (int64_t j = blockIdx.x; j < items; j += gridDim.x) {
-    const int64_t row = j / row_items;
-    const int xc = (int)(j - row * row_items);
-    const int y = box.lo[1] + (int)(row % w1), z = box.lo[2] + (int)(row / w1);
-    const int x0 = box.lo[0] + xc * kReduceMapRow + (int)threadIdx.x;
+  for (I j0 = (I)blockIdx.x; j0 < n; j0 += U * step) {
+    // Items past the end, and lanes past a row's end, evaluate a point of
+    // the domain (the last item, the row's last point) and drop the value:
+    // no branch around the loads, so all the points' loads are in flight
+    // together.
+    double v[U][M];
+    bool ok[U][M];
 #pragma unroll
-    for (int m = 0; m < kReduceMapRow / kReduceMapThreads; ++m) {
-      const int x = x0 + m * kReduceMapThreads;
-      if (x < box.hi[0]) acc = op(acc, (double)RP::value(a, x, y, z));
+    for (int u = 0; u < U; ++u) {
+      const I j = j0 + (I)u * step;
+      const I jj = j < n ? j : n - 1;
+      const I row = jj / ri;
+      const int xc = (int)(jj - row * ri);
+      const int y = box.lo[1] + (int)(row % w1), z = box.lo[2] + (int)(row / w1);
+      const int x0 = box.lo[0] + xc * (M * kReduceMapThreads) + (int)threadIdx.x;
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const int x = x0 + m * kReduceMapThreads;
+        v[u][m] = (double)RP::value(a, min(x, box.hi[0] - 1), y, z);
+        ok[u][m] = j < n && x < box.hi[0];
+      }
     }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int m = 0; m < M; ++m)
+        if (ok[u][m]) acc = op(acc, v[u][m]);
   }
   const double r = block_reduce<kReduceMapThreads>(acc, op, lds, (int)threadIdx.x);
   if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+template <class RP, class Op, int M>
+inline void LaunchReduceMap(const typename RP::Args &a, const Box &box, int row_items, int64_t items, int nb,
+                            double *partials, hipStream_t s) {
+  const dim3 grid(nb), block(kReduceMapThreads);
+  if (items < ((int64_t)1 << 31))
+    hipLaunchKernelGGL((ReduceMapKernel<RP, Op, uint32_t, M>), grid, block, 0, s, a, box, row_items, items, partials);
+  else
+    hipLaunchKernelGGL((ReduceMapKernel<RP, Op, int64_t, M>), grid, block, 0, s, a, box, row_items, items, partials);
+}
+
+template <class RP, class Op>
+inline void LaunchReduceMapOp(int M, const typename RP::Args &a, const Box &box, int row_items, int64_t items,
+                              int nb, double *partials, hipStream_t s) {
+  if (M == 1) LaunchReduceMap<RP, Op, 1>(a, box, row_items, items, nb, partials, s);
+  else if (M == 2) LaunchReduceMap<RP, Op, 2>(a, box, row_items, items, nb, partials, s);
+  else LaunchReduceMap<RP, Op, 4>(a, box, row_items, items, nb, partials, s);
 }
 
 // Reduces RP's values over the box with op; the FP64 result goes to *out
@@ -350,23 +388,23 @@ ReduceMapKernel(typename RP::Args a, Box box, int row_items, int64_t items, doub
 template <class RP>
 inline hipError_t RunReduceMap(const typename RP::Args &a, const Box &box, enum PSReduceOp op, double *out,
                                hipStream_t s) {
+  if (op < PS_MAX || op > PS_PROD) return hipErrorInvalidValue;
   const int64_t w0 = box.hi[0] - box.lo[0], w1 = box.hi[1] - box.lo[1], w2 = box.hi[2] - box.lo[2];
   if (w0 <= 0 || w1 <= 0 || w2 <= 0) {
     *out = op == PS_SUM ? 0.0 : op == PS_PROD ? 1.0 : op == PS_MAX ? -__builtin_inf() : __builtin_inf();
-    return op >= PS_MAX && op <= PS_PROD ? hipSuccess : hipErrorInvalidValue;
+    return hipSuccess;
   }
-  const int row_items = (int)((w0 + kReduceMapRow - 1) / kReduceMapRow);
+  const int M = w0 <= kReduceMapThreads ? 1 : w0 <= 2 * kReduceMapThreads ? 2 : 4;
+  const int row_items = (int)((w0 + M * kReduceMapThreads - 1) / (M * kReduceMapThreads));
   const int64_t items = (int64_t)row_items * w1 * w2;
   const int nb = (int)std::min<int64_t>(items, kReduceMapMaxGroups);
   double *partials = __PSHipReduceMapPartials(nb);
   if (!partials) return hipErrorOutOfMemory;
-  const dim3 grid(nb), block(kReduceMapThreads);
   switch (op) {
-    case PS_SUM: hipLaunchKernelGGL((ReduceMapKernel<RP, OpSum>), grid, block, 0, s, a, box, row_items, items, partials); break;
-    case PS_PROD: hipLaunchKernelGGL((ReduceMapKernel<RP, OpProd>), grid, block, 0, s, a, box, row_items, items, partials); break;
-    case PS_MAX: hipLaunchKernelGGL((ReduceMapKernel<RP, OpMax>), grid, block, 0, s, a, box, row_items, items, partials); break;
-    case PS_MIN: hipLaunchKernelGGL((ReduceMapKernel<RP, OpMin>), grid, block, 0, s, a, box, row_items, items, partials); break;
-    default: return hipErrorInvalidValue;
+    case PS_SUM: LaunchReduceMapOp<RP, OpSum>(M, a, box, row_items, items, nb, partials, s); break;
+    case PS_PROD: LaunchReduceMapOp<RP, OpProd>(M, a, box, row_items, items, nb, partials, s); break;
+    case PS_MAX: LaunchReduceMapOp<RP, OpMax>(M, a, box, row_items, items, nb, partials, s); break;
+    default: LaunchReduceMapOp<RP, OpMin>(M, a, box, row_items, items, nb, partials, s); break;
   }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

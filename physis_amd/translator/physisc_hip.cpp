@@ -1245,7 +1245,7 @@ std::string Translator::EmitReduceKernel(const Kernel &k) {
   std::ostringstream o;
   o << "// Reduction kernel " << f.name << " (rank " << k.rank << ", value " << T << ")\n";
   o << "template <class SF, bool __kU = false>\nstruct " << P << " {\n";
-  o << "  struct Args {\n";
+  o << "  struct Args {\n    using SFT = SF;\n";
   for (const Param &p : k.all) {
     if (p.is_grid && !p.gt.utype.empty()) o << "    struct __PSURef_" << p.gt.utype << " " << p.name << ";\n";
     else if (p.is_grid) o << "    psk::GridRef<" << StorageOf(p.gt) << "> " << p.name << ";\n";
@@ -1262,9 +1262,9 @@ std::string Translator::EmitReduceKernel(const Kernel &k) {
     decl += std::string(", ") + (p.is_grid ? "__PSGrid *" : p.type + " ") + p.name;
     names += ", " + p.name;
   }
-  o << "template <class SF>\nstatic void __PSReduceMapT_" << f.name << "(" << T
-    << " *__out, enum PSReduceOp __op, __PSDomain __dom" << decl << ") {\n";
-  o << "  typename " << P << "<SF>::Args __a;\n";
+  // The plan's Args for either instantiation (__kU or not).
+  o << "template <class __A>\nstatic void __PSRFill_" << f.name << "(__A &__a" << decl << ") {\n";
+  o << "  using SF = typename __A::SFT;\n";
   for (const Param &p : k.all) {
     if (p.is_grid && !p.gt.utype.empty())
       o << "  __a." << p.name << " = __PSUMakeRef_" << p.gt.utype << "(" << p.name << ");\n";
@@ -1274,9 +1274,26 @@ std::string Translator::EmitReduceKernel(const Kernel &k) {
     else
       o << "  __a." << p.name << " = " << p.name << ";\n";
   }
-  o << "  double __r = 0.0;\n";
-  o << "  psk::CheckLaunch(psk::RunReduceMap<" << P << "<SF>>(__a, psk::BoxFromDomain(__dom, " << k.rank
-    << "), __op, &__r, (hipStream_t)__PSGetStream()), \"" << f.name << "\");\n";
+  o << "  (void)sizeof(SF);\n}\n\n";
+  o << "template <class SF>\nstatic void __PSReduceMapT_" << f.name << "(" << T
+    << " *__out, enum PSReduceOp __op, __PSDomain __dom" << decl << ") {\n";
+  o << "  const psk::Box __box = psk::BoxFromDomain(__dom, " << k.rank << ");\n";
+  o << "  double __r = 0.0;\n  hipError_t __e;\n";
+  o << "  typename " << P << "<SF>::Args __a;\n  __PSRFill_" << f.name << "(__a" << names << ");\n";
+  const std::string uref = UniformRef(k);
+  if (!uref.empty()) {
+    // Every rank-3 grid with the extents of one of them and 32-bit byte
+    // offsets: one element index per point (psk::Get3 with __kU).
+    o << "  if (psk::UniformExtents(__a." << uref;
+    for (const Param &p : k.all)
+      if (p.is_grid && p.gt.rank == 3 && p.gt.utype.empty() && p.name != uref) o << ", __a." << p.name;
+    o << ")) {\n    typename " << P << "<SF, true>::Args __u;\n    __PSRFill_" << f.name << "(__u" << names
+      << ");\n    __e = psk::RunReduceMap<" << P << "<SF, true>>(__u, __box, __op, &__r, (hipStream_t)__PSGetStream());\n"
+      << "  } else {\n  ";
+  }
+  o << "  __e = psk::RunReduceMap<" << P << "<SF>>(__a, __box, __op, &__r, (hipStream_t)__PSGetStream());\n";
+  if (!uref.empty()) o << "  }\n";
+  o << "  psk::CheckLaunch(__e, \"" << f.name << "\");\n";
   o << "  *__out = (" << T << ")__r;\n}\n\n";
   std::vector<std::string> floats;
   for (const Param &p : k.all)

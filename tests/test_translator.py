@@ -307,7 +307,10 @@ def test_reduction_kernel_lowered_to_a_value_plan(tmp_path):
     val = hip[hip.find("static float value("):hip.find("\n  }\n", hip.find("static float value("))]
     assert "return w * d * d + (float)psk::Get3<__kU>(h, __ref, x, y, z);" in val and "__ctx" not in val
     assert "psk::GridRef<psk::Native<double>> h;" in hip and "psk::GridRef<SF> g;" in hip
-    assert "psk::RunReduceMap<__PSRPlan_energy<SF>>(__a, psk::BoxFromDomain(__dom, 3), __op, &__r" in hip
+    assert "psk::RunReduceMap<__PSRPlan_energy<SF>>(__a, __box, __op, &__r" in hip
+    # One element index per point when every grid has the same extents.
+    assert "if (psk::UniformExtents(__a.g, __a.h)) {" in hip
+    assert "psk::RunReduceMap<__PSRPlan_energy<SF, true>>(__u, __box, __op, &__r" in hip
     assert "case PS_HIP_STORAGE_BF16: __PSReduceMapT_energy<psk::BF16>" in hip
     # The kernel's definition does not reach the host code.
     assert "energy(const int x" not in hip
