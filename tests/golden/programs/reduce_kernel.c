@@ -1,15 +1,16 @@
 /*
  * Kernel reductions: PSReduce(&v, op, kernel, dom, args...) with reduction
  * kernels that return float or double (the interface of the reference's
- * docs/programming.md, "Reduction"), on 3-D float and double grids and a
- * 2-D float grid, over interior, whole, small and empty domains.
+ * docs/programming.md, "Reduction"), on 3-D float and double grids, a 2-D
+ * and a 1-D float grid, over interior, whole, small and empty domains.
  *
  *   reduce_kernel NX NY NZ IN BF16
  *
  * reads NX*NY*NZ floats (x fastest) from IN into the float grid g (stored
- * in BF16 when BF16 is 1), fills the double grid h with 0.5 * f + 0.25 and
- * the 2-D grid a with the first plane of f, and prints one "name value"
- * line per reduction.  Built by tests/golden/build_corpus.py.
+ * in BF16 when BF16 is 1), fills the double grid h with 0.5 * f + 0.25,
+ * the 2-D grid a with the first plane of f and the 1-D grid b with its
+ * first row, and prints one "name value" line per reduction.  Built by
+ * tests/golden/build_corpus.py.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -41,6 +42,10 @@ float row2(const int x, const int y, PSGrid2DFloat a, float s) {
   return s * PSGridGet(a, x, y);
 }
 
+float line1(const int x, PSGrid1DFloat b) {
+  return PSGridGet(b, x) - PSGridGetPeriodic(b, x + 1);
+}
+
 int main(int argc, char *argv[]) {
   PSInit(&argc, &argv, 3, 0, 0, 0);
   if (argc != 6) {
@@ -62,11 +67,13 @@ int main(int argc, char *argv[]) {
   PSGrid3DFloat g = PSGrid3DFloatNew(nx, ny, nz, attr);
   PSGrid3DDouble h = PSGrid3DDoubleNew(nx, ny, nz);
   PSGrid2DFloat a = PSGrid2DFloatNew(nx, ny);
+  PSGrid1DFloat b = PSGrid1DFloatNew(nx);
   PSGridCopyin(g, f);
   PSGridCopyin(h, d);
   PSGridCopyin(a, f);
+  PSGridCopyin(b, f);
 
-  float fs, fmax, fmin, r2, es, emax;
+  float fs, fmax, fmin, r2, es, emax, l1;
   double ds, dp;
   PSReduce(&fs, PS_SUM, sqdiff, PSDomain3DNew(0, nx - 1, 0, ny, 0, nz), g, 0.75f);
   PSReduce(&fmax, PS_MAX, curv, PSDomain3DNew(0, nx, 0, ny, 1, nz - 1), g);
@@ -74,6 +81,7 @@ int main(int argc, char *argv[]) {
   PSReduce(&ds, PS_SUM, dsq, PSDomain3DNew(0, nx, 0, ny, 0, nz), h, g);
   PSReduce(&dp, PS_PROD, dprod, PSDomain3DNew(0, nx, 0, 2, 0, 2), h);
   PSReduce(&r2, PS_SUM, row2, PSDomain2DNew(0, nx, 0, ny), a, 0.25f);
+  PSReduce(&l1, PS_MIN, line1, PSDomain1DNew(0, nx), b);
   PSReduce(&es, PS_SUM, sqdiff, PSDomain3DNew(0, 0, 0, ny, 0, nz), g, 1.0f);
   PSReduce(&emax, PS_MAX, curv, PSDomain3DNew(0, nx, 2, 2, 0, nz), g);
   printf("sqdiff_sum %.9g\n", (double)fs);
@@ -82,11 +90,13 @@ int main(int argc, char *argv[]) {
   printf("dsq_sum %.17g\n", ds);
   printf("dprod_prod %.17g\n", dp);
   printf("row2_sum %.9g\n", (double)r2);
+  printf("line1_min %.9g\n", (double)l1);
   printf("empty_sum %.9g\n", (double)es);
   printf("empty_max %.9g\n", (double)emax);
   PSGridFree(g);
   PSGridFree(h);
   PSGridFree(a);
+  PSGridFree(b);
   free(f);
   free(d);
   PSFinalize();

@@ -382,6 +382,7 @@ def reduce_kernel_oracle(f, bf16):
            "dsq_sum": float(np.sum(h * h + g.astype(np.float64))),
            "dprod_prod": float(np.prod(1.0 + 1e-3 * h[0:2, 0:2, :])),
            "row2_sum": np.float32(np.sum(np.float32(0.25) * f[0].astype(np.float32), dtype=np.float64)),
+           "line1_min": np.float32((f[0, 0] - np.roll(f[0, 0], -1)).min()),
            "empty_sum": np.float32(0.0), "empty_max": np.float32(-np.inf)}
     return out
 
@@ -393,8 +394,8 @@ def test_kernel_reductions_match_oracle(tmp_path, shape, bf16):
     interface the reference documents but does not implement): float SUM
     (an x difference, squared and weighted), MAX and MIN (a periodic y
     curvature over a z-interior), double SUM and PROD (a double grid mixed
-    with the float one), a 2-D kernel, and empty domains (the operator's
-    identity).  MAX and MIN equal the oracle exactly; the FP64-folded sums
+    with the float one), a 2-D kernel, a 1-D one with a periodic read, and
+    empty domains (the operator's identity).  MAX and MIN equal the oracle exactly; the FP64-folded sums
     and products agree to FP64 rounding (the float results to one FP32
     ulp).  Rows longer than one workgroup item (1030 points) and more items
     than workgroups (60000 rows) are covered."""
@@ -411,7 +412,7 @@ def test_kernel_reductions_match_oracle(tmp_path, shape, bf16):
     got = dict(l.split() for l in out.stdout.decode().splitlines())
     want = reduce_kernel_oracle(f, bf16)
     assert set(got) == set(want)
-    for name in ("curv_max", "curv_min", "empty_sum", "empty_max"):
+    for name in ("curv_max", "curv_min", "line1_min", "empty_sum", "empty_max"):
         assert np.float32(float(got[name])) == want[name], (name, got[name], want[name])
     for name in ("sqdiff_sum", "row2_sum"):
         g32, w32 = np.float32(float(got[name])), want[name]
