@@ -419,3 +419,25 @@ def test_kernel_reductions_match_oracle(tmp_path, shape, bf16):
         assert abs(float(g32) - float(w32)) <= 2.0 ** -23 * abs(float(w32)), (name, got[name], w32)
     for name in ("dsq_sum", "dprod_prod"):
         assert abs(float(got[name]) - want[name]) <= 1e-13 * abs(want[name]), (name, got[name], want[name])
+
+
+@pytest.mark.parametrize("n", [9, 70])
+def test_kernel_reduction_over_a_point_type_grid(n):
+    """A reduction kernel reading members of a user-defined point type grid
+    (tests/golden/programs/reduce_point_type.c): the SUM of x(p)^2 +
+    y(p + e_x) * z(p) agrees with the FP32 values folded in FP64 to one FP32
+    ulp, the MAX of x - y exactly."""
+    import numpy as np
+    print("reduce_point_type", n, flush=True)
+    out = subprocess.run([exe("reduce_point_type"), str(n)], capture_output=True, timeout=120)
+    assert out.returncode == 0, out.stderr.decode()[-2000:]
+    got = dict(l.split() for l in out.stdout.decode().splitlines())
+    i = np.arange(n ** 3, dtype=np.float32).reshape(n, n, n)
+    px = np.float32(0.01) * i
+    py = np.float32(1.0) - np.float32(0.001) * i
+    pz = np.full_like(px, np.float32(0.5))
+    vals = px[:, :, :-1] * px[:, :, :-1] + py[:, :, 1:] * pz[:, :, :-1]
+    want_s = np.float32(np.sum(vals, dtype=np.float64))
+    want_m = np.float32((px - py).max())
+    assert abs(float(got["norm2_sum"]) - float(want_s)) <= 2.0 ** -23 * abs(float(want_s))
+    assert np.float32(float(got["spread_max"])) == want_m
