@@ -87,8 +87,11 @@ extern PSStencil __PSStencilMapBlack(void *kernel, ...);
 #define PSStencilMapBlack(k, ...) __PSStencilMapBlack((void *)(k), __VA_ARGS__)
 extern void PSStencilRun(PSStencil s, ...);
 
-/* PSReduce(&result, op, grid) */
+/* PSReduce(&result, op, grid), or PSReduce(&result, op, kernel, domain,
+ * args...) with a reduction kernel; PSGridReduce is accepted as another name
+ * for it. */
 extern void PSReduce(void *result, ...);
+extern void PSGridReduce(void *result, ...);
 
 #ifdef __cplusplus
 }

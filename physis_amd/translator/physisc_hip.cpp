@@ -17,6 +17,7 @@
 //                                      defaulting to 1 (reference_translator.cc:1249-1283)
 //   PSReduce(&v, op, g)              -> __PSReduceGridFloat/Double(&v, op, g)
 //                                      (reference_translator.cc:1315-1351)
+//   PSGridReduce(...)                -> the same as PSReduce(...) (an alias)
 //   PSReduce(&v, op, k, dom, args..) -> __PSReduceMap_k(&v, op, dom, args..):
 //                                      the reduction kernel k (returns float or
 //                                      double) evaluated at every point of dom
@@ -1602,7 +1603,7 @@ std::string Translator::RewriteHost(size_t b, size_t e) {
       i = close;
       continue;
     }
-    if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" && tk.s == "PSReduce") {
+    if (tk.kind == Tok::kIdent && i + 1 < e && t_[i + 1].s == "(" && (tk.s == "PSReduce" || tk.s == "PSGridReduce")) {
       size_t close = MatchClose(t_, i + 1);
       auto args = SplitArgs(t_, i + 2, close);
       if (args.size() >= 4) {
@@ -1707,7 +1708,8 @@ std::string Translator::Run(const std::vector<std::string> &includes) {
   }
   // Reduction kernels: third argument of every PSReduce with a domain.
   for (size_t i = 0; i + 2 < t_.size(); ++i) {
-    if (t_[i].kind != Tok::kIdent || t_[i].s != "PSReduce" || t_[i + 1].s != "(") continue;
+    if (t_[i].kind != Tok::kIdent || (t_[i].s != "PSReduce" && t_[i].s != "PSGridReduce") || t_[i + 1].s != "(")
+      continue;
     const size_t close = MatchClose(t_, i + 1);
     const auto args = SplitArgs(t_, i + 2, close);
     if (args.size() < 4) continue;

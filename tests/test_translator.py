@@ -327,3 +327,13 @@ def test_reduction_kernels_outside_the_interface_are_rejected(tmp_path, ret, bod
     src.write_text(REDUCE_KERNEL % {"ret": ret, "body": body})
     out = subprocess.run([PHYSISC, "-o", str(tmp_path / "r.hip"), str(src)], capture_output=True, text=True)
     assert out.returncode != 0 and msg in out.stderr
+
+
+def test_psgridreduce_is_an_alias_of_psreduce(tmp_path):
+    """PSGridReduce(&v, op, g) and PSGridReduce(&v, op, kernel, dom, ...)
+    lower exactly as PSReduce does."""
+    text = (REDUCE_KERNEL % {"ret": "float", "body": "return d;"}).replace("PSReduce(", "PSGridReduce(")
+    text = text.replace("  PSFinalize();", "  float t;\n  PSGridReduce(&t, PS_MAX, g);\n  PSFinalize();")
+    hip = translate_text(tmp_path, text)
+    assert "__PSReduceMap_energy(& v, PS_SUM, PSDomain3DNew(0, 15, 0, 16, 0, 16), g, h, 0.5f);" in hip
+    assert "__PSReduceGridFloat(& t, PS_MAX, g)" in hip
