@@ -309,12 +309,12 @@ inline void RunPingPong(typename P::Args ab, typename P::Args ba, int iter, int 
 // reduction kernel's body evaluated at a point of the domain.  The domain is
 // cut into items of M x 256 consecutive x points of one row (M = 1, 2 or 4,
 // the fewest that cover the row's width up to 1024); workgroup b takes items
-// b, b + nb, ... in order, 4 / M at a time, each thread folding the values of
+// b, b + nb, ... in order, 8 / M at a time, each thread folding the values of
 // its points in FP64 (exact for MAX and MIN), and writes one partial; the
 // runtime then folds the partials in index order.  The split depends only on
 // the domain's extents, so a domain reduces to the same bits on every call.
 constexpr int kReduceMapThreads = 256;
-constexpr int kReduceMapPoints = 4;  // points per thread in flight
+constexpr int kReduceMapPoints = 8;  // points per thread in flight
 constexpr int kReduceMapMaxGroups = 16384;
 
 template <class Op> struct ReduceIdentity;
